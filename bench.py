@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the network-construction hot path on one MI355X.
+
+Workload (the headline configuration): one subject, N = 45,000 voxels x
+L = 1,200 time points, seeded factor-model series (K = 20 AR(1) latents,
+phi = 0.6), Pearson r > 0.4, weighted CSR.  One step = z-normalise + MFMA
+correlation with the fused threshold + FP64 recheck of band pairs + CSR written
+to device memory, with the raw series already resident in HBM.
+
+value = voxel pairs per second, N(N-1)/2 / step time, split-BF16 (bf16x3).
+The plain FP32 path is timed the same way and reported beside it.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import cuda_newest20160331_amd as bna  # noqa: E402
+
+PEAK = {"bf16x3": 2500.0, "bf16": 2500.0, "fp32": 157.3}  # TFLOP/s, MI355X dense
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_path(ctx, series, N, L, prec, thr, steps, warmup):
+    """Returns (ms per step by host clock, mean corr-kernel ms, last graph stats)."""
+    last = None
+    for _ in range(warmup):
+        series.normalize(bna.PRECISIONS[prec])
+        g = series.build_device(thr=thr, precision=prec)
+        last = (g.nnz, g.n_candidates, g.n_flips, g.band, g.max_err)
+        series.free_device(g)
+    ctx._L.bna_synchronize(ctx._h)
+    corr = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        series.normalize(bna.PRECISIONS[prec])
+        g = series.build_device(thr=thr, precision=prec)
+        corr.append(ctx.timings()["corr_ms"])
+        last = (g.nnz, g.n_candidates, g.n_flips, g.band, g.max_err)
+        series.free_device(g)
+    ctx._L.bna_synchronize(ctx._h)
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, float(np.mean(corr)), last
+
+
+def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float):
+    """The reference CPU path (CorMat.cpp:954-965 arithmetic, CorMat_CPU.cpp
+    j-striping) from the oracle restatement, on a bounded sample of rows."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import orc
+
+    threads = min(16, os.cpu_count() or 1)
+    _, z32, _, _, _ = orc.znorm(x, 0)
+    rows = 64
+    t0 = time.perf_counter()
+    orc.ref_cpu_rows(z32, 0, rows, thr, threads)
+    dt = time.perf_counter() - t0
+    pairs = sum(N - 1 - i for i in range(rows))
+    rate = pairs / dt
+    # grow the sample to about target_s seconds of CPU work
+    want = rate * target_s
+    rows2, acc = 0, 0
+    while acc < want and rows2 < N - 1:
+        acc += N - 1 - (rows + rows2)
+        rows2 += 1
+    t0 = time.perf_counter()
+    orc.ref_cpu_rows(z32, rows, rows + rows2, thr, threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": acc / dt,
+        "unit": "voxel-pairs/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"rows {rows}..{rows + rows2 - 1} of N={N}, L={L}: {acc} pairs, "
+                  f"r>{thr} count, {dt:.1f} s on {threads} threads",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=45000)
+    ap.add_argument("--l", type=int, default=1200)
+    ap.add_argument("--thr", type=float, default=0.4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
+        return 2
+
+    N, L, thr = a.n, a.l, a.thr
+    seed = 0xB17A0000 + 3000
+    log(f"bench: generating N={N} L={L} seed={seed:#x}")
+    x = bna.synth_factor_model(N, L, 20, 0.6, seed)
+    M1 = N * (N - 1) // 2
+    flops = float(N) * (N - 1) * L  # 2L per pair
+
+    res = {}
+    with bna.Context(0) as ctx, ctx.series(x) as s:
+        for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"])):
+            log(f"bench: {prec} warmup={a.warmup} steps={a.steps}")
+            ms, corr_ms, st = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
+            res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st}
+            log(f"bench: {prec} {ms:.3f} ms/step, corr kernel {corr_ms:.3f} ms, nnz={st[0]} "
+                f"cand={st[1]} flips={st[2]} band={st[3]:.3g} max_err={st[4]:.3g}")
+
+    b = res["bf16x3"]
+    value = M1 / (b["ms"] / 1e3)
+    achieved = flops / (b["corr_ms"] / 1e3) / 1e12
+    line = {
+        "metric": "corr+CSR voxel-pairs/s @N=45k,L=1200 (split-BF16 vs FP32); p99 subject latency",
+        "value": value,
+        "unit": "voxel-pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": b["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3",
+        "data": "synthetic",
+        "config": {"workload": f"one subject, N={N} x L={L} fp32 factor-model series (K=20, phi=0.6), "
+                               f"r>{thr}, weighted CSR, split-BF16 MFMA + FP64 band recheck",
+                   "N": N, "L": L, "thr": thr, "precision": "bf16x3", "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["bf16x3"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["bf16x3"], "traffic": None},
+        "edges": b["stats"][0] // 2,
+        "edges_per_s": (b["stats"][0] // 2) / (b["ms"] / 1e3),
+        "band_pairs_rechecked": b["stats"][1],
+        "max_abs_err_vs_fp64": b["stats"][4],
+        "p99_subject_latency_ms": None,
+    }
+    if "fp32" in res:
+        f = res["fp32"]
+        fval = M1 / (f["ms"] / 1e3)
+        line["fp32_value"] = fval
+        line["speedup_vs_fp32"] = value / fval
+        line["fp32_roofline_frac"] = flops / (f["corr_ms"] / 1e3) / 1e12 / PEAK["fp32"]
+        line["same_csr_as_fp32"] = f["stats"][0] == b["stats"][0]
+    if not a.no_cpu:
+        log("bench: CPU baseline")
+        line["cpu_baseline"] = cpu_baseline(x, N, L, thr, a.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

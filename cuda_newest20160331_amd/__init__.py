@@ -1,0 +1,319 @@
+"""cuda_newest20160331_amd -- MI355X network construction for BNAPlatform.
+
+The product is libbna.so (HIP kernels for gfx950 behind the C ABI declared in
+include/bna.h) and the command-line tools bna_cormat / bna_degree / bna_ec.
+This module is a thin ctypes binding to that library, used by the tests and
+the benchmark; every call goes through the C ABI.  There is no fallback: if the
+library is missing, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libbna.so"
+
+TIME_MAJOR, VOXEL_MAJOR = 0, 1
+FP32, BF16, BF16X3 = 0, 1, 2
+THR_R, THR_DENSITY = 0, 1
+EC_WEIGHTED_REF, EC_BINARY_REF = 0, 1
+PRECISIONS = {"fp32": FP32, "bf16": BF16, "bf16x3": BF16X3}
+
+# every function declared in include/bna.h
+EXPORTS = [
+    "bna_opts_default", "bna_csr_free", "bna_create", "bna_destroy", "bna_synchronize",
+    "bna_status_string", "bna_last_error", "bna_get_timings", "bna_workspace_bytes",
+    "bna_build_csr", "bna_series_create", "bna_series_destroy", "bna_series_normalize",
+    "bna_build_csr_series", "bna_csr_to_host", "bna_series_download", "bna_degree",
+    "bna_eigcent", "bna_density_to_k", "bna_synth_factor_model", "bna_write_csr",
+    "bna_read_csr", "bna_write_nm",
+]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("thr_type", C.c_int), ("thr", C.c_double), ("weighted", C.c_int),
+        ("precision", C.c_int), ("recheck", C.c_int), ("ref_compat_eps", C.c_int),
+        ("tile", C.c_int),
+    ]
+
+
+class CSR(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32), ("nnz", C.c_int64), ("R", C.POINTER(C.c_int32)),
+        ("C", C.POINTER(C.c_int32)), ("V", C.POINTER(C.c_float)), ("on_device", C.c_int),
+        ("thr_used", C.c_double), ("density_pct", C.c_double), ("n_candidates", C.c_int64),
+        ("n_flips", C.c_int64), ("band", C.c_double), ("max_err", C.c_double),
+    ]
+
+
+class Timings(C.Structure):
+    _fields_ = [
+        ("znorm_ms", C.c_double), ("corr_ms", C.c_double), ("recheck_ms", C.c_double),
+        ("csr_ms", C.c_double), ("select_ms", C.c_double), ("total_ms", C.c_double),
+        ("corr_launches", C.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libbna.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m cuda_newest20160331_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, i64, dbl, sz = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_size_t
+    fp = C.POINTER(C.c_float)
+    sig = {
+        "bna_opts_default": (None, [C.POINTER(Opts)]),
+        "bna_csr_free": (None, [C.POINTER(CSR)]),
+        "bna_create": (i32, [C.POINTER(vp), i32, sz]),
+        "bna_destroy": (i32, [vp]),
+        "bna_synchronize": (i32, [vp]),
+        "bna_status_string": (C.c_char_p, [i32]),
+        "bna_last_error": (C.c_char_p, [vp]),
+        "bna_get_timings": (i32, [vp, C.POINTER(Timings)]),
+        "bna_workspace_bytes": (i64, [vp]),
+        "bna_build_csr": (i32, [vp, fp, i32, i32, i32, C.POINTER(Opts), C.POINTER(CSR)]),
+        "bna_series_create": (i32, [vp, vp, i32, i32, i32, i32, C.POINTER(vp)]),
+        "bna_series_destroy": (i32, [vp, vp]),
+        "bna_series_normalize": (i32, [vp, vp, i32]),
+        "bna_build_csr_series": (i32, [vp, vp, C.POINTER(Opts), C.POINTER(CSR)]),
+        "bna_csr_to_host": (i32, [vp, C.POINTER(CSR), C.POINTER(CSR)]),
+        "bna_series_download": (i32, [vp, vp, vp, vp, vp]),
+        "bna_degree": (i32, [vp, C.POINTER(CSR), fp]),
+        "bna_eigcent": (i32, [vp, C.POINTER(CSR), dbl, i32, i32, C.POINTER(C.c_double),
+                              C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "bna_density_to_k": (i64, [i32, C.c_float]),
+        "bna_synth_factor_model": (i32, [fp, i32, i32, i32, dbl, C.c_uint64, i32]),
+        "bna_write_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
+        "bna_read_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
+        "bna_write_nm": (i32, [C.c_char_p, fp, i32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+class BnaError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+def _f32p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def synth_factor_model(N: int, L: int, K: int, phi: float, seed: int,
+                       layout: int = TIME_MAJOR) -> np.ndarray:
+    """Seeded factor-model series (float32); shape (L, N) time-major or (N, L)."""
+    x = np.empty((L, N) if layout == TIME_MAJOR else (N, L), dtype=np.float32)
+    rc = lib().bna_synth_factor_model(_f32p(x), N, L, K, phi, seed, layout)
+    if rc:
+        raise BnaError(rc, "bna_synth_factor_model failed")
+    return x
+
+
+def density_to_k(N: int, spa_percent: float) -> int:
+    return int(lib().bna_density_to_k(N, spa_percent))
+
+
+def make_opts(thr: float = 0.0, *, density: float | None = None, weighted: bool = True,
+              precision: int | str = BF16X3, recheck: bool = True,
+              ref_compat_eps: bool = False) -> Opts:
+    o = Opts()
+    lib().bna_opts_default(C.byref(o))
+    if isinstance(precision, str):
+        precision = PRECISIONS[precision]
+    if density is not None:
+        o.thr_type, o.thr = THR_DENSITY, float(density)
+    else:
+        o.thr_type, o.thr = THR_R, float(thr)
+    o.weighted = int(bool(weighted))
+    o.precision = int(precision)
+    o.recheck = int(bool(recheck))
+    o.ref_compat_eps = int(bool(ref_compat_eps))
+    return o
+
+
+class Graph:
+    """A host CSR copied out of a bna_csr (int64 R for convenience)."""
+
+    def __init__(self, g: CSR):
+        n, nnz = g.N, g.nnz
+        self.N = n
+        self.nnz = nnz
+        self.R = np.ctypeslib.as_array(g.R, shape=(n + 1,)).astype(np.int64).copy()
+        self.C = (np.ctypeslib.as_array(g.C, shape=(nnz,)).copy() if nnz
+                  else np.zeros(0, np.int32))
+        self.V = None
+        if g.V:
+            self.V = (np.ctypeslib.as_array(g.V, shape=(nnz,)).copy() if nnz
+                      else np.zeros(0, np.float32))
+        self.thr_used = g.thr_used
+        self.density_pct = g.density_pct
+        self.n_candidates = g.n_candidates
+        self.n_flips = g.n_flips
+        self.band = g.band
+        self.max_err = g.max_err
+
+
+class Context:
+    """One bna_ctx on a HIP device."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._L = lib()
+        self._h = C.c_void_p()
+        rc = self._L.bna_create(C.byref(self._h), device, hbm_budget)
+        if rc:
+            raise BnaError(rc, self._L.bna_last_error(None).decode())
+
+    def close(self):
+        if self._h:
+            self._L.bna_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            raise BnaError(rc, f"{what}: {self._L.bna_last_error(self._h).decode()}")
+
+    def build_csr(self, x: np.ndarray, layout: int = TIME_MAJOR, **kw) -> Graph:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if layout == TIME_MAJOR:
+            L, N = x.shape
+        else:
+            N, L = x.shape
+        o = make_opts(**kw)
+        g = CSR()
+        self._check(self._L.bna_build_csr(self._h, _f32p(x), N, L, layout, C.byref(o), C.byref(g)),
+                    "bna_build_csr")
+        try:
+            return Graph(g)
+        finally:
+            self._L.bna_csr_free(C.byref(g))
+
+    def timings(self) -> dict:
+        t = Timings()
+        self._check(self._L.bna_get_timings(self._h, C.byref(t)), "bna_get_timings")
+        return {k: getattr(t, k) for k, _ in Timings._fields_}
+
+    # device-resident series
+    def series(self, x: np.ndarray, layout: int = TIME_MAJOR) -> "Series":
+        return Series(self, x, layout)
+
+    def degree(self, g: Graph) -> np.ndarray:
+        h = _host_csr(g)
+        deg = np.empty(g.N, np.float32)
+        self._check(self._L.bna_degree(self._h, C.byref(h[0]), _f32p(deg)), "bna_degree")
+        return deg
+
+    def eigcent(self, g: Graph, tol: float = 1e-6, max_iter: int = 10000,
+                output: int = EC_WEIGHTED_REF, binary: bool = False):
+        h = _host_csr(g, drop_values=binary)
+        ec = np.empty(g.N, np.float64)
+        lam = C.c_double()
+        it = C.c_int()
+        self._check(self._L.bna_eigcent(self._h, C.byref(h[0]), tol, max_iter, output,
+                                        ec.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam),
+                                        C.byref(it)), "bna_eigcent")
+        return ec, lam.value, it.value
+
+
+def _host_csr(g: Graph, drop_values: bool = False):
+    """A bna_csr view of a Graph's arrays (kept alive by the returned tuple)."""
+    R = np.ascontiguousarray(g.R, dtype=np.int32)
+    Cc = np.ascontiguousarray(g.C, dtype=np.int32)
+    V = None if (drop_values or g.V is None) else np.ascontiguousarray(g.V, dtype=np.float32)
+    h = CSR()
+    h.N = g.N
+    h.nnz = g.nnz
+    h.R = R.ctypes.data_as(C.POINTER(C.c_int32))
+    h.C = Cc.ctypes.data_as(C.POINTER(C.c_int32))
+    h.V = V.ctypes.data_as(C.POINTER(C.c_float)) if V is not None else C.POINTER(C.c_float)()
+    h.on_device = 0
+    return (h, R, Cc, V)
+
+
+class Series:
+    """A device-resident series (bna_series)."""
+
+    def __init__(self, ctx: Context, x: np.ndarray, layout: int = TIME_MAJOR):
+        self.ctx = ctx
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if layout == TIME_MAJOR:
+            self.L, self.N = x.shape
+        else:
+            self.N, self.L = x.shape
+        self._h = C.c_void_p()
+        ctx._check(ctx._L.bna_series_create(ctx._h, x.ctypes.data, self.N, self.L, layout, 0,
+                                            C.byref(self._h)), "bna_series_create")
+
+    def close(self):
+        if self._h:
+            self.ctx._L.bna_series_destroy(self.ctx._h, self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def normalize(self, precision: int = BF16X3):
+        self.ctx._check(self.ctx._L.bna_series_normalize(self.ctx._h, self._h, precision),
+                        "bna_series_normalize")
+
+    def download(self):
+        z = np.empty((self.N, self.L), np.float64)
+        mu = np.empty(self.N, np.float64)
+        sd = np.empty(self.N, np.float64)
+        self.ctx._check(self.ctx._L.bna_series_download(self.ctx._h, self._h, z.ctypes.data,
+                                                        mu.ctypes.data, sd.ctypes.data),
+                        "bna_series_download")
+        return z, mu, sd
+
+    def build_device(self, **kw) -> CSR:
+        """Builds and keeps the CSR on the device; returns the raw bna_csr
+        (release with free_device)."""
+        o = make_opts(**kw)
+        g = CSR()
+        self.ctx._check(self.ctx._L.bna_build_csr_series(self.ctx._h, self._h, C.byref(o),
+                                                         C.byref(g)), "bna_build_csr_series")
+        return g
+
+    def to_host(self, g: CSR) -> Graph:
+        h = CSR()
+        self.ctx._check(self.ctx._L.bna_csr_to_host(self.ctx._h, C.byref(g), C.byref(h)),
+                        "bna_csr_to_host")
+        try:
+            return Graph(h)
+        finally:
+            self.ctx._L.bna_csr_free(C.byref(h))
+
+    def free_device(self, g: CSR):
+        self.ctx._L.bna_csr_free(C.byref(g))
+
+    def build(self, **kw) -> Graph:
+        g = self.build_device(**kw)
+        try:
+            return self.to_host(g)
+        finally:
+            self.free_device(g)

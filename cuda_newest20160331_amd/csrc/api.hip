@@ -1,0 +1,779 @@
+// api.hip -- C-ABI implementation: contexts, device workspaces, the
+// construction pipeline (znorm -> MFMA tiles -> FP64 recheck -> CSR) and the
+// graph maps.  Declarations and reference citations are in include/bna.h.
+#include "common.hpp"
+#include "bna.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace bna;
+
+namespace bna {
+// Error band: a bound on |r_mfma - r64| for any pair, see band_for().
+double band_for(int prec, int L_pad);
+}  // namespace bna
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct bna_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;
+    size_t held = 0;  // workspace + series + csr bytes allocated through the context
+    std::string err;
+    bna_timings tm{};
+    hipEvent_t ev[8] = {};
+    std::map<std::string, DevBuf> ws;
+    // tile table cache
+    int tt_nb = -1;
+    int64_t tt_count = 0;
+};
+
+struct bna_series {
+    int N = 0, L = 0, N_pad = 0, L_pad = 0, layout = 0;
+    float *x = nullptr;
+    bool own_x = false;
+    double *z64 = nullptr, *mu = nullptr, *sd = nullptr;
+    uint8_t *valid = nullptr;
+    uint16_t *hi = nullptr, *lo = nullptr;
+    float *z32 = nullptr;
+    int prec_ready = -1;  // precision whose operand panel is present
+    size_t bytes = 0;
+};
+
+static thread_local std::string g_err;
+
+static int fail(bna_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), BNA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+static int dev_alloc(bna_ctx *ctx, void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    if (ctx->budget && ctx->held + bytes > ctx->budget)
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, "HBM budget exceeded: need " + std::to_string(bytes) +
+                                                   " more bytes, holding " + std::to_string(ctx->held));
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    ctx->held += bytes;
+    return BNA_OK;
+}
+
+// Output arrays belong to the returned bna_csr, not to the context.
+static int out_alloc(bna_ctx *ctx, void **p, size_t bytes)
+{
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMalloc (output): ") + hipGetErrorString(e));
+    }
+    return BNA_OK;
+}
+
+static void dev_free(bna_ctx *ctx, void *p, size_t bytes)
+{
+    if (!p) return;
+    (void)hipFree(p);
+    if (bytes == 0) bytes = 256;
+    ctx->held -= std::min(ctx->held, bytes);
+}
+
+// Grow-only named workspace buffer.
+static int ws_get(bna_ctx *ctx, const char *name, size_t bytes, void **out)
+{
+    DevBuf &b = ctx->ws[name];
+    if (b.p && b.bytes >= bytes) {
+        *out = b.p;
+        return BNA_OK;
+    }
+    if (b.p) {
+        dev_free(ctx, b.p, b.bytes);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)256);
+    int rc = dev_alloc(ctx, &b.p, want);
+    if (rc) return rc;
+    b.bytes = want;
+    *out = b.p;
+    return BNA_OK;
+}
+
+template <typename T>
+static int ws_get_t(bna_ctx *ctx, const char *name, size_t count, T **out)
+{
+    void *p = nullptr;
+    int rc = ws_get(ctx, name, count * sizeof(T), &p);
+    *out = (T *)p;
+    return rc;
+}
+
+static inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// ------------------------------------------------------------------ context
+extern "C" {
+
+void bna_opts_default(bna_opts *o)
+{
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->thr_type = BNA_THR_R;
+    o->thr = 0.0;
+    o->weighted = 1;
+    o->precision = BNA_BF16X3;
+    o->recheck = 1;
+}
+
+const char *bna_status_string(int s)
+{
+    switch (s) {
+    case BNA_OK: return "ok";
+    case BNA_ERR_ARG: return "invalid argument";
+    case BNA_ERR_HIP: return "HIP runtime error";
+    case BNA_ERR_NOMEM: return "host out of memory";
+    case BNA_ERR_DEVICE_NOMEM: return "device out of memory or over the HBM budget";
+    case BNA_ERR_IO: return "file I/O error";
+    case BNA_ERR_FORMAT: return "malformed file";
+    case BNA_ERR_OVERFLOW: return "result exceeds the int32 CSR format";
+    case BNA_ERR_NOT_FOUND: return "unknown request";
+    case BNA_ERR_BUSY: return "request not finished";
+    case BNA_ERR_CANCELLED: return "request cancelled";
+    case BNA_ERR_DENSITY: return "density target not reachable";
+    default: return "unknown status";
+    }
+}
+
+const char *bna_last_error(bna_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int bna_create(bna_ctx **out, int device, size_t hbm_budget)
+{
+    if (!out) return fail(nullptr, BNA_ERR_ARG, "bna_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(nullptr, BNA_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(nullptr, BNA_ERR_ARG, "bna_create: bad device ordinal");
+    bna_ctx *ctx = new (std::nothrow) bna_ctx();
+    if (!ctx) return fail(nullptr, BNA_ERR_NOMEM, "bna_create: host allocation failed");
+    ctx->device = device;
+    if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+        std::string m = std::string("bna_create: ") + hipGetErrorString(e);
+        delete ctx;
+        return fail(nullptr, BNA_ERR_HIP, m);
+    }
+    for (auto &ev : ctx->ev) {
+        if ((e = hipEventCreate(&ev)) != hipSuccess) {
+            std::string m = std::string("bna_create: ") + hipGetErrorString(e);
+            bna_destroy(ctx);
+            return fail(nullptr, BNA_ERR_HIP, m);
+        }
+    }
+    if (hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) hbm_budget = (size_t)(0.9 * (double)fr);
+    }
+    ctx->budget = hbm_budget;
+    *out = ctx;
+    return BNA_OK;
+}
+
+int bna_destroy(bna_ctx *ctx)
+{
+    if (!ctx) return BNA_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->ws)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto &ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return BNA_OK;
+}
+
+int bna_synchronize(bna_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_synchronize: ctx is NULL");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BNA_OK;
+}
+
+int bna_get_timings(bna_ctx *ctx, bna_timings *t)
+{
+    if (!ctx || !t) return fail(ctx, BNA_ERR_ARG, "bna_get_timings: NULL argument");
+    *t = ctx->tm;
+    return BNA_OK;
+}
+
+int64_t bna_workspace_bytes(bna_ctx *ctx) { return ctx ? (int64_t)ctx->held : -1; }
+
+// ------------------------------------------------------------------ series
+int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, int x_on_device, bna_series **out)
+{
+    if (!ctx || !x || !out) return fail(ctx, BNA_ERR_ARG, "bna_series_create: NULL argument");
+    *out = nullptr;
+    if (N < 2 || L < 2) return fail(ctx, BNA_ERR_ARG, "bna_series_create: need N >= 2 and L >= 2");
+    if (layout != BNA_TIME_MAJOR && layout != BNA_VOXEL_MAJOR) return fail(ctx, BNA_ERR_ARG, "bad layout");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    bna_series *s = new (std::nothrow) bna_series();
+    if (!s) return fail(ctx, BNA_ERR_NOMEM, "bna_series_create: host allocation failed");
+    s->N = N;
+    s->L = L;
+    s->layout = layout;
+    s->N_pad = round_up(N, TILE);
+    s->L_pad = round_up(L, KPAD);
+    const size_t xb = sizeof(float) * (size_t)N * L;
+    if (x_on_device) {
+        s->x = (float *)x;
+        s->own_x = false;
+    } else {
+        int rc = dev_alloc(ctx, (void **)&s->x, xb);
+        if (rc) {
+            delete s;
+            return rc;
+        }
+        s->own_x = true;
+        s->bytes += xb;
+        hipError_t e = hipMemcpyAsync(s->x, x, xb, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            bna_series_destroy(ctx, s);
+            return fail(ctx, BNA_ERR_HIP, std::string("series upload: ") + hipGetErrorString(e));
+        }
+    }
+    *out = s;
+    return BNA_OK;
+}
+
+static void series_free_norm(bna_ctx *ctx, bna_series *s)
+{
+    const size_t NL = (size_t)s->N * s->L, NP = (size_t)s->N_pad * s->L_pad;
+    dev_free(ctx, s->z64, NL * 8);
+    dev_free(ctx, s->mu, (size_t)s->N * 8);
+    dev_free(ctx, s->sd, (size_t)s->N * 8);
+    dev_free(ctx, s->valid, (size_t)s->N);
+    dev_free(ctx, s->hi, NP * 2);
+    dev_free(ctx, s->lo, NP * 2);
+    dev_free(ctx, s->z32, NP * 4);
+    s->z64 = s->mu = s->sd = nullptr;
+    s->valid = nullptr;
+    s->hi = s->lo = nullptr;
+    s->z32 = nullptr;
+    s->prec_ready = -1;
+}
+
+int bna_series_destroy(bna_ctx *ctx, bna_series *s)
+{
+    if (!s) return BNA_OK;
+    if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_series_destroy: ctx is NULL");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    series_free_norm(ctx, s);
+    if (s->own_x) dev_free(ctx, s->x, sizeof(float) * (size_t)s->N * s->L);
+    delete s;
+    return BNA_OK;
+}
+
+static int series_normalize(bna_ctx *ctx, bna_series *s, int prec)
+{
+    if (prec < BNA_FP32 || prec > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bad precision");
+    const size_t NL = (size_t)s->N * s->L, NP = (size_t)s->N_pad * s->L_pad;
+    int rc;
+    if (!s->z64) {
+        if ((rc = dev_alloc(ctx, (void **)&s->z64, NL * 8))) return rc;
+        if ((rc = dev_alloc(ctx, (void **)&s->mu, (size_t)s->N * 8))) return rc;
+        if ((rc = dev_alloc(ctx, (void **)&s->sd, (size_t)s->N * 8))) return rc;
+        if ((rc = dev_alloc(ctx, (void **)&s->valid, (size_t)s->N))) return rc;
+    }
+    // keep only the operand panel of this precision
+    if (prec == BNA_FP32) {
+        if (s->hi) { dev_free(ctx, s->hi, NP * 2); s->hi = nullptr; }
+        if (s->lo) { dev_free(ctx, s->lo, NP * 2); s->lo = nullptr; }
+        if (!s->z32 && (rc = dev_alloc(ctx, (void **)&s->z32, NP * 4))) return rc;
+    } else {
+        if (s->z32) { dev_free(ctx, s->z32, NP * 4); s->z32 = nullptr; }
+        if (!s->hi && (rc = dev_alloc(ctx, (void **)&s->hi, NP * 2))) return rc;
+        if (prec == BNA_BF16X3) {
+            if (!s->lo && (rc = dev_alloc(ctx, (void **)&s->lo, NP * 2))) return rc;
+        } else if (s->lo) {
+            dev_free(ctx, s->lo, NP * 2);
+            s->lo = nullptr;
+        }
+    }
+    HIPCHK(ctx, launch_znorm(s->x, s->layout, s->N, s->L, s->N_pad, s->L_pad, s->mu, s->sd, s->z64, s->valid,
+                             prec == BNA_FP32 ? nullptr : s->hi, prec == BNA_BF16X3 ? s->lo : nullptr,
+                             prec == BNA_FP32 ? s->z32 : nullptr, ctx->stream));
+    s->prec_ready = prec;
+    return BNA_OK;
+}
+
+int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision)
+{
+    if (!ctx || !s) return fail(ctx, BNA_ERR_ARG, "bna_series_normalize: NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    int rc = series_normalize(ctx, s, precision);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->tm.znorm_ms = ms;
+    return BNA_OK;
+}
+
+int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *mu, double *sd)
+{
+    if (!ctx || !s) return fail(ctx, BNA_ERR_ARG, "bna_series_download: NULL argument");
+    if (!s->z64) return fail(ctx, BNA_ERR_ARG, "bna_series_download: series not normalised");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (z64) HIPCHK(ctx, hipMemcpyAsync(z64, s->z64, (size_t)s->N * s->L * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (mu) HIPCHK(ctx, hipMemcpyAsync(mu, s->mu, (size_t)s->N * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (sd) HIPCHK(ctx, hipMemcpyAsync(sd, s->sd, (size_t)s->N * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BNA_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ tile table
+// Upper-triangle tiles in bands of 8 tile-rows: within a band, tile-columns
+// ascend and the band's rows vary fastest, so a run of consecutive items shares
+// a few A panels (the band) and a few B panels (neighbouring columns).
+static void make_tile_table(int nb, int subj, std::vector<TileItem> &out)
+{
+    const int G = 8;
+    for (int I0 = 0; I0 < nb; I0 += G) {
+        const int I1 = std::min(nb, I0 + G);
+        for (int J = I0; J < nb; J++)
+            for (int I = I0; I < I1 && I <= J; I++) out.push_back(TileItem{subj, I, J, 0});
+    }
+}
+
+static int get_tile_table(bna_ctx *ctx, int nb, TileItem **dev, int64_t *count)
+{
+    if (ctx->tt_nb == nb) {
+        *dev = (TileItem *)ctx->ws["tiles"].p;
+        *count = ctx->tt_count;
+        return BNA_OK;
+    }
+    std::vector<TileItem> t;
+    make_tile_table(nb, 0, t);
+    TileItem *d = nullptr;
+    int rc = ws_get_t(ctx, "tiles", t.size(), &d);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(d, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->tt_nb = nb;
+    ctx->tt_count = (int64_t)t.size();
+    *dev = d;
+    *count = ctx->tt_count;
+    return BNA_OK;
+}
+
+// ------------------------------------------------------------ thresholds
+namespace bna {
+
+// Bound on |r_mfma - r64| per pair.  z64 has unit norm, so sum_l |z_i z_j| <= 1.
+// Representation error of the operands: BF16X3 drops lo*lo and the residual
+// below lo (<= 3.1 * 2^-16), BF16 keeps only hi (<= 2.01 * 2^-8), FP32 rounds
+// z to float (<= 2.01 * 2^-24).  Accumulation: every product is added to an FP32
+// accumulator; with at most one ulp (2^-23 relative) lost per addition the
+// recursive-summation bound gives gamma_n = n u / (1 - n u), n = number of
+// products per pair (3 L_pad for BF16X3, L_pad for BF16, 2 L_pad for FP32,
+// where an FP32 product may round too), times the sum of |terms| (<= 1.02).
+// A 1.25 safety factor and 1e-30 for flushed subnormals are added on top.
+double band_for(int prec, int L_pad)
+{
+    const double u = std::ldexp(1.0, -23);
+    double rep, n;
+    if (prec == BNA_BF16X3) {
+        rep = 3.1 * std::ldexp(1.0, -16);
+        n = 3.0 * L_pad;
+    } else if (prec == BNA_BF16) {
+        rep = 2.01 * std::ldexp(1.0, -8);
+        n = (double)L_pad;
+    } else {
+        rep = 2.01 * std::ldexp(1.0, -24);
+        n = 2.0 * L_pad;
+    }
+    const double gam = n * u / (1.0 - n * u);
+    return 1.25 * (rep + 1.02 * gam) + 1e-30;
+}
+
+}  // namespace bna
+
+static float f_below(double t)  // largest float < t (strictly below)
+{
+    float f = (float)t;
+    while ((double)f >= t) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+static float f_floor(double t)  // largest float <= t
+{
+    float f = (float)t;
+    while ((double)f > t) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+static float f_ceil_above(double t)  // a float >= t, one ulp above the nearest
+{
+    float f = (float)t;
+    while ((double)f < t) f = std::nextafter(f, INFINITY);
+    return std::nextafter(f, INFINITY);
+}
+
+// ------------------------------------------------------------ build
+static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, double thr_eff, bna_csr *out,
+                           bool timing_from_norm)
+{
+    const int N = s->N, nb = s->N_pad / TILE, nb1 = nb + 1;
+    const int prec = o->precision;
+    const bool recheck = o->recheck != 0;
+    const double band = band_for(prec, s->L_pad);
+
+    TileItem *tiles = nullptr;
+    int64_t ntiles = 0;
+    int rc = get_tile_table(ctx, nb, &tiles, &ntiles);
+    if (rc) return rc;
+
+    int32_t *frag_orig, *frag_live;
+    int64_t *frag_off;
+    const size_t nfrag = (size_t)N * nb1;
+    if ((rc = ws_get_t(ctx, "frag_orig", nfrag, &frag_orig))) return rc;
+    if ((rc = ws_get_t(ctx, "frag_live", nfrag, &frag_live))) return rc;
+    if ((rc = ws_get_t(ctx, "frag_off", nfrag, &frag_off))) return rc;
+    unsigned long long *ctr;  // [0] entries, [1] candidates, [2] max err bits, [3] flips
+    if ((rc = ws_get_t(ctx, "counters", 8, &ctr))) return rc;
+    SubjDesc *dsd;
+    if ((rc = ws_get_t(ctx, "subjdesc", 1, &dsd))) return rc;
+
+    // capacities: start from what the workspace already holds
+    int64_t ecap = std::max<int64_t>(1 << 20, (int64_t)(ctx->ws["entries"].bytes / sizeof(Entry)));
+    int64_t ccap = std::max<int64_t>(1 << 16, (int64_t)(ctx->ws["cands"].bytes / sizeof(Candidate)));
+
+    SubjDesc h{};
+    h.a0 = prec == BNA_FP32 ? (const void *)s->z32 : (const void *)s->hi;
+    h.a1 = prec == BNA_BF16X3 ? (const void *)s->lo : nullptr;
+    h.N = N;
+    h.N_pad = s->N_pad;
+    h.L_pad = s->L_pad;
+    h.nb = nb;
+    if (recheck) {
+        h.keep_thr = f_below(thr_eff - band);
+        h.sure_thr = f_ceil_above(thr_eff + band);
+    } else {
+        h.keep_thr = f_floor(thr_eff);
+        h.sure_thr = h.keep_thr;
+    }
+    h.frag_orig = frag_orig;
+    h.frag_live = frag_live;
+    h.frag_off = frag_off;
+    h.entry_count = ctr;
+    h.cand_count = ctr + 1;
+    h.hist = nullptr;
+
+    unsigned long long hc[8];
+    float ms_corr = 0.f;
+    int64_t launches = 0;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        Entry *entries;
+        Candidate *cands;
+        if ((rc = ws_get_t(ctx, "entries", (size_t)ecap, &entries))) return rc;
+        if ((rc = ws_get_t(ctx, "cands", (size_t)ccap, &cands))) return rc;
+        h.entries = entries;
+        h.entry_cap = ecap;
+        h.cands = cands;
+        h.cand_cap = ccap;
+        HIPCHK(ctx, hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(dsd, &h, sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, ntiles, ctx->stream));
+        launches++;
+        HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        ms_corr += ms;
+        const bool e_over = (int64_t)hc[0] > ecap, c_over = (int64_t)hc[1] > ccap;
+        if (!e_over && !c_over) break;
+        if (attempt == 2) return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
+        if (e_over) ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
+        if (c_over) ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
+    }
+    const int64_t ncand = (int64_t)hc[1];
+
+    // FP64 recheck
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    if (ncand > 0)
+        HIPCHK(ctx, launch_recheck(s->z64, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
+                                   ctr + 2, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+
+    // rows, scan, placement
+    int64_t *rowlen, *R64;
+    if ((rc = ws_get_t(ctx, "rowlen", (size_t)N, &rowlen))) return rc;
+    if ((rc = ws_get_t(ctx, "R64", (size_t)N + 1, &R64))) return rc;
+    int32_t *R32 = nullptr;
+    if ((rc = out_alloc(ctx, (void **)&R32, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
+    HIPCHK(ctx, launch_rowlen(frag_live, N, nb, rowlen, ctx->stream));
+    HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, ctx->stream));
+    int64_t nnz = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&nnz, R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (nnz > 0x7fffffffLL) {
+        (void)hipFree(R32);
+        return fail(ctx, BNA_ERR_OVERFLOW, "nnz exceeds int32 (" + std::to_string(nnz) + ")");
+    }
+    int32_t *Cd = nullptr;
+    float *Vd = nullptr;
+    if ((rc = out_alloc(ctx, (void **)&Cd, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) {
+        (void)hipFree(R32);
+        return rc;
+    }
+    if (o->weighted && (rc = out_alloc(ctx, (void **)&Vd, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)))) {
+        (void)hipFree(R32);
+        (void)hipFree(Cd);
+        return rc;
+    }
+    HIPCHK(ctx, launch_place(frag_orig, frag_live, frag_off, h.entries, R64, N, nb, Cd, Vd, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+
+    float ms_re = 0.f, ms_csr = 0.f, ms_tot = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms_re, ctx->ev[4], ctx->ev[5]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[5], ctx->ev[6]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms_tot, timing_from_norm ? ctx->ev[0] : ctx->ev[2], ctx->ev[6]));
+    ctx->tm.corr_ms = ms_corr;
+    ctx->tm.recheck_ms = ms_re;
+    ctx->tm.csr_ms = ms_csr;
+    ctx->tm.total_ms = ms_tot;
+    ctx->tm.corr_launches = launches;
+
+    memset(out, 0, sizeof(*out));
+    out->N = N;
+    out->nnz = nnz;
+    out->R = R32;
+    out->C = Cd;
+    out->V = Vd;
+    out->on_device = 1;
+    out->thr_used = thr_eff;
+    const double M1 = (double)N * (N - 1) / 2.0;
+    out->density_pct = 100.0 * (double)nnz / M1 / 2.0;
+    out->n_candidates = ncand;
+    out->n_flips = (int64_t)hc[3];
+    out->band = recheck ? band : 0.0;
+    double me = 0.0;
+    memcpy(&me, &hc[2], sizeof(double));
+    out->max_err = me;
+    return BNA_OK;
+}
+
+extern "C" {
+
+int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *o, bna_csr *out)
+{
+    if (!ctx || !s || !o || !out) return fail(ctx, BNA_ERR_ARG, "bna_build_csr_series: NULL argument");
+    if (o->precision < BNA_FP32 || o->precision > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bad precision");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    memset(&ctx->tm, 0, sizeof(ctx->tm));
+    bool normed = false;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    if (s->prec_ready != o->precision) {
+        int rc = series_normalize(ctx, s, o->precision);
+        if (rc) return rc;
+        normed = true;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    if (o->thr_type == BNA_THR_R) {
+        double thr = o->thr;
+        if (o->ref_compat_eps) thr -= 1e-6;
+        int rc = build_threshold(ctx, s, o, thr, out, normed);
+        if (rc) return rc;
+    } else {
+        return fail(ctx, BNA_ERR_ARG, "density threshold: not available in this build");
+    }
+    if (normed) {
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->tm.znorm_ms = ms;
+    }
+    return BNA_OK;
+}
+
+void bna_csr_free(bna_csr *g)
+{
+    if (!g) return;
+    if (g->on_device) {
+        if (g->R) (void)hipFree(g->R);
+        if (g->C) (void)hipFree(g->C);
+        if (g->V) (void)hipFree(g->V);
+    } else {
+        free(g->R);
+        free(g->C);
+        free(g->V);
+    }
+    g->R = nullptr;
+    g->C = nullptr;
+    g->V = nullptr;
+    g->nnz = 0;
+}
+
+int bna_csr_to_host(bna_ctx *ctx, const bna_csr *dev, bna_csr *out)
+{
+    if (!ctx || !dev || !out) return fail(ctx, BNA_ERR_ARG, "bna_csr_to_host: NULL argument");
+    *out = *dev;
+    out->on_device = 0;
+    const size_t nz = (size_t)std::max<int64_t>(dev->nnz, 1);
+    out->R = (int32_t *)malloc(sizeof(int32_t) * ((size_t)dev->N + 1));
+    out->C = (int32_t *)malloc(sizeof(int32_t) * nz);
+    out->V = dev->V ? (float *)malloc(sizeof(float) * nz) : nullptr;
+    if (!out->R || !out->C || (dev->V && !out->V)) {
+        free(out->R);
+        free(out->C);
+        free(out->V);
+        out->R = out->C = nullptr;
+        out->V = nullptr;
+        return fail(ctx, BNA_ERR_NOMEM, "bna_csr_to_host: host allocation failed");
+    }
+    if (!dev->on_device) {
+        memcpy(out->R, dev->R, sizeof(int32_t) * ((size_t)dev->N + 1));
+        memcpy(out->C, dev->C, sizeof(int32_t) * (size_t)dev->nnz);
+        if (dev->V) memcpy(out->V, dev->V, sizeof(float) * (size_t)dev->nnz);
+        return BNA_OK;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(out->R, dev->R, sizeof(int32_t) * ((size_t)dev->N + 1), hipMemcpyDeviceToHost, ctx->stream));
+    if (dev->nnz > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(out->C, dev->C, sizeof(int32_t) * (size_t)dev->nnz, hipMemcpyDeviceToHost, ctx->stream));
+        if (dev->V)
+            HIPCHK(ctx, hipMemcpyAsync(out->V, dev->V, sizeof(float) * (size_t)dev->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BNA_OK;
+}
+
+int bna_build_csr(bna_ctx *ctx, const float *x, int N, int L, int layout, const bna_opts *o, bna_csr *out)
+{
+    if (!ctx || !x || !o || !out) return fail(ctx, BNA_ERR_ARG, "bna_build_csr: NULL argument");
+    memset(out, 0, sizeof(*out));
+    bna_series *s = nullptr;
+    int rc = bna_series_create(ctx, x, N, L, layout, 0, &s);
+    if (rc) return rc;
+    bna_csr dev{};
+    rc = bna_build_csr_series(ctx, s, o, &dev);
+    if (rc == BNA_OK) {
+        rc = bna_csr_to_host(ctx, &dev, out);
+        bna_csr_free(&dev);
+    }
+    bna_series_destroy(ctx, s);
+    return rc;
+}
+
+// ------------------------------------------------------------ graph maps
+// Makes a device view of g (uploading host arrays into the workspace).
+static int device_view(bna_ctx *ctx, const bna_csr *g, const int32_t **R, const int32_t **C, const float **V)
+{
+    if (g->on_device) {
+        *R = g->R;
+        *C = g->C;
+        *V = g->V;
+        return BNA_OK;
+    }
+    // host arrays are checked before any kernel indexes with them
+    if (g->R[0] != 0 || (int64_t)g->R[g->N] != g->nnz) return fail(ctx, BNA_ERR_FORMAT, "CSR: R[0] != 0 or R[N] != nnz");
+    for (int32_t k = 0; k < g->N; k++)
+        if (g->R[k + 1] < g->R[k]) return fail(ctx, BNA_ERR_FORMAT, "CSR: R not monotone");
+    for (int64_t k = 0; k < g->nnz; k++)
+        if (g->C[k] < 0 || g->C[k] >= g->N) return fail(ctx, BNA_ERR_FORMAT, "CSR: column out of range");
+    int32_t *r, *c;
+    float *v = nullptr;
+    int rc;
+    const size_t nz = (size_t)std::max<int64_t>(g->nnz, 1);
+    if ((rc = ws_get_t(ctx, "g_R", (size_t)g->N + 1, &r))) return rc;
+    if ((rc = ws_get_t(ctx, "g_C", nz, &c))) return rc;
+    if (g->V && (rc = ws_get_t(ctx, "g_V", nz, &v))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(r, g->R, sizeof(int32_t) * ((size_t)g->N + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (g->nnz > 0) {
+        HIPCHK(ctx, hipMemcpyAsync(c, g->C, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice, ctx->stream));
+        if (g->V) HIPCHK(ctx, hipMemcpyAsync(v, g->V, sizeof(float) * (size_t)g->nnz, hipMemcpyHostToDevice, ctx->stream));
+    }
+    *R = r;
+    *C = c;
+    *V = v;
+    return BNA_OK;
+}
+
+int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg)
+{
+    if (!ctx || !g || !deg) return fail(ctx, BNA_ERR_ARG, "bna_degree: NULL argument");
+    if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_degree: empty graph");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    float *d;
+    if ((rc = ws_get_t(ctx, "deg", (size_t)g->N, &d))) return rc;
+    HIPCHK(ctx, launch_degree(R, V, g->N, d, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(deg, d, sizeof(float) * (size_t)g->N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BNA_OK;
+}
+
+int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec, double *lambda,
+                int *iters)
+{
+    if (!ctx || !g || !ec || !lambda) return fail(ctx, BNA_ERR_ARG, "bna_eigcent: NULL argument");
+    if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_eigcent: empty graph");
+    if (output != BNA_EC_WEIGHTED_REF && output != BNA_EC_BINARY_REF) return fail(ctx, BNA_ERR_ARG, "bad output mode");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    const int N = g->N;
+    double *xa, *xb, *part, *outd;
+    void *stt;
+    if ((rc = ws_get_t(ctx, "ec_xa", (size_t)N, &xa))) return rc;
+    if ((rc = ws_get_t(ctx, "ec_xb", (size_t)N, &xb))) return rc;
+    if ((rc = ws_get_t(ctx, "ec_out", (size_t)N, &outd))) return rc;
+    if ((rc = ws_get_t(ctx, "ec_part", (size_t)3 * ec_nparts(N), &part))) return rc;
+    if ((rc = ws_get(ctx, "ec_state", ec_state_bytes(), &stt))) return rc;
+    int it = 0;
+    HIPCHK(ctx, run_eigcent(R, C, V, N, tol, max_iter, output, xa, xb, part, stt, outd, lambda, &it, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ec, outd, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (iters) *iters = it;
+    return BNA_OK;
+}
+
+}  // extern "C"

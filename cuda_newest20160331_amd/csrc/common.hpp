@@ -1,0 +1,101 @@
+// common.hpp -- shared definitions of libbna's kernels and host orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bna {
+
+// Correlation tile edge (voxels) and the K padding of the operand panels.
+constexpr int TILE = 256;
+constexpr int KPAD = 32;
+
+// One entry of the per-tile compacted edge lists: the neighbour's voxel index
+// and its r.  col < 0 marks an entry the FP64 recheck rejected.
+struct Entry {
+    int32_t col;
+    float r;
+};
+
+// A pair inside the error band, queued for the FP64 recheck.
+struct Candidate {
+    int32_t i, j;       // i < j, global voxel indices
+    int64_t slot_row;   // entry index of (i -> j) in the row-view list
+    int64_t slot_col;   // entry index of (j -> i) in the column-view list
+    float r;            // the MFMA value
+    int32_t pad;
+};
+
+// Tile work item: subject index and tile coordinates (I <= J).
+struct TileItem {
+    int32_t subj;
+    int32_t I, J;
+    int32_t pad;
+};
+
+// Per-subject description used by the tile kernels (device memory).
+struct SubjDesc {
+    const void *a0;      // operand panel 0: bf16 hi or fp32 z, N_pad x L_pad
+    const void *a1;      // operand panel 1: bf16 lo (BF16X3) or null
+    int32_t N, N_pad, L_pad, nb;
+    float keep_thr;      // entry kept iff r > keep_thr
+    float sure_thr;      // kept entry is sure iff r > sure_thr, else a candidate
+    int32_t *frag_orig;  // N x (nb + 1)
+    int32_t *frag_live;  // N x (nb + 1)
+    int64_t *frag_off;   // N x (nb + 1)
+    Entry *entries;      // compacted lists
+    unsigned long long *entry_count; // running total (may exceed capacity)
+    int64_t entry_cap;
+    Candidate *cands;
+    unsigned long long *cand_count;
+    int64_t cand_cap;
+    unsigned long long *hist; // histogram pass: HIST_BINS counters
+};
+
+constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
+
+// bf16 round-to-nearest-even from float bits; NaN stays a quiet NaN.
+__host__ __device__ inline uint16_t f32_to_bf16_rn(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+// ------------------------------------------------------------ launchers
+// z-normalisation (kernel a): mean, norm, z64 and the MFMA operand form.
+hipError_t launch_znorm(const float *x, int layout, int N, int L, int N_pad, int L_pad, double *mu,
+                        double *sd, double *z64, uint8_t *valid, uint16_t *hi, uint16_t *lo,
+                        float *z32, hipStream_t st);
+
+// Correlation + threshold + compaction over a tile table (kernel b + first half of c).
+// prec: 0 FP32, 1 BF16, 2 BF16X3.  hist_mode: 1 = histogram epilogue.
+hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const TileItem *items,
+                       int64_t n_items, hipStream_t st);
+
+// FP64 recheck of candidates.
+hipError_t launch_recheck(const double *z64, int L, int nb, const Candidate *cands,
+                          const unsigned long long *cand_count, int64_t cand_cap, double thr,
+                          int32_t *frag_live, Entry *entries, unsigned long long *stats,
+                          hipStream_t st);
+
+// Row lengths, exclusive scan into R, placement into C/V.
+hipError_t launch_rowlen(const int32_t *frag_live, int N, int nb, int64_t *rowlen, hipStream_t st);
+hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, hipStream_t st);
+hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
+                        const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C,
+                        float *V, hipStream_t st);
+
+// Degree and eigenvector centrality (kernel d).
+hipError_t launch_degree(const int32_t *R, const float *V, int N, float *deg, hipStream_t st);
+// Whole power iteration (see k_graph.hip).  output: 0 last A*y, 1 last y.
+hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, double tol, int max_iter,
+                       int output, double *xa, double *xb, double *part, void *state_dev, double *out,
+                       double *lambda, int *iters, hipStream_t st);
+size_t ec_state_bytes();
+int ec_nparts(int N);
+
+}  // namespace bna

@@ -1,0 +1,190 @@
+// host_util.cpp -- host-only parts of the C ABI: the density -> edge count rule,
+// the seeded factor-model generator, and the .csr / .nm file formats.
+#include "bna.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+extern "C" {
+
+// CorMat.cpp:327-328: `(long long) 2*M1*s/100.0` multiplies the long long 2*M1
+// by the float s in float arithmetic, divides by 100.0 in double and truncates;
+// an odd count is then made even.  Returns undirected edges (Clength / 2).
+int64_t bna_density_to_k(int N, float spa_percent)
+{
+    if (N < 2) return 0;
+    const long long M1 = (long long)(N - 1) * N / 2;
+    const long long twoM1 = 2LL * M1;
+    const float prod = (float)twoM1 * spa_percent;
+    long long Clength = (long long)((double)prod / 100.0);
+    Clength += Clength % 2;
+    return Clength / 2;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- generator
+namespace {
+
+inline uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// xoshiro256** seeded through splitmix64 from (seed, stream id).
+struct Rng {
+    uint64_t s[4];
+    Rng(uint64_t seed, uint64_t stream)
+    {
+        uint64_t x = seed ^ (stream * 0xd1b54a32d192ed03ull + 0x8bb84b93962eacc9ull);
+        for (auto &w : s) w = splitmix64(x);
+    }
+    static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        const uint64_t r = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }  // [0, 1)
+    // standard normal, Box-Muller (one value per call, the partner is dropped)
+    double normal()
+    {
+        double u1 = uniform();
+        while (u1 <= 0.0) u1 = uniform();
+        const double u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+    int below(int n) { return (int)((next() >> 33) % (uint64_t)n); }
+};
+
+}  // namespace
+
+extern "C" int bna_synth_factor_model(float *x, int N, int L, int K, double phi, uint64_t seed, int layout)
+{
+    if (!x || N < 1 || L < 1 || K < 2 || !(phi > -1.0 && phi < 1.0)) return BNA_ERR_ARG;
+    if (layout != BNA_TIME_MAJOR && layout != BNA_VOXEL_MAJOR) return BNA_ERR_ARG;
+    // latent AR(1) signals, unit variance: f0 ~ N(0,1), f_t = phi f_{t-1} + sqrt(1-phi^2) e_t
+    std::vector<double> f((size_t)K * L);
+    const double c = std::sqrt(1.0 - phi * phi);
+    for (int k = 0; k < K; k++) {
+        Rng r(seed, 0x1000000000ull + (uint64_t)k);
+        double v = r.normal();
+        f[(size_t)k * L] = v;
+        for (int t = 1; t < L; t++) {
+            v = phi * v + c * r.normal();
+            f[(size_t)k * L + t] = v;
+        }
+    }
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](unsigned tid) {
+        for (int v = (int)tid; v < N; v += (int)nt) {
+            Rng r(seed, (uint64_t)v);
+            const int k1 = r.below(K);
+            int k2 = r.below(K - 1);
+            if (k2 >= k1) k2++;
+            const double a1 = 0.5 * r.normal(), a2 = 0.5 * r.normal();
+            const double *f1 = &f[(size_t)k1 * L], *f2 = &f[(size_t)k2 * L];
+            for (int t = 0; t < L; t++) {
+                const double val = a1 * f1[t] + a2 * f2[t] + r.normal();
+                if (layout == BNA_VOXEL_MAJOR)
+                    x[(size_t)v * L + t] = (float)val;
+                else
+                    x[(size_t)t * N + v] = (float)val;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &t : th) t.join();
+    return BNA_OK;
+}
+
+// ---------------------------------------------------------------- files
+extern "C" int bna_write_csr(const char *path, const bna_csr *g)
+{
+    if (!path || !g || g->on_device || !g->R || (g->nnz > 0 && !g->C)) return BNA_ERR_ARG;
+    if (g->nnz > 0x7fffffffLL || g->N >= 0x7fffffff) return BNA_ERR_OVERFLOW;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return BNA_ERR_IO;
+    const int32_t rl = g->N + 1, cl = (int32_t)g->nnz;
+    bool ok = fwrite(&rl, 4, 1, fp) == 1 && fwrite(g->R, 4, (size_t)rl, fp) == (size_t)rl &&
+              fwrite(&cl, 4, 1, fp) == 1 && fwrite(g->C, 4, (size_t)cl, fp) == (size_t)cl;
+    if (ok && g->V) ok = fwrite(&cl, 4, 1, fp) == 1 && fwrite(g->V, 4, (size_t)cl, fp) == (size_t)cl;
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? BNA_OK : BNA_ERR_IO;
+}
+
+extern "C" int bna_read_csr(const char *path, bna_csr *out)
+{
+    if (!path || !out) return BNA_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return BNA_ERR_IO;
+    int32_t rl = 0, cl = 0, vl = 0;
+    int rc = BNA_OK;
+    int32_t *R = nullptr, *C = nullptr;
+    float *V = nullptr;
+    if (fread(&rl, 4, 1, fp) != 1 || rl < 1) { rc = BNA_ERR_FORMAT; goto done; }
+    R = (int32_t *)malloc(4 * (size_t)rl);
+    if (!R) { rc = BNA_ERR_NOMEM; goto done; }
+    if (fread(R, 4, (size_t)rl, fp) != (size_t)rl) { rc = BNA_ERR_FORMAT; goto done; }
+    if (fread(&cl, 4, 1, fp) != 1 || cl < 0) { rc = BNA_ERR_FORMAT; goto done; }
+    C = (int32_t *)malloc(4 * (size_t)std::max(cl, 1));
+    if (!C) { rc = BNA_ERR_NOMEM; goto done; }
+    if (fread(C, 4, (size_t)cl, fp) != (size_t)cl) { rc = BNA_ERR_FORMAT; goto done; }
+    if (fread(&vl, 4, 1, fp) == 1) {
+        // weighted-net.cu:283-284 calls a file with Vlength != Clength damaged
+        if (vl != cl) { rc = BNA_ERR_FORMAT; goto done; }
+        V = (float *)malloc(4 * (size_t)std::max(vl, 1));
+        if (!V) { rc = BNA_ERR_NOMEM; goto done; }
+        if (fread(V, 4, (size_t)vl, fp) != (size_t)vl) { rc = BNA_ERR_FORMAT; goto done; }
+    }
+    // structural checks: R monotone from 0 to nnz, columns in range
+    if (R[0] != 0 || R[rl - 1] != cl) { rc = BNA_ERR_FORMAT; goto done; }
+    for (int32_t k = 0; k + 1 < rl; k++)
+        if (R[k + 1] < R[k]) { rc = BNA_ERR_FORMAT; goto done; }
+    for (int32_t k = 0; k < cl; k++)
+        if (C[k] < 0 || C[k] >= rl - 1) { rc = BNA_ERR_FORMAT; goto done; }
+done:
+    fclose(fp);
+    if (rc != BNA_OK) {
+        free(R);
+        free(C);
+        free(V);
+        return rc;
+    }
+    out->N = rl - 1;
+    out->nnz = cl;
+    out->R = R;
+    out->C = C;
+    out->V = V;
+    out->on_device = 0;
+    return BNA_OK;
+}
+
+extern "C" int bna_write_nm(const char *path, const float *v, int N)
+{
+    if (!path || !v || N < 0) return BNA_ERR_ARG;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return BNA_ERR_IO;
+    const int32_t n = N;
+    bool ok = fwrite(&n, 4, 1, fp) == 1 && fwrite(v, 4, (size_t)N, fp) == (size_t)N;
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? BNA_OK : BNA_ERR_IO;
+}

@@ -1,0 +1,200 @@
+// k_csr.hip -- FP64 recheck of band pairs and the second half of kernel (c):
+// row lengths, the exclusive scan into R, and the placement of the per-tile
+// lists into CSR rows.
+//
+// CSR layout follows FormCSRGraph (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:889-918):
+// row i lists every neighbour j != i in ascending order with V = r(i, j).
+// Row i's neighbours come in nb + 1 fragments: fragment k <= I(i) is the
+// column-view list of tile (k, I(i)) for column i (neighbours in tile-row k,
+// ascending), fragment k > I(i) is the row-view list of tile (I(i), k - 1) for
+// row i (neighbours in tile-column k - 1, ascending).  Concatenating the
+// fragments in k order is therefore already column-sorted, and the placement
+// is a copy with the recheck's rejected entries squeezed out.
+#include "common.hpp"
+
+namespace bna {
+
+// One wave per candidate.  r64 is evaluated in the FP64 oracle's order: lane q
+// accumulates l = q, q + 64, ... with fused multiply-add, then a halving tree
+// (lane q adds lane q + s for s = 32, 16, ..., 1).  Edge iff r64 > thr.
+// stats[0]: max |r_mfma - r64| as double bits, stats[1]: decisions that differ
+// from the MFMA value's own (r > thr) decision.
+__global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__ z64, int L, int nb,
+                                                      const Candidate *__restrict__ cands,
+                                                      const unsigned long long *__restrict__ cand_count,
+                                                      int64_t cand_cap, double thr,
+                                                      int32_t *__restrict__ frag_live, Entry *__restrict__ entries,
+                                                      unsigned long long *__restrict__ stats)
+{
+    const int64_t n = (int64_t)min((unsigned long long)cand_cap, *cand_count);
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t c = wave; c < n; c += nwaves) {
+        const Candidate cd = cands[c];
+        const double *zi = z64 + (int64_t)cd.i * L;
+        const double *zj = z64 + (int64_t)cd.j * L;
+        double p = 0.0;
+        for (int l = lane; l < L; l += 64) p = __builtin_fma(zi[l], zj[l], p);
+#pragma unroll
+        for (int s = 32; s >= 1; s >>= 1) {
+            double o = __shfl_down(p, s, 64);
+            if (lane < s) p = p + o;
+        }
+        if (lane == 0) {
+            const bool edge = p > thr;
+            const bool raw = (double)cd.r > thr;
+            if (edge) {
+                entries[cd.slot_row].r = (float)p;
+                entries[cd.slot_col].r = (float)p;
+            } else {
+                entries[cd.slot_row].col = -1;
+                entries[cd.slot_col].col = -1;
+                const int nb1 = nb + 1;
+                atomicSub(&frag_live[(int64_t)cd.i * nb1 + (cd.j / TILE) + 1], 1);
+                atomicSub(&frag_live[(int64_t)cd.j * nb1 + (cd.i / TILE)], 1);
+            }
+            const double err = fabs((double)cd.r - p);
+            atomicMax(&stats[0], (unsigned long long)__builtin_bit_cast(unsigned long long, err));
+            if (edge != raw) atomicAdd(&stats[1], 1ull);
+        }
+    }
+}
+
+// One wave per row: sum of the live fragment counts.
+__global__ __launch_bounds__(256) void rowlen_kernel(const int32_t *__restrict__ frag_live, int N, int nb,
+                                                     int64_t *__restrict__ rowlen)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (row >= N) return;
+    const int nb1 = nb + 1;
+    const int32_t *f = frag_live + row * nb1;
+    int64_t s = 0;
+    for (int k = lane; k < nb1; k += 64) s += f[k];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) rowlen[row] = s;
+}
+
+// Single-workgroup exclusive scan over N row lengths (N is at most a few
+// hundred thousand): each thread scans a contiguous run, then the run totals
+// are scanned across the block.
+constexpr int SCAN_T = 1024;
+__global__ __launch_bounds__(SCAN_T) void scan_kernel(const int64_t *__restrict__ rowlen, int N,
+                                                      int64_t *__restrict__ R64, int32_t *__restrict__ R32)
+{
+    __shared__ int64_t part[SCAN_T];
+    const int t = threadIdx.x;
+    const int per = (N + SCAN_T - 1) / SCAN_T;
+    const int b = t * per, e = min(N, b + per);
+    int64_t s = 0;
+    for (int k = b; k < e; k++) s += rowlen[k];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < SCAN_T; d <<= 1) {
+        int64_t v = (t >= d) ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - s;
+    for (int k = b; k < e; k++) {
+        R64[k] = run;
+        if (R32) R32[k] = (int32_t)run;
+        run += rowlen[k];
+    }
+    if (t == SCAN_T - 1) {
+        R64[N] = part[t];
+        if (R32) R32[N] = (int32_t)part[t];
+    }
+}
+
+// One wave per row: lane k takes fragment k (in chunks of 64), a wave scan of
+// the live counts gives each fragment its destination, and the lane copies the
+// fragment's entries, skipping rejected ones.
+__global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ frag_orig,
+                                                    const int32_t *__restrict__ frag_live,
+                                                    const int64_t *__restrict__ frag_off,
+                                                    const Entry *__restrict__ entries,
+                                                    const int64_t *__restrict__ R64, int N, int nb,
+                                                    int32_t *__restrict__ Cout, float *__restrict__ Vout)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (row >= N) return;
+    const int nb1 = nb + 1;
+    int64_t run = R64[row];
+    for (int k0 = 0; k0 < nb1; k0 += 64) {
+        const int k = k0 + lane;
+        int live = 0, orig = 0;
+        int64_t off = 0;
+        if (k < nb1) {
+            const int64_t f = row * nb1 + k;
+            live = frag_live[f];
+            orig = frag_orig[f];
+            off = frag_off[f];
+        }
+        int inc = live;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int t = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += t;
+        }
+        const int tot = __shfl(inc, 63, 64);
+        int64_t dst = run + inc - live;
+        if (live == orig) {
+            for (int e = 0; e < orig; e++) {
+                const Entry en = entries[off + e];
+                Cout[dst + e] = en.col;
+                if (Vout) Vout[dst + e] = en.r;
+            }
+        } else {
+            for (int e = 0; e < orig; e++) {
+                const Entry en = entries[off + e];
+                if (en.col >= 0) {
+                    Cout[dst] = en.col;
+                    if (Vout) Vout[dst] = en.r;
+                    dst++;
+                }
+            }
+        }
+        run += tot;
+    }
+}
+
+hipError_t launch_recheck(const double *z64, int L, int nb, const Candidate *cands,
+                          const unsigned long long *cand_count, int64_t cand_cap, double thr,
+                          int32_t *frag_live, Entry *entries, unsigned long long *stats, hipStream_t st)
+{
+    // enough waves to cover the device; candidates are grid-strided
+    hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, z64, L, nb, cands, cand_count, cand_cap,
+                       thr, frag_live, entries, stats);
+    return hipGetLastError();
+}
+
+hipError_t launch_rowlen(const int32_t *frag_live, int N, int nb, int64_t *rowlen, hipStream_t st)
+{
+    const int64_t threads = (int64_t)N * 64;
+    hipLaunchKernelGGL(rowlen_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, frag_live, N,
+                       nb, rowlen);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, hipStream_t st)
+{
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(SCAN_T), 0, st, rowlen, N, R64, R32);
+    return hipGetLastError();
+}
+
+hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
+                        const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C, float *V,
+                        hipStream_t st)
+{
+    const int64_t threads = (int64_t)N * 64;
+    hipLaunchKernelGGL(place_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, frag_orig,
+                       frag_live, frag_off, entries, R64, N, nb, C, V);
+    return hipGetLastError();
+}
+
+}  // namespace bna

@@ -1,0 +1,189 @@
+/*
+ * bna.h -- C ABI of libbna, the MI355X (gfx950) network-construction library.
+ *
+ * Voxel time series in; CSR graphs, degree maps and eigenvector-centrality maps
+ * out.  Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns an int status (BNA_OK = 0, negative on error) unless noted, never
+ * exits the process, and frees what it allocated on failure.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ *   W/ = BNAPlatform-weighted-win64-cuda7.0-20160331/src
+ *   B/ = BNAPlatform-win64-cuda7.0-20160115/src
+ *
+ * Threading: a bna_ctx is driven by one host thread at a time; calls on one
+ * context are not re-entrant.  Different contexts may be used concurrently.
+ */
+#ifndef BNA_H
+#define BNA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BNA_ABI_VERSION 1
+
+/* ------------------------------------------------------------ status codes */
+enum bna_status {
+    BNA_OK = 0,
+    BNA_ERR_ARG = -1,          /* bad argument (NULL, N < 2, L < 2, bad enum) */
+    BNA_ERR_HIP = -2,          /* HIP runtime error (message: bna_last_error) */
+    BNA_ERR_NOMEM = -3,        /* host allocation failed */
+    BNA_ERR_DEVICE_NOMEM = -4, /* device allocation failed or over the HBM budget */
+    BNA_ERR_IO = -5,           /* file open/read/write failed */
+    BNA_ERR_FORMAT = -6,       /* malformed .csr / NIfTI / mask */
+    BNA_ERR_OVERFLOW = -7,     /* result exceeds the int32 .csr format */
+    BNA_ERR_NOT_FOUND = -8,    /* unknown request id */
+    BNA_ERR_BUSY = -9,         /* request not finished yet */
+    BNA_ERR_CANCELLED = -10,   /* request was cancelled */
+    BNA_ERR_DENSITY = -11      /* density target not reachable (fewer valid pairs than k) */
+};
+
+/* --------------------------------------------------------------- options */
+enum bna_layout {
+    BNA_TIME_MAJOR = 0, /* x[l*N + i], the reference's BOLD buffer (CorMat.cpp:218) */
+    BNA_VOXEL_MAJOR = 1 /* x[i*L + l] */
+};
+
+enum bna_precision {
+    BNA_FP32 = 0,  /* FP32 MFMA, FP32 accumulate */
+    BNA_BF16 = 1,  /* one BF16 product per term, FP32 accumulate */
+    BNA_BF16X3 = 2 /* split BF16: hi*hi + hi*lo + lo*hi, FP32 accumulate */
+};
+
+enum bna_thr_type {
+    BNA_THR_R = 0,      /* edge iff r > thr (CorMat.cpp:696) */
+    BNA_THR_DENSITY = 1 /* exactly k edges, k from a percentage (CorMat.cpp:327-328, 743-822) */
+};
+
+enum bna_ec_output {
+    BNA_EC_WEIGHTED_REF = 0, /* last A*y, the weighted tool's output (W/CUEC/weighted-net.cu:190) */
+    BNA_EC_BINARY_REF = 1,   /* last unit-norm y, the binary tool's output (B/CUEC/CUEC/CUEC_unweighted.cu:164) */
+};
+
+typedef struct bna_opts {
+    int thr_type;       /* enum bna_thr_type */
+    double thr;         /* r threshold, or density in percent (the CLI's s value) */
+    int weighted;       /* 1: keep r values (weighted .csr); 0: binary */
+    int precision;      /* enum bna_precision */
+    int recheck;        /* 1: FP64 recheck of pairs within the error band (exact edge set) */
+    int ref_compat_eps; /* 1: edge iff r > thr - 1e-6 (W/CuCorMat-2mm-fuse/CorMat-2mm-fuse.cu:13,158) */
+    int tile;           /* reserved, 0 */
+} bna_opts;
+
+/* Fills the defaults: r threshold 0, weighted, BF16X3, recheck on. */
+void bna_opts_default(bna_opts *o);
+
+/* A CSR graph.  R/C/V are host arrays when on_device == 0 and device pointers
+ * otherwise; the library allocates them (bna_csr_free releases).  V is NULL for
+ * a binary graph.  The layout is the reference's: symmetric, no diagonal, rows
+ * ascending, columns ascending within a row (CorMat.cpp:889-918). */
+typedef struct bna_csr {
+    int32_t N;
+    int64_t nnz;   /* directed entries, 2 x undirected edges (the file's Clength) */
+    int32_t *R;    /* N + 1 */
+    int32_t *C;    /* nnz */
+    float *V;      /* nnz or NULL */
+    int on_device; /* 0 host, 1 device */
+    double thr_used;      /* r threshold applied; density mode: the k-th largest r */
+    double density_pct;   /* 100 * edges / M1, the reference's s value (CorMat.cpp:321) */
+    int64_t n_candidates; /* pairs inside the error band, rechecked in FP64 */
+    int64_t n_flips;      /* band pairs whose FP64 decision differs from the MFMA one */
+    double band;          /* error band half-width used */
+    double max_err;       /* max |r_mfma - r64| over rechecked pairs */
+} bna_csr;
+
+void bna_csr_free(bna_csr *g);
+
+/* Per-call timings of the last build, milliseconds, measured with HIP events on
+ * the context's stream. */
+typedef struct bna_timings {
+    double znorm_ms;
+    double corr_ms;    /* MFMA tile kernel(s), all passes */
+    double recheck_ms;
+    double csr_ms;     /* scan + placement */
+    double select_ms;  /* density selection */
+    double total_ms;   /* first to last event of the build */
+    int64_t corr_launches;
+} bna_timings;
+
+/* ----------------------------------------------------------------- context */
+typedef struct bna_ctx bna_ctx;
+
+/* device: HIP ordinal; hbm_budget: bytes the context may hold (0 = 90% of free). */
+int bna_create(bna_ctx **ctx, int device, size_t hbm_budget);
+int bna_destroy(bna_ctx *ctx);
+int bna_synchronize(bna_ctx *ctx);
+const char *bna_status_string(int status);
+/* Last error message recorded on this context (thread-local when ctx is NULL). */
+const char *bna_last_error(bna_ctx *ctx);
+int bna_get_timings(bna_ctx *ctx, bna_timings *t);
+/* Device memory held by the context's workspace, bytes. */
+int64_t bna_workspace_bytes(bna_ctx *ctx);
+
+/* ------------------------------------------------------ construction path */
+/* Host-buffer drop-in for one subject: normalise, correlate, threshold and
+ * build the CSR.  Replaces CorMat_gpu (W/CorMat/CUCorMat/CUCorMat/CorMat.cu:17,
+ * called at CorMat.cpp:259) together with FormFullGraph / FormFullGraph_s and
+ * FormCSRGraph (CorMat.cpp:683-918); the result lands in host arrays. */
+int bna_build_csr(bna_ctx *ctx, const float *x, int N, int L, int layout, const bna_opts *opts,
+                  bna_csr *out);
+
+/* Device-resident form: the series is uploaded (or taken from device memory)
+ * and normalised once; builds leave the CSR in device memory. */
+typedef struct bna_series bna_series;
+int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, int x_on_device,
+                      bna_series **out);
+int bna_series_destroy(bna_ctx *ctx, bna_series *s);
+/* Normalisation only (z-score, kernel a); run again to re-time it. */
+int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision);
+/* Correlate + threshold + CSR on a normalised series; out->on_device = 1. */
+int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *opts, bna_csr *out);
+/* Copies a device CSR to host arrays (out gets fresh host buffers). */
+int bna_csr_to_host(bna_ctx *ctx, const bna_csr *dev, bna_csr *out);
+/* Normalised series back to the host, for tests: z as double (N x L,
+ * voxel-major), mean and norm per voxel (each may be NULL). */
+int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *mu, double *sd);
+
+/* ------------------------------------------------------------ graph maps */
+/* Degree.  Weighted: float sum of V along each row in CSR order
+ * (W/Degree/Degree/Degree.cpp:83-87); binary or V == NULL: row length as float
+ * (B/Degree/Degree/Degree.cpp:84-91).  g may be host or device; deg is host, N floats. */
+int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg);
+
+/* Eigenvector centrality by FP64 power iteration with the reference's loop:
+ * start x[i] = i, stop when ||x - x0|| <= tol or ||x + x0|| <= tol or after
+ * max_iter iterations (W/CUEC/weighted-net.cu:55-221, Lead_Vector_GPU).
+ * output: enum bna_ec_output.  ec is host, N doubles; lambda gets the last norm
+ * (the value written to _evalue.txt); iters may be NULL.  A binary graph
+ * (V == NULL) uses weight 1 for every entry. */
+int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec,
+                double *lambda, int *iters);
+
+/* -------------------------------------------------------- host utilities */
+/* Number of undirected edges for a density percentage, with the reference's
+ * float arithmetic (CorMat.cpp:327-328). */
+int64_t bna_density_to_k(int N, float spa_percent);
+
+/* Seeded factor-model series: x[v,t] = sum_k a[v,k] f[k,t] + e[v,t] with K AR(1)
+ * latent signals (coefficient phi, unit variance), each voxel loading on two
+ * distinct factors with a ~ N(0, 0.5^2), e ~ N(0, 1).  Deterministic for a seed
+ * whatever the thread count. */
+int bna_synth_factor_model(float *x, int N, int L, int K, double phi, uint64_t seed, int layout);
+
+/* .csr files (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:353-360; binary
+ * B/CorMat/CUCorMat/CUCORMAT/CorMat.cpp:350-353): int32 N+1, int32 R[N+1],
+ * int32 nnz, int32 C[nnz] and, weighted, int32 nnz, float V[nnz]. */
+int bna_write_csr(const char *path, const bna_csr *g);
+/* Reads a .csr (weighted when a V section follows); out gets host arrays. */
+int bna_read_csr(const char *path, bna_csr *out);
+/* .nm node maps (Degree.cpp:95-99, weighted-net.cu:305-322): int32 N, float[N]. */
+int bna_write_nm(const char *path, const float *v, int N);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BNA_H */

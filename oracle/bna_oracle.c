@@ -1,0 +1,616 @@
+/*
+ * bna_oracle.c -- CPU restatement of BNAPlatform's network-construction path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the GPU library is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it.  The product library never links or calls it.
+ *
+ * Reference paths below are relative to the reference tree:
+ *   W/ = BNAPlatform-weighted-win64-cuda7.0-20160331/src
+ *   B/ = BNAPlatform-win64-cuda7.0-20160115/src
+ *
+ * Contents and what each one follows:
+ *   orc_znorm           W/CorMat/CUCorMat/CUCorMat/CorMat.cu:33-52 (mean and
+ *                       norm in double, z = (x - mean) / norm), kept in double
+ *                       (z64) and narrowed to float (z32) as the reference stores.
+ *   orc_dot64           The FP64 oracle's Pearson r: sum_l z64_i[l] * z64_j[l]
+ *                       in a fixed evaluation order (64 strided fused
+ *                       multiply-add partial sums, then a halving tree).  The GPU
+ *                       recheck evaluates the same order, so both give the same
+ *                       double bit for bit.
+ *   orc_ref_corr        The reference CPU path W/CorMat/CUCorMat/CUCorMat/
+ *                       CorMat.cpp:954-965: float z products accumulated in
+ *                       double, stored as float.
+ *   orc_build_threshold FormFullGraph (CorMat.cpp:683-703, strict r > thr) followed
+ *                       by FormCSRGraph (CorMat.cpp:889-918): rows in ascending
+ *                       order, columns ascending, V = r of the pair.
+ *   orc_build_density   FormFullGraph_s (CorMat.cpp:743-822): k-th largest r,
+ *                       accept r >= r*, LIFO tie trimming in row-major
+ *                       upper-triangle order.  Both the literal stack loop and the
+ *                       closed form (first m ties in row-major order) are here so
+ *                       tests can pin one against the other.
+ *   orc_density_k       CorMat.cpp:327-328: Clength from the sparsity percentage,
+ *                       with the reference's float arithmetic.
+ *   orc_degree          W/Degree/Degree/Degree.cpp:83-87 (float sum of V in CSR
+ *                       order); binary B/Degree/Degree/Degree.cpp:84-91.
+ *   orc_eigcent         W/CUEC/weighted-net.cu:129-190 (weighted, output A*y) and
+ *                       B/CUEC/CUEC/CUEC_unweighted.cu:156-195 (binary, output y).
+ *
+ * Pairs involving a voxel whose z has a non-finite entry (a constant series gives
+ * 0/0) are never edges and take no part in density selection.
+ *
+ * Build: oracle/Makefile (gcc -O3 -mavx2 -mfma -ffp-contract=off).  Explicit
+ * fma() calls are the only fused operations; nothing is contracted implicitly.
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+int orc_cpu_has_fma(void) { return __builtin_cpu_supports("fma") && __builtin_cpu_supports("avx2"); }
+
+/* ------------------------------------------------------------------ z-norm */
+
+/* layout 0: time-major x[l*N + i] (the reference's BOLD); 1: voxel-major x[i*L + l].
+ * Outputs are voxel-major N x L.  mu/sd may be NULL; z32/valid may be NULL. */
+void orc_znorm(const float *x, int N, int L, int layout, double *mu, double *sd,
+               double *z64, float *z32, unsigned char *valid)
+{
+    for (int i = 0; i < N; i++) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int l = 0; l < L; l++) {
+            double v = layout ? x[(int64_t)i * L + l] : x[(int64_t)l * N + i];
+            s1 += v;
+        }
+        s1 /= L;
+        for (int l = 0; l < L; l++) {
+            double v = layout ? x[(int64_t)i * L + l] : x[(int64_t)l * N + i];
+            double d = v - s1;
+            double p = d * d;
+            s2 += p;
+        }
+        s2 = sqrt(s2);
+        int ok = 1;
+        for (int l = 0; l < L; l++) {
+            double v = layout ? x[(int64_t)i * L + l] : x[(int64_t)l * N + i];
+            double z = (v - s1) / s2;
+            if (!isfinite(z)) ok = 0;
+            z64[(int64_t)i * L + l] = z;
+            if (z32) z32[(int64_t)i * L + l] = (float)z;
+        }
+        if (mu) mu[i] = s1;
+        if (sd) sd[i] = s2;
+        if (valid) valid[i] = (unsigned char)ok;
+    }
+}
+
+/* ------------------------------------------------------------- FP64 r order */
+
+double orc_dot64(const double *a, const double *b, int L)
+{
+    double p[64];
+    for (int q = 0; q < 64; q++) p[q] = 0.0;
+    int l0 = 0;
+    for (; l0 + 64 <= L; l0 += 64)
+        for (int q = 0; q < 64; q++) p[q] = fma(a[l0 + q], b[l0 + q], p[q]);
+    for (int q = 0; l0 + q < L; q++) p[q] = fma(a[l0 + q], b[l0 + q], p[q]);
+    for (int s = 32; s >= 1; s >>= 1)
+        for (int q = 0; q < s; q++) p[q] = p[q] + p[q + s];
+    return p[0];
+}
+
+void orc_corr_pairs(const double *z64, int L, const int *ii, const int *jj, int64_t n, double *out)
+{
+    for (int64_t t = 0; t < n; t++)
+        out[t] = orc_dot64(z64 + (int64_t)ii[t] * L, z64 + (int64_t)jj[t] * L, L);
+}
+
+/* Reference CPU path, CorMat.cpp:954-965: sum3 (double) += float * float. */
+static inline float ref_pair(const float *a, const float *b, int L)
+{
+    double s = 0.0;
+    for (int l = 0; l < L; l++) {
+        float p = a[l] * b[l];
+        s += (double)p;
+    }
+    return (float)s;
+}
+
+/* Packed upper triangle of the reference-path r for rows [i0, i1), written in
+ * row-major pair order starting at out (caller sizes it). */
+void orc_ref_corr_rows(const float *z32, int N, int L, int i0, int i1, float *out)
+{
+    int64_t k = 0;
+    for (int i = i0; i < i1; i++)
+        for (int j = i + 1; j < N; j++) out[k++] = ref_pair(z32 + (int64_t)i * L, z32 + (int64_t)j * L, L);
+}
+
+/* -------------------------------------------------- threaded row partitions */
+
+typedef struct {
+    int64_t n, cap;
+    int *i, *j;
+    double *r;
+} edge_vec;
+
+static void ev_push(edge_vec *v, int i, int j, double r)
+{
+    if (v->n == v->cap) {
+        v->cap = v->cap ? v->cap * 2 : 1024;
+        v->i = (int *)realloc(v->i, sizeof(int) * v->cap);
+        v->j = (int *)realloc(v->j, sizeof(int) * v->cap);
+        v->r = (double *)realloc(v->r, sizeof(double) * v->cap);
+    }
+    v->i[v->n] = i;
+    v->j[v->n] = j;
+    v->r[v->n] = r;
+    v->n++;
+}
+
+static void ev_free(edge_vec *v)
+{
+    free(v->i);
+    free(v->j);
+    free(v->r);
+    memset(v, 0, sizeof(*v));
+}
+
+typedef struct {
+    const double *z64;
+    const unsigned char *valid;
+    int N, L, tid, nthreads;
+    double thr;
+    edge_vec out;
+} thr_arg;
+
+/* Rows are dealt round-robin so the triangular work is balanced. */
+static void *thr_worker(void *p)
+{
+    thr_arg *a = (thr_arg *)p;
+    for (int i = a->tid; i < a->N; i += a->nthreads) {
+        if (a->valid && !a->valid[i]) continue;
+        const double *zi = a->z64 + (int64_t)i * a->L;
+        for (int j = i + 1; j < a->N; j++) {
+            if (a->valid && !a->valid[j]) continue;
+            double r = orc_dot64(zi, a->z64 + (int64_t)j * a->L, a->L);
+            if (r > a->thr) ev_push(&a->out, i, j, r);
+        }
+    }
+    return NULL;
+}
+
+/* ----------------------------------------------------------------- graphs */
+
+typedef struct {
+    int N;
+    int64_t nnz;       /* directed entries = 2 x undirected edges */
+    int64_t *R;        /* N + 1 */
+    int *C;            /* nnz */
+    float *V;          /* nnz, (float) r64 */
+    double *V64;       /* nnz, r64 */
+    double r_star;     /* density mode: the k-th largest r */
+    int64_t n_ties;    /* density mode: pairs with r == r_star */
+} orc_graph;
+
+/* FormCSRGraph order (CorMat.cpp:889-918): for each row ascending, columns
+ * ascending; each undirected edge appears in both rows. */
+static orc_graph *graph_from_edges(int N, edge_vec *e)
+{
+    orc_graph *g = (orc_graph *)calloc(1, sizeof(orc_graph));
+    g->N = N;
+    g->R = (int64_t *)calloc((size_t)N + 1, sizeof(int64_t));
+    for (int64_t t = 0; t < e->n; t++) {
+        g->R[e->i[t] + 1]++;
+        g->R[e->j[t] + 1]++;
+    }
+    for (int i = 0; i < N; i++) g->R[i + 1] += g->R[i];
+    g->nnz = g->R[N];
+    g->C = (int *)malloc(sizeof(int) * (g->nnz ? g->nnz : 1));
+    g->V = (float *)malloc(sizeof(float) * (g->nnz ? g->nnz : 1));
+    g->V64 = (double *)malloc(sizeof(double) * (g->nnz ? g->nnz : 1));
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * ((size_t)N + 1));
+    memcpy(cur, g->R, sizeof(int64_t) * ((size_t)N + 1));
+    /* Edges sorted by (i, j): put (i -> j) entries; lower (j -> i) entries of
+     * row j come from rows i < j which are visited first, so a sweep over the
+     * edges sorted by (i, j) fills every row in ascending column order when the
+     * lower entries are placed before the upper ones.  Do it in two sweeps. */
+    /* sweep 1: count lower part per row to know where upper entries start */
+    int64_t *lowcnt = (int64_t *)calloc((size_t)N, sizeof(int64_t));
+    for (int64_t t = 0; t < e->n; t++) lowcnt[e->j[t]]++;
+    int64_t *upcur = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    for (int i = 0; i < N; i++) upcur[i] = g->R[i] + lowcnt[i];
+    /* edges must be in (i, j) order for both sweeps */
+    for (int64_t t = 0; t < e->n; t++) {
+        int i = e->i[t], j = e->j[t];
+        int64_t pu = upcur[i]++;
+        g->C[pu] = j;
+        g->V[pu] = (float)e->r[t];
+        g->V64[pu] = e->r[t];
+        int64_t pl = cur[j]++;
+        g->C[pl] = i;
+        g->V[pl] = (float)e->r[t];
+        g->V64[pl] = e->r[t];
+    }
+    free(cur);
+    free(lowcnt);
+    free(upcur);
+    return g;
+}
+
+static int cmp_edge_idx(const void *a, const void *b, void *ctx)
+{
+    const edge_vec *e = (const edge_vec *)ctx;
+    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    if (e->i[x] != e->i[y]) return e->i[x] < e->i[y] ? -1 : 1;
+    if (e->j[x] != e->j[y]) return e->j[x] < e->j[y] ? -1 : 1;
+    return 0;
+}
+
+/* merge per-thread vectors into one (i, j)-sorted vector */
+static void merge_sorted(edge_vec *parts, int np, edge_vec *out)
+{
+    int64_t n = 0;
+    for (int p = 0; p < np; p++) n += parts[p].n;
+    edge_vec all = {0};
+    all.cap = n ? n : 1;
+    all.i = (int *)malloc(sizeof(int) * all.cap);
+    all.j = (int *)malloc(sizeof(int) * all.cap);
+    all.r = (double *)malloc(sizeof(double) * all.cap);
+    for (int p = 0; p < np; p++) {
+        memcpy(all.i + all.n, parts[p].i, sizeof(int) * parts[p].n);
+        memcpy(all.j + all.n, parts[p].j, sizeof(int) * parts[p].n);
+        memcpy(all.r + all.n, parts[p].r, sizeof(double) * parts[p].n);
+        all.n += parts[p].n;
+    }
+    int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (n ? n : 1));
+    for (int64_t t = 0; t < n; t++) idx[t] = t;
+    qsort_r(idx, (size_t)n, sizeof(int64_t), cmp_edge_idx, &all);
+    out->n = 0;
+    out->cap = n ? n : 1;
+    out->i = (int *)malloc(sizeof(int) * out->cap);
+    out->j = (int *)malloc(sizeof(int) * out->cap);
+    out->r = (double *)malloc(sizeof(double) * out->cap);
+    for (int64_t t = 0; t < n; t++) {
+        out->i[t] = all.i[idx[t]];
+        out->j[t] = all.j[idx[t]];
+        out->r[t] = all.r[idx[t]];
+    }
+    out->n = n;
+    free(idx);
+    ev_free(&all);
+}
+
+/* Binary or weighted graph of all valid pairs with r64 > thr (FormFullGraph's
+ * strict comparison, CorMat.cpp:696). */
+orc_graph *orc_build_threshold(const double *z64, const unsigned char *valid, int N, int L,
+                               double thr, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    thr_arg *args = (thr_arg *)calloc((size_t)nthreads, sizeof(thr_arg));
+    for (int t = 0; t < nthreads; t++) {
+        args[t].z64 = z64;
+        args[t].valid = valid;
+        args[t].N = N;
+        args[t].L = L;
+        args[t].tid = t;
+        args[t].nthreads = nthreads;
+        args[t].thr = thr;
+        pthread_create(&th[t], NULL, thr_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    edge_vec parts_sorted = {0};
+    edge_vec *parts = (edge_vec *)malloc(sizeof(edge_vec) * nthreads);
+    for (int t = 0; t < nthreads; t++) parts[t] = args[t].out;
+    merge_sorted(parts, nthreads, &parts_sorted);
+    for (int t = 0; t < nthreads; t++) ev_free(&parts[t]);
+    orc_graph *g = graph_from_edges(N, &parts_sorted);
+    g->r_star = thr;
+    ev_free(&parts_sorted);
+    free(parts);
+    free(args);
+    free(th);
+    return g;
+}
+
+/* CorMat.cpp:327-328.  `(long long) 2*M1*s/100.0` multiplies the long long 2*M1
+ * by the float s in float arithmetic, divides by 100.0 in double and truncates;
+ * an odd count is rounded up to even.  Returns the number of undirected edges. */
+int64_t orc_density_k(int N, float spa_percent)
+{
+    long long M1 = (long long)(N - 1) * N / 2;
+    long long twoM1 = 2LL * M1;
+    float prod = (float)twoM1 * spa_percent;
+    long long Clength = (long long)((double)prod / 100.0);
+    Clength += Clength % 2;
+    return Clength / 2;
+}
+
+typedef struct {
+    const double *z64;
+    const unsigned char *valid;
+    int N, L, tid, nthreads;
+    double *packed; /* row-major upper triangle, NaN for invalid pairs */
+} all_arg;
+
+static inline int64_t packed_index(int N, int i, int j)
+{
+    return (int64_t)i * N - (int64_t)(1 + i) * i / 2 + j - (i + 1);
+}
+
+static void *all_worker(void *p)
+{
+    all_arg *a = (all_arg *)p;
+    for (int i = a->tid; i < a->N; i += a->nthreads) {
+        const double *zi = a->z64 + (int64_t)i * a->L;
+        for (int j = i + 1; j < a->N; j++) {
+            double r = NAN;
+            if (!a->valid || (a->valid[i] && a->valid[j])) r = orc_dot64(zi, a->z64 + (int64_t)j * a->L, a->L);
+            a->packed[packed_index(a->N, i, j)] = r;
+        }
+    }
+    return NULL;
+}
+
+static int cmp_desc(const void *a, const void *b)
+{
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x < y) - (x > y);
+}
+
+/* All pairs' r64 in row-major upper-triangle order (NaN for invalid pairs). */
+void orc_all_pairs(const double *z64, const unsigned char *valid, int N, int L, int nthreads, double *packed)
+{
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    all_arg *args = (all_arg *)calloc((size_t)nthreads, sizeof(all_arg));
+    for (int t = 0; t < nthreads; t++) {
+        args[t] = (all_arg){z64, valid, N, L, t, nthreads, packed};
+        pthread_create(&th[t], NULL, all_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    free(args);
+    free(th);
+}
+
+/* Exact-k selection over a packed triangle (NaN entries excluded).
+ * literal = 1 runs the reference's stack loop (CorMat.cpp:784-812) verbatim in
+ * behaviour; literal = 0 applies the closed form: every r > r*, then the first
+ * m ties at r* in row-major order.  Returns NULL if fewer than k valid pairs. */
+orc_graph *orc_select_packed(const double *packed, int N, int64_t k, int literal)
+{
+    int64_t M1 = (int64_t)N * (N - 1) / 2;
+    int64_t nv = 0;
+    for (int64_t t = 0; t < M1; t++) nv += !isnan(packed[t]);
+    if (k < 1 || k > nv) return NULL;
+    double *cp = (double *)malloc(sizeof(double) * (nv ? nv : 1));
+    int64_t c = 0;
+    for (int64_t t = 0; t < M1; t++)
+        if (!isnan(packed[t])) cp[c++] = packed[t];
+    qsort(cp, (size_t)nv, sizeof(double), cmp_desc);
+    double rs = cp[k - 1];
+    free(cp);
+
+    edge_vec e = {0};
+    int64_t n_ties = 0;
+    if (literal) {
+        unsigned char *adj = (unsigned char *)calloc((size_t)M1 ? (size_t)M1 : 1, 1);
+        int64_t *stack = (int64_t *)malloc(sizeof(int64_t) * (M1 ? M1 : 1));
+        int64_t sp = 0, cnt = 0, idx = 0;
+        for (int i = 0; i < N; i++)
+            for (int j = i + 1; j < N; j++, idx++) {
+                double r = packed[idx];
+                if (r >= rs) { adj[idx] = 1; cnt++; }
+                if (r == rs) { stack[sp++] = idx; n_ties++; }
+                if (cnt > k) { adj[stack[--sp]] = 0; cnt--; }
+            }
+        idx = 0;
+        for (int i = 0; i < N; i++)
+            for (int j = i + 1; j < N; j++, idx++)
+                if (adj[idx]) ev_push(&e, i, j, packed[idx]);
+        free(adj);
+        free(stack);
+    } else {
+        int64_t above = 0, idx = 0;
+        for (int64_t t = 0; t < M1; t++) {
+            above += packed[t] > rs;
+            n_ties += packed[t] == rs;
+        }
+        int64_t m = k - above;
+        for (int i = 0; i < N; i++)
+            for (int j = i + 1; j < N; j++, idx++) {
+                double r = packed[idx];
+                if (r > rs) ev_push(&e, i, j, r);
+                else if (r == rs && m > 0) { ev_push(&e, i, j, r); m--; }
+            }
+    }
+    orc_graph *g = graph_from_edges(N, &e);
+    g->r_star = rs;
+    g->n_ties = n_ties;
+    ev_free(&e);
+    return g;
+}
+
+orc_graph *orc_build_density(const double *z64, const unsigned char *valid, int N, int L,
+                             int64_t k, int nthreads, int literal)
+{
+    int64_t M1 = (int64_t)N * (N - 1) / 2;
+    double *packed = (double *)malloc(sizeof(double) * (M1 ? M1 : 1));
+    orc_all_pairs(z64, valid, N, L, nthreads, packed);
+    orc_graph *g = orc_select_packed(packed, N, k, literal);
+    free(packed);
+    return g;
+}
+
+int64_t orc_graph_nnz(const orc_graph *g) { return g ? g->nnz : -1; }
+double orc_graph_rstar(const orc_graph *g) { return g ? g->r_star : NAN; }
+int64_t orc_graph_ties(const orc_graph *g) { return g ? g->n_ties : -1; }
+
+void orc_graph_copy(const orc_graph *g, int64_t *R, int *C, float *V, double *V64)
+{
+    memcpy(R, g->R, sizeof(int64_t) * ((size_t)g->N + 1));
+    if (C) memcpy(C, g->C, sizeof(int) * g->nnz);
+    if (V) memcpy(V, g->V, sizeof(float) * g->nnz);
+    if (V64) memcpy(V64, g->V64, sizeof(double) * g->nnz);
+}
+
+void orc_graph_free(orc_graph *g)
+{
+    if (!g) return;
+    free(g->R);
+    free(g->C);
+    free(g->V);
+    free(g->V64);
+    free(g);
+}
+
+/* ------------------------------------------------------------------ degree */
+
+/* weighted: float sum of V in CSR order (Degree.cpp:85-87); V == NULL gives the
+ * binary degree (float)(R[k+1] - R[k]) (B/Degree/Degree/Degree.cpp:84-91). */
+void orc_degree(const int64_t *R, const float *V, int N, float *deg)
+{
+    for (int k = 0; k < N; k++) {
+        if (V) {
+            float s = 0.0f;
+            for (int64_t t = R[k]; t < R[k + 1]; t++) s += V[t];
+            deg[k] = s;
+        } else {
+            deg[k] = (float)(R[k + 1] - R[k]);
+        }
+    }
+}
+
+/* ----------------------------------------------------- eigenvector centrality */
+
+static double nrm2(const double *x, int N)
+{
+    double s = 0.0;
+    for (int i = 0; i < N; i++) s += x[i] * x[i];
+    return sqrt(s);
+}
+
+/* Power iteration as W/CUEC/weighted-net.cu:154-189:
+ *   x0 <- x; n = ||x||; x <- x * (1/n); y = A x; x <- y;
+ *   err1 = ||x - x0||, err2 = ||x + x0||; loop while both > tol and it < max_iter.
+ * Start x[i] = i (init_AD, weighted-net.cu:38-43).  V == NULL is the binary graph
+ * (init_unweightednet sets every value to 1, CUEC_unweighted.cu:47-52).
+ * out_mode 0: weighted tree's output, the last A*y (weighted-net.cu:190);
+ * out_mode 1: binary tree's output, the last unit-norm y (CUEC_unweighted.cu:164).
+ * Returns n (the eigenvalue estimate written to _evalue.txt). */
+double orc_eigcent(const int64_t *R, const int *C, const float *V, int N, double tol, int max_iter,
+                   int out_mode, double *ec, int *iters)
+{
+    double *x = (double *)malloc(sizeof(double) * N);
+    double *x0 = (double *)malloc(sizeof(double) * N);
+    double *y = (double *)malloc(sizeof(double) * N);
+    double *d = (double *)malloc(sizeof(double) * N);
+    for (int i = 0; i < N; i++) x[i] = (double)i;
+    double err1 = 1.0, err2 = 1.0, n = 0.0;
+    int it = 0;
+    while (err1 > tol && err2 > tol && it < max_iter) {
+        memcpy(x0, x, sizeof(double) * N);
+        n = nrm2(x, N);
+        double inv = 1.0 / n;
+        for (int i = 0; i < N; i++) x[i] *= inv;
+        if (out_mode == 1) memcpy(ec, x, sizeof(double) * N);
+        for (int i = 0; i < N; i++) {
+            double s = 0.0;
+            for (int64_t t = R[i]; t < R[i + 1]; t++) s += (V ? (double)V[t] : 1.0) * x[C[t]];
+            y[i] = s;
+        }
+        memcpy(x, y, sizeof(double) * N);
+        for (int i = 0; i < N; i++) d[i] = x[i] - x0[i];
+        err1 = nrm2(d, N);
+        for (int i = 0; i < N; i++) d[i] = x[i] + x0[i];
+        err2 = nrm2(d, N);
+        it++;
+    }
+    if (out_mode == 0) memcpy(ec, x, sizeof(double) * N);
+    if (iters) *iters = it;
+    free(x);
+    free(x0);
+    free(y);
+    free(d);
+    return n;
+}
+
+/* ------------------------------------------------ reference CPU path timing */
+
+typedef struct {
+    const float *z32;
+    int N, L, i0, i1, tid, nthreads;
+    float thr;
+    int64_t edges;
+    double checksum;
+} ref_arg;
+
+/* CorMat_cpu arithmetic (CorMat.cpp:954-965) with CorMat_cpu_MT's j-striping
+ * over threads (B/CorMat/CorMat_CPU/CorMat/CorMat_CPU.cpp:371), counting the
+ * pairs with r > thr (FormFullGraph's comparison).  Eight pairs run side by side
+ * so the double accumulation chains overlap; each pair's sum keeps the
+ * reference's order. */
+static void *ref_worker(void *p)
+{
+    ref_arg *a = (ref_arg *)p;
+    const int L = a->L;
+    int64_t edges = 0;
+    double cs = 0.0;
+    for (int i = a->i0; i < a->i1; i++) {
+        const float *zi = a->z32 + (int64_t)i * L;
+        int j = i + 1 + a->tid;
+        const int st = a->nthreads;
+        for (; j + 7 * st < a->N; j += 8 * st) {
+            double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            const float *zj[8];
+            for (int u = 0; u < 8; u++) zj[u] = a->z32 + (int64_t)(j + u * st) * L;
+            for (int l = 0; l < L; l++) {
+                float zl = zi[l];
+                for (int u = 0; u < 8; u++) {
+                    float pr = zl * zj[u][l];
+                    s[u] += (double)pr;
+                }
+            }
+            for (int u = 0; u < 8; u++) {
+                float r = (float)s[u];
+                edges += r > a->thr;
+                cs += r;
+            }
+        }
+        for (; j < a->N; j += st) {
+            float r = ref_pair(zi, a->z32 + (int64_t)j * L, L);
+            edges += r > a->thr;
+            cs += r;
+        }
+    }
+    a->edges = edges;
+    a->checksum = cs;
+    return NULL;
+}
+
+/* Runs the reference CPU path over rows [i0, i1) with nthreads threads; returns
+ * the number of pairs with r > thr and a checksum of the r values. */
+int64_t orc_ref_cpu_rows(const float *z32, int N, int L, int i0, int i1, float thr, int nthreads,
+                         double *checksum)
+{
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    ref_arg *args = (ref_arg *)calloc((size_t)nthreads, sizeof(ref_arg));
+    for (int t = 0; t < nthreads; t++) {
+        args[t] = (ref_arg){z32, N, L, i0, i1, t, nthreads, thr, 0, 0.0};
+        pthread_create(&th[t], NULL, ref_worker, &args[t]);
+    }
+    int64_t e = 0;
+    double cs = 0.0;
+    for (int t = 0; t < nthreads; t++) {
+        pthread_join(th[t], NULL);
+        e += args[t].edges;
+        cs += args[t].checksum;
+    }
+    if (checksum) *checksum = cs;
+    free(args);
+    free(th);
+    return e;
+}

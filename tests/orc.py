@@ -1,0 +1,165 @@
+"""ctypes binding of the CPU oracle (oracle/bna_oracle.c) for the tests.
+
+Test infrastructure only: the oracle is the checker, never the thing measured.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "_build" / "libbna_oracle.so"
+NTHREADS = min(16, os.cpu_count() or 1)
+
+_o = None
+
+
+def olib() -> C.CDLL:
+    global _o
+    if _o is not None:
+        return _o
+    if not ORACLE_SO.exists():
+        raise RuntimeError(f"{ORACLE_SO} missing: run `make -C oracle`")
+    L = C.CDLL(str(ORACLE_SO))
+    vp, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    sig = {
+        "orc_cpu_has_fma": (i32, []),
+        "orc_znorm": (None, [vp, i32, i32, i32, vp, vp, vp, vp, vp]),
+        "orc_dot64": (dbl, [vp, vp, i32]),
+        "orc_corr_pairs": (None, [vp, i32, vp, vp, i64, vp]),
+        "orc_ref_corr_rows": (None, [vp, i32, i32, i32, i32, vp]),
+        "orc_build_threshold": (vp, [vp, vp, i32, i32, dbl, i32]),
+        "orc_density_k": (i64, [i32, C.c_float]),
+        "orc_all_pairs": (None, [vp, vp, i32, i32, i32, vp]),
+        "orc_select_packed": (vp, [vp, i32, i64, i32]),
+        "orc_build_density": (vp, [vp, vp, i32, i32, i64, i32, i32]),
+        "orc_graph_nnz": (i64, [vp]),
+        "orc_graph_rstar": (dbl, [vp]),
+        "orc_graph_ties": (i64, [vp]),
+        "orc_graph_copy": (None, [vp, vp, vp, vp, vp]),
+        "orc_graph_free": (None, [vp]),
+        "orc_degree": (None, [vp, vp, i32, vp]),
+        "orc_eigcent": (dbl, [vp, vp, vp, i32, dbl, i32, i32, vp, C.POINTER(C.c_int)]),
+        "orc_ref_cpu_rows": (i64, [vp, i32, i32, i32, i32, C.c_float, i32, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _o = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def znorm(x: np.ndarray, layout: int = 0):
+    """x float32, (L, N) time-major (layout 0) or (N, L) voxel-major (layout 1).
+    Returns z64 (N, L), z32 (N, L), mu, sd, valid."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    L, N = x.shape if layout == 0 else x.shape[::-1]
+    z64 = np.empty((N, L), np.float64)
+    z32 = np.empty((N, L), np.float32)
+    mu = np.empty(N, np.float64)
+    sd = np.empty(N, np.float64)
+    valid = np.empty(N, np.uint8)
+    olib().orc_znorm(_p(x), N, L, layout, _p(mu), _p(sd), _p(z64), _p(z32), _p(valid))
+    return z64, z32, mu, sd, valid
+
+
+def dot64(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return olib().orc_dot64(_p(a), _p(b), a.shape[0])
+
+
+class OGraph:
+    def __init__(self, h):
+        L = olib()
+        if not h:
+            raise ValueError("oracle returned no graph")
+        n = L.orc_graph_nnz(h)
+        # N from the handle is not exported; callers pass it via attribute below
+        self._h = h
+        self.nnz = n
+        self.r_star = L.orc_graph_rstar(h)
+        self.n_ties = L.orc_graph_ties(h)
+
+    def load(self, N: int):
+        L = olib()
+        self.N = N
+        self.R = np.empty(N + 1, np.int64)
+        self.C = np.empty(self.nnz, np.int32)
+        self.V = np.empty(self.nnz, np.float32)
+        self.V64 = np.empty(self.nnz, np.float64)
+        L.orc_graph_copy(self._h, _p(self.R), _p(self.C), _p(self.V), _p(self.V64))
+        L.orc_graph_free(self._h)
+        self._h = None
+        return self
+
+
+def build_threshold(z64, valid, thr: float, nthreads: int = NTHREADS) -> OGraph:
+    N, L = z64.shape
+    h = olib().orc_build_threshold(_p(z64), _p(valid), N, L, thr, nthreads)
+    return OGraph(h).load(N)
+
+
+def density_k(N: int, spa: float) -> int:
+    return int(olib().orc_density_k(N, spa))
+
+
+def all_pairs(z64, valid, nthreads: int = NTHREADS) -> np.ndarray:
+    N, L = z64.shape
+    out = np.empty(N * (N - 1) // 2, np.float64)
+    olib().orc_all_pairs(_p(z64), _p(valid), N, L, nthreads, _p(out))
+    return out
+
+
+def select_packed(packed: np.ndarray, N: int, k: int, literal: bool) -> OGraph:
+    packed = np.ascontiguousarray(packed, np.float64)
+    h = olib().orc_select_packed(_p(packed), N, k, int(literal))
+    return OGraph(h).load(N)
+
+
+def build_density(z64, valid, k: int, literal: bool = False, nthreads: int = NTHREADS) -> OGraph:
+    N, L = z64.shape
+    h = olib().orc_build_density(_p(z64), _p(valid), N, L, k, nthreads, int(literal))
+    return OGraph(h).load(N)
+
+
+def degree(R, V, N):
+    R = np.ascontiguousarray(R, np.int64)
+    V = None if V is None else np.ascontiguousarray(V, np.float32)
+    deg = np.empty(N, np.float32)
+    olib().orc_degree(_p(R), _p(V), N, _p(deg))
+    return deg
+
+
+def eigcent(R, Cc, V, N, tol=1e-6, max_iter=10000, out_mode=0):
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    V = None if V is None else np.ascontiguousarray(V, np.float32)
+    ec = np.empty(N, np.float64)
+    it = C.c_int()
+    lam = olib().orc_eigcent(_p(R), _p(Cc), _p(V), N, tol, max_iter, out_mode, _p(ec), C.byref(it))
+    return ec, lam, it.value
+
+
+def ref_cpu_rows(z32, i0, i1, thr, nthreads=NTHREADS):
+    N, L = z32.shape
+    cs = C.c_double()
+    e = olib().orc_ref_cpu_rows(_p(np.ascontiguousarray(z32, np.float32)), N, L, i0, i1, thr,
+                                nthreads, C.byref(cs))
+    return int(e), cs.value
+
+
+def ref_corr_rows(z32, i0, i1):
+    N, L = z32.shape
+    n = sum(N - 1 - i for i in range(i0, i1))
+    out = np.empty(n, np.float32)
+    olib().orc_ref_corr_rows(_p(np.ascontiguousarray(z32, np.float32)), N, L, i0, i1, _p(out))
+    return out
