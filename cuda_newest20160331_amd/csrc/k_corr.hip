@@ -30,6 +30,10 @@ namespace bna {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+// global-address-space views, so panel loads compile to global_load_dwordx4
+typedef __attribute__((address_space(1))) const unsigned char gbyte;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 guint4;
 
 constexpr int CT_THREADS = 256;
 constexpr int KB = 32;  // time points per stage
@@ -88,6 +92,96 @@ __device__ __forceinline__ int popc_below(const uint32_t *bits, int pos) {
     return n;
 }
 
+// ---------------------------------------------------------------- epilogue
+// One 32x32 accumulator block at a time.  The block indices are template
+// arguments so every accumulator access is constant-indexed and the 256
+// accumulators stay in registers.  Element e of a block sits at tile row
+// wr0 + 32 RB + 8 (e / 4) + 4 half + e % 4 and tile column wc0 + 32 CB + lane % 32.
+struct EpiCtx {
+    int N, gi0, gj0, wr0, wc0, half, lrow;
+    bool diag;
+    float keep, sure;
+};
+
+struct EmitOut {
+    Entry *entries;
+    Candidate *cands;
+    unsigned long long *cand_count;
+    int64_t cand_cap;
+    int64_t base;
+    int total;
+    bool fits;
+};
+
+__device__ __forceinline__ bool epi_in(const EpiCtx &c, int lr, int lc) {
+    return (c.gi0 + lr < c.N) && (c.gj0 + lc < c.N) && (!c.diag || lr < lc);
+}
+
+template <int RB, int CB>
+__device__ __forceinline__ void epi_hist(const f32x16_t &a, const EpiCtx &c, uint32_t *hist) {
+    const int lc = c.wc0 + CB * 32 + c.lrow;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
+        const float v = a[e];
+        if (epi_in(c, lr, lc) && v == v) {
+            int bin = (int)floorf((v + 1.0f) * (HIST_BINS / 2));
+            bin = bin < 0 ? 0 : (bin >= HIST_BINS ? HIST_BINS - 1 : bin);
+            atomicAdd(&hist[bin], 1u);
+        }
+    }
+}
+
+template <int RB, int CB>
+__device__ __forceinline__ void epi_mark(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits, uint32_t *colbits) {
+    const int lc = c.wc0 + CB * 32 + c.lrow;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
+        if (a[e] > c.keep && epi_in(c, lr, lc)) {
+            atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
+            atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
+        }
+    }
+}
+
+template <int RB, int CB>
+__device__ __forceinline__ void epi_emit(const f32x16_t &a, const EpiCtx &c, const uint32_t *rowbits,
+                                         const uint32_t *colbits, const int *rowpre, const int *colpre,
+                                         const EmitOut &o) {
+    const int lc = c.wc0 + CB * 32 + c.lrow;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
+        const float v = a[e];
+        if (v > c.keep && epi_in(c, lr, lc)) {
+            const int rw = lr * 8 + (lc >> 5), cw = lc * 8 + (lr >> 5);
+            const int64_t p = o.base + rowpre[rw] + __popc(rowbits[rw] & ((1u << (lc & 31)) - 1u));
+            const int64_t q = o.base + o.total + colpre[cw] + __popc(colbits[cw] & ((1u << (lr & 31)) - 1u));
+            if (o.fits) {
+                o.entries[p] = Entry{c.gj0 + lc, v};
+                o.entries[q] = Entry{c.gi0 + lr, v};
+            }
+            if (!(v > c.sure)) {
+                const unsigned long long ci = atomicAdd(o.cand_count, 1ull);
+                if ((int64_t)ci < o.cand_cap) o.cands[ci] = Candidate{c.gi0 + lr, c.gj0 + lc, p, q, v, 0};
+            }
+        }
+    }
+}
+
+#define BNA_FOR_BLOCKS(F, ...)                                                                     \
+    do {                                                                                           \
+        F<0, 0>(acc[0][0], __VA_ARGS__); F<0, 1>(acc[0][1], __VA_ARGS__);                          \
+        F<0, 2>(acc[0][2], __VA_ARGS__); F<0, 3>(acc[0][3], __VA_ARGS__);                          \
+        F<1, 0>(acc[1][0], __VA_ARGS__); F<1, 1>(acc[1][1], __VA_ARGS__);                          \
+        F<1, 2>(acc[1][2], __VA_ARGS__); F<1, 3>(acc[1][3], __VA_ARGS__);                          \
+        F<2, 0>(acc[2][0], __VA_ARGS__); F<2, 1>(acc[2][1], __VA_ARGS__);                          \
+        F<2, 2>(acc[2][2], __VA_ARGS__); F<2, 3>(acc[2][3], __VA_ARGS__);                          \
+        F<3, 0>(acc[3][0], __VA_ARGS__); F<3, 1>(acc[3][1], __VA_ARGS__);                          \
+        F<3, 2>(acc[3][2], __VA_ARGS__); F<3, 3>(acc[3][3], __VA_ARGS__);                          \
+    } while (0)
+
 template <int PREC, int HIST>
 __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc *__restrict__ subjs,
                                                                   const TileItem *__restrict__ items,
@@ -114,41 +208,42 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     const int wr0 = (wave >> 1) * 128;
     const int wc0 = (wave & 1) * 128;
 
-    // global sources: [side][arr]
-    const unsigned char *src[2][2];
-    src[0][0] = (const unsigned char *)sd.a0 + (int64_t)I * TILE * L_pad * C::ESZ;
-    src[1][0] = (const unsigned char *)sd.a0 + (int64_t)J * TILE * L_pad * C::ESZ;
-    src[0][1] = (C::NARR == 2) ? (const unsigned char *)sd.a1 + (int64_t)I * TILE * L_pad * C::ESZ : nullptr;
-    src[1][1] = (C::NARR == 2) ? (const unsigned char *)sd.a1 + (int64_t)J * TILE * L_pad * C::ESZ : nullptr;
+    // Global sources, one per (side, array): uniform per workgroup.  Each
+    // thread owns NPRE 16-byte chunks per stage at fixed offsets within the
+    // panels; the stage index only adds kt * KB elements.
     const int64_t row_stride = (int64_t)L_pad * C::ESZ;
+    const gbyte *gsrc[2 * C::NARR];
+    gsrc[0] = (const gbyte *)sd.a0 + (int64_t)I * TILE * row_stride;
+    gsrc[C::NARR] = (const gbyte *)sd.a0 + (int64_t)J * TILE * row_stride;
+    if constexpr (C::NARR == 2) {
+        gsrc[1] = (const gbyte *)sd.a1 + (int64_t)I * TILE * row_stride;
+        gsrc[3] = (const gbyte *)sd.a1 + (int64_t)J * TILE * row_stride;
+    }
+    uint32_t goff[C::CH];
+    uint32_t loff[C::CH];
+#pragma unroll
+    for (int m = 0; m < C::CH; m++) {
+        const int id = m * CT_THREADS + tid;
+        const int row = id / C::CH, c = id % C::CH;
+        goff[m] = (uint32_t)(row * row_stride + c * 16);
+        loff[m] = (uint32_t)lds_off<PREC>(0, 0, row, c);
+    }
 
-    uint4 pre[C::NPRE];
-    auto gload = [&](int kt) {
-#pragma unroll
-        for (int s = 0; s < 2; s++)
-#pragma unroll
-            for (int a = 0; a < C::NARR; a++)
-#pragma unroll
-                for (int m = 0; m < C::CH; m++) {
-                    const int id = m * CT_THREADS + tid;
-                    const int row = id / C::CH, c = id % C::CH;
-                    pre[(s * C::NARR + a) * C::CH + m] =
-                        *(const uint4 *)(src[s][a] + row * row_stride + (int64_t)kt * KB * C::ESZ + c * 16);
-                }
-    };
-    auto lstore = [&](int stage) {
-        unsigned char *base = smem + stage * C::STAGE;
-#pragma unroll
-        for (int s = 0; s < 2; s++)
-#pragma unroll
-            for (int a = 0; a < C::NARR; a++)
-#pragma unroll
-                for (int m = 0; m < C::CH; m++) {
-                    const int id = m * CT_THREADS + tid;
-                    const int row = id / C::CH, c = id % C::CH;
-                    *(uint4 *)(base + lds_off<PREC>(s, a, row, c)) = pre[(s * C::NARR + a) * C::CH + m];
-                }
-    };
+    u32x4 pre[C::NPRE];
+#define BNA_GLOAD(KT)                                                                              \
+    do {                                                                                           \
+        const uint32_t kofs_ = (uint32_t)(KT) * (KB * C::ESZ);                                     \
+        _Pragma("unroll") for (int sa = 0; sa < 2 * C::NARR; sa++)                                 \
+            _Pragma("unroll") for (int m = 0; m < C::CH; m++)                                      \
+                pre[sa * C::CH + m] = *(const guint4 *)(gsrc[sa] + goff[m] + kofs_);               \
+    } while (0)
+#define BNA_LSTORE(STG_)                                                                          \
+    do {                                                                                           \
+        unsigned char *b_ = smem + (STG_) * C::STAGE;                                             \
+        _Pragma("unroll") for (int sa = 0; sa < 2 * C::NARR; sa++)                                 \
+            _Pragma("unroll") for (int m = 0; m < C::CH; m++)                                      \
+                *(u32x4 *)(b_ + sa * C::PANEL + loff[m]) = pre[sa * C::CH + m];                    \
+    } while (0)
 
     f32x16_t acc[4][4];
 #pragma unroll
@@ -158,8 +253,8 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[rb][cb][e] = 0.0f;
 
-    gload(0);
-    lstore(0);
+    BNA_GLOAD(0);
+    BNA_LSTORE(0);
     __syncthreads();
 
     const int lrow = lane & 31;
@@ -167,7 +262,9 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
 
     for (int kt = 0; kt < nk; kt++) {
         const int cur = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
+        // the last stage reloads itself so the loop body has no branch; the
+        // copy lands in the idle buffer and is never read
+        BNA_GLOAD(kt + 1 < nk ? kt + 1 : kt);
         const unsigned char *base = smem + cur * C::STAGE;
         if constexpr (PREC == 0) {
 #pragma unroll
@@ -224,124 +321,91 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
                 }
             }
         }
-        if (kt + 1 < nk) lstore(cur ^ 1);
+        BNA_LSTORE(cur ^ 1);
         __syncthreads();
     }
+#undef BNA_GLOAD
+#undef BNA_LSTORE
 
     // ------------------------------------------------------------ epilogue
-    const int N = sd.N;
-    const int gi0 = I * TILE, gj0 = J * TILE;
-    const bool diag = (I == J);
+    EpiCtx ec;
+    ec.N = sd.N;
+    ec.gi0 = I * TILE;
+    ec.gj0 = J * TILE;
+    ec.wr0 = wr0;
+    ec.wc0 = wc0;
+    ec.half = half;
+    ec.lrow = lrow;
+    ec.diag = (I == J);
+    ec.keep = sd.keep_thr;
+    ec.sure = sd.sure_thr;
 
     if constexpr (HIST) {
         uint32_t *hist = (uint32_t *)smem;
         for (int k = tid; k < HIST_BINS; k += CT_THREADS) hist[k] = 0;
         __syncthreads();
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const int lr = wr0 + rb * 32 + (e >> 2) * 8 + half * 4 + (e & 3);
-                    const int lc = wc0 + cb * 32 + lrow;
-                    const float v = acc[rb][cb][e];
-                    const bool inb = (gi0 + lr < N) && (gj0 + lc < N) && (!diag || lr < lc) && (v == v);
-                    if (inb) {
-                        int bin = (int)floorf((v + 1.0f) * (HIST_BINS / 2));
-                        bin = bin < 0 ? 0 : (bin >= HIST_BINS ? HIST_BINS - 1 : bin);
-                        atomicAdd(&hist[bin], 1u);
-                    }
-                }
+        BNA_FOR_BLOCKS(epi_hist, ec, hist);
         __syncthreads();
         for (int k = tid; k < HIST_BINS; k += CT_THREADS)
             if (hist[k]) atomicAdd(&sd.hist[k], (unsigned long long)hist[k]);
         return;
     } else {
-        uint32_t *rowbits = (uint32_t *)smem;        // [256][8]
-        uint32_t *colbits = rowbits + TILE * 8;      // [256][8]
-        int *rowoff = (int *)(colbits + TILE * 8);   // [256]
-        int *coloff = rowoff + TILE;                 // [256]
-        int *wsum = coloff + TILE;                   // [4]
+        uint32_t *rowbits = (uint32_t *)smem;        // [256][8] tile-row bitmaps
+        uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
+        int *rowpre = (int *)(colbits + TILE * 8);   // [256][8] slot of each bitmap word
+        int *colpre = rowpre + TILE * 8;             // [256][8]
+        int *wsum = colpre + TILE * 8;               // [4]
         int *wsum2 = wsum + 4;                       // [4]
         unsigned long long *sbase = (unsigned long long *)(wsum2 + 4);
         for (int k = tid; k < TILE * 8 * 2; k += CT_THREADS) rowbits[k] = 0;
         __syncthreads();
 
-        const float keep = sd.keep_thr;
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const int lr = wr0 + rb * 32 + (e >> 2) * 8 + half * 4 + (e & 3);
-                    const int lc = wc0 + cb * 32 + lrow;
-                    const float v = acc[rb][cb][e];
-                    const bool kp = (v > keep) && (gi0 + lr < N) && (gj0 + lc < N) && (!diag || lr < lc);
-                    if (kp) {
-                        atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
-                        atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
-                    }
-                }
+        BNA_FOR_BLOCKS(epi_mark, ec, rowbits, colbits);
         __syncthreads();
 
+        int rw[8], cw[8];
         int rc = 0, cc = 0;
 #pragma unroll
         for (int w = 0; w < 8; w++) {
-            rc += __popc(rowbits[tid * 8 + w]);
-            cc += __popc(colbits[tid * 8 + w]);
+            rw[w] = __popc(rowbits[tid * 8 + w]);
+            cw[w] = __popc(colbits[tid * 8 + w]);
+            rc += rw[w];
+            cc += cw[w];
         }
         int total = 0, total2 = 0;
         const int ro = block_excl_scan(rc, wsum, total);
         const int co = block_excl_scan(cc, wsum2, total2);
-        rowoff[tid] = ro;
-        coloff[tid] = co;
+        {
+            int a = ro, b2 = co;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                rowpre[tid * 8 + w] = a;
+                colpre[tid * 8 + w] = b2;
+                a += rw[w];
+                b2 += cw[w];
+            }
+        }
         if (tid == 0) *sbase = atomicAdd(sd.entry_count, (unsigned long long)(2 * total));
         __syncthreads();
         const int64_t base = (int64_t)*sbase;
         const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
 
         const int nb1 = sd.nb + 1;
-        if (gi0 + tid < N) {
-            const int64_t f = (int64_t)(gi0 + tid) * nb1 + (J + 1);
+        if (ec.gi0 + tid < ec.N) {
+            const int64_t f = (int64_t)(ec.gi0 + tid) * nb1 + (J + 1);
             sd.frag_orig[f] = rc;
             sd.frag_live[f] = rc;
             sd.frag_off[f] = base + ro;
         }
-        if (gj0 + tid < N) {
-            const int64_t f = (int64_t)(gj0 + tid) * nb1 + I;
+        if (ec.gj0 + tid < ec.N) {
+            const int64_t f = (int64_t)(ec.gj0 + tid) * nb1 + I;
             sd.frag_orig[f] = cc;
             sd.frag_live[f] = cc;
             sd.frag_off[f] = base + total + co;
         }
         if (total == 0) return;
-
-        const float sure = sd.sure_thr;
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const int lr = wr0 + rb * 32 + (e >> 2) * 8 + half * 4 + (e & 3);
-                    const int lc = wc0 + cb * 32 + lrow;
-                    const float v = acc[rb][cb][e];
-                    const bool kp = (v > keep) && (gi0 + lr < N) && (gj0 + lc < N) && (!diag || lr < lc);
-                    if (kp) {
-                        const int64_t p = base + rowoff[lr] + popc_below(rowbits + lr * 8, lc);
-                        const int64_t q = base + total + coloff[lc] + popc_below(colbits + lc * 8, lr);
-                        if (fits) {
-                            sd.entries[p] = Entry{gj0 + lc, v};
-                            sd.entries[q] = Entry{gi0 + lr, v};
-                        }
-                        if (!(v > sure)) {
-                            const unsigned long long ci = atomicAdd(sd.cand_count, 1ull);
-                            if ((int64_t)ci < sd.cand_cap)
-                                sd.cands[ci] = Candidate{gi0 + lr, gj0 + lc, p, q, v, 0};
-                        }
-                    }
-                }
+        EmitOut eo{sd.entries, sd.cands, sd.cand_count, sd.cand_cap, base, total, fits};
+        BNA_FOR_BLOCKS(epi_emit, ec, rowbits, colbits, rowpre, colpre, eo);
     }
 }
 
@@ -349,7 +413,7 @@ template <int PREC, int HIST>
 static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, int64_t n_items, hipStream_t st)
 {
     using C = CorrCfg<PREC>;
-    constexpr size_t epi = (size_t)TILE * 8 * 4 * 2 + TILE * 4 * 2 + 64;
+    constexpr size_t epi = (size_t)TILE * 8 * 4 * 4 + 64;
     constexpr size_t hst = (size_t)HIST_BINS * 4;
     constexpr size_t need = 2 * (size_t)C::STAGE;
     constexpr size_t shm = need > epi ? (need > hst ? need : hst) : (epi > hst ? epi : hst);
