@@ -45,8 +45,8 @@ struct bna_series {
     bool own_x = false;
     double *z64 = nullptr, *mu = nullptr, *sd = nullptr;
     uint8_t *valid = nullptr;
-    uint16_t *hi = nullptr, *lo = nullptr;
-    float *z32 = nullptr;
+    unsigned char *panel = nullptr;  // k-blocked MFMA operands of prec_ready
+    size_t panel_bytes = 0;
     int prec_ready = -1;  // precision whose operand panel is present
     size_t bytes = 0;
 };
@@ -274,18 +274,16 @@ int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, in
 
 static void series_free_norm(bna_ctx *ctx, bna_series *s)
 {
-    const size_t NL = (size_t)s->N * s->L, NP = (size_t)s->N_pad * s->L_pad;
+    const size_t NL = (size_t)s->N * s->L;
     dev_free(ctx, s->z64, NL * 8);
     dev_free(ctx, s->mu, (size_t)s->N * 8);
     dev_free(ctx, s->sd, (size_t)s->N * 8);
     dev_free(ctx, s->valid, (size_t)s->N);
-    dev_free(ctx, s->hi, NP * 2);
-    dev_free(ctx, s->lo, NP * 2);
-    dev_free(ctx, s->z32, NP * 4);
+    dev_free(ctx, s->panel, s->panel_bytes);
     s->z64 = s->mu = s->sd = nullptr;
     s->valid = nullptr;
-    s->hi = s->lo = nullptr;
-    s->z32 = nullptr;
+    s->panel = nullptr;
+    s->panel_bytes = 0;
     s->prec_ready = -1;
 }
 
@@ -304,7 +302,7 @@ int bna_series_destroy(bna_ctx *ctx, bna_series *s)
 static int series_normalize(bna_ctx *ctx, bna_series *s, int prec)
 {
     if (prec < BNA_FP32 || prec > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bad precision");
-    const size_t NL = (size_t)s->N * s->L, NP = (size_t)s->N_pad * s->L_pad;
+    const size_t NL = (size_t)s->N * s->L;
     int rc;
     if (!s->z64) {
         if ((rc = dev_alloc(ctx, (void **)&s->z64, NL * 8))) return rc;
@@ -313,23 +311,18 @@ static int series_normalize(bna_ctx *ctx, bna_series *s, int prec)
         if ((rc = dev_alloc(ctx, (void **)&s->valid, (size_t)s->N))) return rc;
     }
     // keep only the operand panel of this precision
-    if (prec == BNA_FP32) {
-        if (s->hi) { dev_free(ctx, s->hi, NP * 2); s->hi = nullptr; }
-        if (s->lo) { dev_free(ctx, s->lo, NP * 2); s->lo = nullptr; }
-        if (!s->z32 && (rc = dev_alloc(ctx, (void **)&s->z32, NP * 4))) return rc;
-    } else {
-        if (s->z32) { dev_free(ctx, s->z32, NP * 4); s->z32 = nullptr; }
-        if (!s->hi && (rc = dev_alloc(ctx, (void **)&s->hi, NP * 2))) return rc;
-        if (prec == BNA_BF16X3) {
-            if (!s->lo && (rc = dev_alloc(ctx, (void **)&s->lo, NP * 2))) return rc;
-        } else if (s->lo) {
-            dev_free(ctx, s->lo, NP * 2);
-            s->lo = nullptr;
-        }
+    const size_t pb = (size_t)(s->L_pad / 16) * s->N_pad * panel_unit(prec);
+    if (s->panel && s->panel_bytes != pb) {
+        dev_free(ctx, s->panel, s->panel_bytes);
+        s->panel = nullptr;
+        s->panel_bytes = 0;
+    }
+    if (!s->panel) {
+        if ((rc = dev_alloc(ctx, (void **)&s->panel, pb))) return rc;
+        s->panel_bytes = pb;
     }
     HIPCHK(ctx, launch_znorm(s->x, s->layout, s->N, s->L, s->N_pad, s->L_pad, s->mu, s->sd, s->z64, s->valid,
-                             prec == BNA_FP32 ? nullptr : s->hi, prec == BNA_BF16X3 ? s->lo : nullptr,
-                             prec == BNA_FP32 ? s->z32 : nullptr, ctx->stream));
+                             prec, s->panel, ctx->stream));
     s->prec_ready = prec;
     return BNA_OK;
 }
@@ -479,8 +472,8 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     int64_t ccap = std::max<int64_t>(1 << 16, (int64_t)(ctx->ws["cands"].bytes / sizeof(Candidate)));
 
     SubjDesc h{};
-    h.a0 = prec == BNA_FP32 ? (const void *)s->z32 : (const void *)s->hi;
-    h.a1 = prec == BNA_BF16X3 ? (const void *)s->lo : nullptr;
+    h.a0 = s->panel;
+    h.a1 = nullptr;
     h.N = N;
     h.N_pad = s->N_pad;
     h.L_pad = s->L_pad;

@@ -35,8 +35,8 @@ struct TileItem {
 
 // Per-subject description used by the tile kernels (device memory).
 struct SubjDesc {
-    const void *a0;      // operand panel 0: bf16 hi or fp32 z, N_pad x L_pad
-    const void *a1;      // operand panel 1: bf16 lo (BF16X3) or null
+    const void *a0;      // k-blocked operand panel (see panel_unit)
+    const void *a1;      // unused
     int32_t N, N_pad, L_pad, nb;
     float keep_thr;      // entry kept iff r > keep_thr
     float sure_thr;      // kept entry is sure iff r > sure_thr, else a candidate
@@ -66,10 +66,14 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 }
 
 // ------------------------------------------------------------ launchers
-// z-normalisation (kernel a): mean, norm, z64 and the MFMA operand form.
+// Bytes per (16-point block, voxel row) record of the k-blocked operand panel:
+// fp32 z[16] = 64, bf16 hi[16] = 32, bf16x3 hi[16] lo[16] = 64.
+__host__ __device__ inline int panel_unit(int prec) { return prec == 1 ? 32 : 64; }
+
+// z-normalisation (kernel a): mean, norm, z64 and the k-blocked MFMA operand panel.
 hipError_t launch_znorm(const float *x, int layout, int N, int L, int N_pad, int L_pad, double *mu,
-                        double *sd, double *z64, uint8_t *valid, uint16_t *hi, uint16_t *lo,
-                        float *z32, hipStream_t st);
+                        double *sd, double *z64, uint8_t *valid, int prec, unsigned char *panel,
+                        hipStream_t st);
 
 // Correlation + threshold + compaction over a tile table (kernel b + first half of c).
 // prec: 0 FP32, 1 BF16, 2 BF16X3.  hist_mode: 1 = histogram epilogue.

@@ -10,15 +10,21 @@
 // be assembled later without a sort.
 //
 // Work split: a 256 x 256 tile per workgroup of 4 waves; each wave owns a
-// 128 x 128 sub-tile held as 4 x 4 MFMA 32x32 accumulator blocks.  K runs in
-// stages of 32 time points, double-buffered in LDS, with the next stage's global
-// loads in flight in registers while the current stage feeds the MFMAs.
+// 128 x 128 sub-tile held as 4 x 4 MFMA 32x32 accumulator blocks (256 AGPRs).
 //
-// Precisions (template PREC):
-//   0 FP32    v_mfma_f32_32x32x2_f32,  z32 operands
-//   1 BF16    v_mfma_f32_32x32x16_bf16, hi operands
-//   2 BF16X3  v_mfma_f32_32x32x16_bf16, hi*hi + hi*lo + lo*hi
-// All accumulate in FP32.
+// Operand panels are k-blocked: for each block of 16 time points kb and each
+// voxel row r there is one UNIT-byte record at ((kb * N_pad) + r) * UNIT, so
+// one stage of a 256-row panel is a contiguous 256 * UNIT bytes.  Stages move
+// global -> LDS with global_load_lds_dwordx4 (no registers in between) into a
+// ring of 4 slots; the load for stage kt + 4 is issued while stage kt is
+// multiplied, so every load has three stages of MFMA work to arrive in.  The
+// MFMA fragments are double-buffered in registers: the LDS reads of stage
+// kt + 1 are issued before the MFMAs of stage kt.
+//
+// Precisions (template PREC), all accumulating in FP32:
+//   0 FP32    v_mfma_f32_32x32x2_f32,   UNIT 64: z[16] fp32
+//   1 BF16    v_mfma_f32_32x32x16_bf16, UNIT 32: hi[16]
+//   2 BF16X3  v_mfma_f32_32x32x16_bf16, UNIT 64: hi[16] lo[16]; hi*hi + hi*lo + lo*hi
 //
 // Epilogue: each kept pair (r > keep_thr) sets a bit in a tile-row and a
 // tile-column bitmap in LDS; per-row and per-column popcounts, two block scans
@@ -30,37 +36,37 @@ namespace bna {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
-// global-address-space views, so panel loads compile to global_load_dwordx4
-typedef __attribute__((address_space(1))) const unsigned char gbyte;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) const u32x4 guint4;
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) void gvoid;
+typedef __attribute__((address_space(3))) void lvoid;
 
 constexpr int CT_THREADS = 256;
-constexpr int KB = 32;  // time points per stage
+constexpr int NSLOT = 4;  // LDS ring of k16 stages
 
 template <int PREC>
 struct CorrCfg {
-    static constexpr int NARR = (PREC == 2) ? 2 : 1;       // arrays per panel
-    static constexpr int ROWB = (PREC == 0) ? 128 : 64;    // bytes per panel row per stage
-    static constexpr int CH = ROWB / 16;                   // 16-byte chunks per row
-    static constexpr int ESZ = (PREC == 0) ? 4 : 2;        // element size
-    static constexpr int PANEL = TILE * ROWB;              // bytes per array per side
-    static constexpr int STAGE = 2 * NARR * PANEL;         // A + B
-    static constexpr int NPRE = 2 * NARR * CH;             // uint4 per thread per stage
+    static constexpr int UNIT = (PREC == 1) ? 32 : 64;   // bytes per (k16 block, row)
+    static constexpr int CH = UNIT / 16;                 // 16-byte chunks per row
+    static constexpr int PANEL = TILE * UNIT;            // bytes of one side of a stage
+    static constexpr int SLOT = 2 * PANEL;               // A + B
+    static constexpr int DMA_PER_SIDE = PANEL / 1024 / 4;  // 1 KB wave instructions per wave per side
 };
 
-// XOR swizzle of the 16-byte chunk index so that the 16 lanes of each
-// ds_read_b128 phase touch 16 different bank groups.
+// XOR swizzle of the 16-byte chunk index within a row so that the 16 lanes of
+// each ds_read_b128 phase (16 consecutive rows, one chunk each) cover 16
+// different bank groups.
 template <int CH>
 __device__ __forceinline__ int swz(int row) {
-    return (CH == 4) ? ((row >> 2) & 3) : ((row >> 1) & 7);
+    return (CH == 4) ? ((row >> 2) & 3) : ((row >> 3) & 1);
 }
 
-template <int PREC>
-__device__ __forceinline__ int lds_off(int side, int arr, int row, int c) {
-    using C = CorrCfg<PREC>;
-    return ((side * C::NARR + arr) * TILE + row) * C::ROWB + ((c ^ swz<C::CH>(row)) << 4);
-}
+// s_waitcnt vmcnt <= n and lgkmcnt == 0 (gfx9 encoding: vmcnt bits 3:0 and
+// 15:14, expcnt 6:4 left unconstrained, lgkmcnt 11:8).  The lgkmcnt part makes
+// sure this wave's LDS reads of a slot have completed before the barrier that
+// lets the slot be refilled.
+#define BNA_WAIT_VMCNT(n) \
+    __builtin_amdgcn_s_waitcnt((((n) & 0xf) | (((n) >> 4) << 14)) | (0x7 << 4))
 
 // Block-wide exclusive scan of one int per thread (256 threads).
 __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
@@ -82,14 +88,6 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
         total += s;
     }
     return pre + inc - v;
-}
-
-__device__ __forceinline__ int popc_below(const uint32_t *bits, int pos) {
-    int n = 0;
-    const int w = pos >> 5;
-    for (int k = 0; k < w; k++) n += __popc(bits[k]);
-    n += __popc(bits[w] & ((1u << (pos & 31)) - 1u));
-    return n;
 }
 
 // ---------------------------------------------------------------- epilogue
@@ -145,29 +143,66 @@ __device__ __forceinline__ void epi_mark(const f32x16_t &a, const EpiCtx &c, uin
     }
 }
 
+// Kept pairs of one block are appended to the wave's LDS list (tile row,
+// tile column, r) with a ballot per element slot; the list is then walked
+// lane-parallel to compute both slots and write the entries.  Only compares
+// and ballots touch the accumulators, which keeps register pressure low.
+struct KeptPair {
+    uint16_t lr, lc;
+    float r;
+};
+
 template <int RB, int CB>
-__device__ __forceinline__ void epi_emit(const f32x16_t &a, const EpiCtx &c, const uint32_t *rowbits,
-                                         const uint32_t *colbits, const int *rowpre, const int *colpre,
-                                         const EmitOut &o) {
+__device__ __forceinline__ int epi_collect(const f32x16_t &a, const EpiCtx &c, KeptPair *list)
+{
     const int lc = c.wc0 + CB * 32 + c.lrow;
+    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    int n = 0;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
         const float v = a[e];
-        if (v > c.keep && epi_in(c, lr, lc)) {
-            const int rw = lr * 8 + (lc >> 5), cw = lc * 8 + (lr >> 5);
-            const int64_t p = o.base + rowpre[rw] + __popc(rowbits[rw] & ((1u << (lc & 31)) - 1u));
-            const int64_t q = o.base + o.total + colpre[cw] + __popc(colbits[cw] & ((1u << (lr & 31)) - 1u));
-            if (o.fits) {
-                o.entries[p] = Entry{c.gj0 + lc, v};
-                o.entries[q] = Entry{c.gi0 + lr, v};
-            }
-            if (!(v > c.sure)) {
-                const unsigned long long ci = atomicAdd(o.cand_count, 1ull);
-                if ((int64_t)ci < o.cand_cap) o.cands[ci] = Candidate{c.gi0 + lr, c.gj0 + lc, p, q, v, 0};
-            }
+        const bool kp = v > c.keep && epi_in(c, lr, lc);
+        const uint64_t m = __ballot(kp);
+        if (kp) list[n + __popcll(m & lt)] = KeptPair{(uint16_t)lr, (uint16_t)lc, v};
+        n += __popcll(m);
+    }
+    return n;
+}
+
+__device__ __forceinline__ void epi_emit_list(const KeptPair *list, int n, const EpiCtx &c, const uint32_t *rowbits,
+                                              const uint32_t *colbits, const int *rowpre, const int *colpre,
+                                              const EmitOut &o)
+{
+    for (int k = (int)(threadIdx.x & 63); k < n; k += 64) {
+        const KeptPair kp = list[k];
+        const int lr = kp.lr, lc = kp.lc;
+        const float v = kp.r;
+        const int rw = lr * 8 + (lc >> 5), cw = lc * 8 + (lr >> 5);
+        const int64_t p = o.base + rowpre[rw] + __popc(rowbits[rw] & ((1u << (lc & 31)) - 1u));
+        const int64_t q = o.base + o.total + colpre[cw] + __popc(colbits[cw] & ((1u << (lr & 31)) - 1u));
+        if (o.fits) {
+            o.entries[p] = Entry{c.gj0 + lc, v};
+            o.entries[q] = Entry{c.gi0 + lr, v};
+        }
+        if (!(v > c.sure)) {
+            const unsigned long long ci = atomicAdd(o.cand_count, 1ull);
+            if ((int64_t)ci < o.cand_cap) o.cands[ci] = Candidate{c.gi0 + lr, c.gj0 + lc, p, q, v, 0};
         }
     }
+}
+
+template <int RB, int CB>
+__device__ __forceinline__ void epi_emit(const f32x16_t &a, const EpiCtx &c, const uint32_t *rowbits,
+                                         const uint32_t *colbits, const int *rowpre, const int *colpre,
+                                         const EmitOut &o, KeptPair *list)
+{
+    const int n = epi_collect<RB, CB>(a, c, list);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    epi_emit_list(list, n, c, rowbits, colbits, rowpre, colpre, o);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 #define BNA_FOR_BLOCKS(F, ...)                                                                     \
@@ -182,13 +217,14 @@ __device__ __forceinline__ void epi_emit(const f32x16_t &a, const EpiCtx &c, con
         F<3, 2>(acc[3][2], __VA_ARGS__); F<3, 3>(acc[3][3], __VA_ARGS__);                          \
     } while (0)
 
+
 template <int PREC, int HIST>
 __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc *__restrict__ subjs,
                                                                   const TileItem *__restrict__ items,
                                                                   int64_t n_items)
 {
     using C = CorrCfg<PREC>;
-    extern __shared__ __align__(16) unsigned char smem[];
+    extern __shared__ __align__(1024) unsigned char smem[];
 
     // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
     // run of the (band-ordered) tile table so tiles that share panels share an L2.
@@ -199,51 +235,45 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     const TileItem it = items[logical];
     const SubjDesc sd = subjs[it.subj];
     const int I = it.I, J = it.J;
-    const int L_pad = sd.L_pad;
-    const int nk = L_pad / KB;
+    const int N_pad = sd.N_pad;
+    const int nk = sd.L_pad / 16;  // k16 stages
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int wr0 = (wave >> 1) * 128;
     const int wc0 = (wave & 1) * 128;
+    const int lrow = lane & 31;
+    const int half = lane >> 5;
 
-    // Global sources, one per (side, array): uniform per workgroup.  Each
-    // thread owns NPRE 16-byte chunks per stage at fixed offsets within the
-    // panels; the stage index only adds kt * KB elements.
-    const int64_t row_stride = (int64_t)L_pad * C::ESZ;
-    const gbyte *gsrc[2 * C::NARR];
-    gsrc[0] = (const gbyte *)sd.a0 + (int64_t)I * TILE * row_stride;
-    gsrc[C::NARR] = (const gbyte *)sd.a0 + (int64_t)J * TILE * row_stride;
-    if constexpr (C::NARR == 2) {
-        gsrc[1] = (const gbyte *)sd.a1 + (int64_t)I * TILE * row_stride;
-        gsrc[3] = (const gbyte *)sd.a1 + (int64_t)J * TILE * row_stride;
-    }
-    uint32_t goff[C::CH];
-    uint32_t loff[C::CH];
+    // DMA addressing.  A wave instruction fills one contiguous KB of LDS, lane l
+    // the 16 bytes at l * 16: row l / CH of the KB's rows, physical chunk l % CH,
+    // which holds logical chunk (l % CH) ^ swz(row).  Wave w fills rows
+    // [w * 64, w * 64 + 64) of each side.
+    const unsigned char *gA = (const unsigned char *)sd.a0 + (int64_t)I * TILE * C::UNIT;
+    const unsigned char *gB = (const unsigned char *)sd.a0 + (int64_t)J * TILE * C::UNIT;
+    const int64_t kstride = (int64_t)N_pad * C::UNIT;  // bytes between k16 blocks
+    constexpr int ROWS_PER_KB = 1024 / C::UNIT;
+    uint32_t dsrc[C::DMA_PER_SIDE];  // byte offset within a stage panel, per instruction
 #pragma unroll
-    for (int m = 0; m < C::CH; m++) {
-        const int id = m * CT_THREADS + tid;
-        const int row = id / C::CH, c = id % C::CH;
-        goff[m] = (uint32_t)(row * row_stride + c * 16);
-        loff[m] = (uint32_t)lds_off<PREC>(0, 0, row, c);
+    for (int q = 0; q < C::DMA_PER_SIDE; q++) {
+        const int row = wave * (TILE / 4) + q * ROWS_PER_KB + lane / C::CH;
+        const int pc = lane % C::CH;
+        dsrc[q] = (uint32_t)(row * C::UNIT + ((pc ^ swz<C::CH>(row)) << 4));
     }
-
-    u32x4 pre[C::NPRE];
-#define BNA_GLOAD(KT)                                                                              \
+#define BNA_DMA(STAGE_IDX)                                                                         \
     do {                                                                                           \
-        const uint32_t kofs_ = (uint32_t)(KT) * (KB * C::ESZ);                                     \
-        _Pragma("unroll") for (int sa = 0; sa < 2 * C::NARR; sa++)                                 \
-            _Pragma("unroll") for (int m = 0; m < C::CH; m++)                                      \
-                pre[sa * C::CH + m] = *(const guint4 *)(gsrc[sa] + goff[m] + kofs_);               \
+        const int st_ = (STAGE_IDX) < nk ? (STAGE_IDX) : nk - 1;                                   \
+        unsigned char *slot_ = smem + ((STAGE_IDX) % NSLOT) * C::SLOT;                             \
+        const unsigned char *a_ = gA + st_ * kstride;                                              \
+        const unsigned char *b_ = gB + st_ * kstride;                                              \
+        _Pragma("unroll") for (int q = 0; q < C::DMA_PER_SIDE; q++) {                              \
+            unsigned char *dA = slot_ + (wave * C::DMA_PER_SIDE + q) * 1024;                       \
+            __builtin_amdgcn_global_load_lds((gvoid *)(a_ + dsrc[q]), (lvoid *)dA, 16, 0, 0);      \
+            __builtin_amdgcn_global_load_lds((gvoid *)(b_ + dsrc[q]), (lvoid *)(dA + C::PANEL), 16, 0, 0); \
+        }                                                                                          \
     } while (0)
-#define BNA_LSTORE(STG_)                                                                          \
-    do {                                                                                           \
-        unsigned char *b_ = smem + (STG_) * C::STAGE;                                             \
-        _Pragma("unroll") for (int sa = 0; sa < 2 * C::NARR; sa++)                                 \
-            _Pragma("unroll") for (int m = 0; m < C::CH; m++)                                      \
-                *(u32x4 *)(b_ + sa * C::PANEL + loff[m]) = pre[sa * C::CH + m];                    \
-    } while (0)
+    constexpr int DMA_PER_STAGE = 2 * C::DMA_PER_SIDE;  // per thread (per wave instruction count)
 
     f32x16_t acc[4][4];
 #pragma unroll
@@ -253,79 +283,89 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[rb][cb][e] = 0.0f;
 
-    BNA_GLOAD(0);
-    BNA_LSTORE(0);
-    __syncthreads();
+    // Fragment registers, two sets.  bf16: hi and lo 8-element vectors per
+    // block; fp32: two chunks (8 consecutive k) per block for lane half h.
+    u32x4 fa0[4][2], fb0[4][2], fa1[4][2], fb1[4][2];
+    // logical chunks a lane reads: bf16 (hi h, lo h) = (h, 2 + h); bf16 plain (h);
+    // fp32 (2h, 2h + 1)
+    const int c0 = (PREC == 0) ? 2 * half : half;
+    const int c1 = (PREC == 0) ? 2 * half + 1 : 2 + half;
+#define BNA_LDFRAG(FA, FB, STAGE_IDX)                                                              \
+    do {                                                                                           \
+        const unsigned char *slot_ = smem + ((STAGE_IDX) % NSLOT) * C::SLOT;                       \
+        _Pragma("unroll") for (int blk = 0; blk < 4; blk++) {                                      \
+            const int ra = wr0 + blk * 32 + lrow, rbw = wc0 + blk * 32 + lrow;                     \
+            FA[blk][0] = *(const u32x4 *)(slot_ + ra * C::UNIT + ((c0 ^ swz<C::CH>(ra)) << 4));    \
+            FB[blk][0] = *(const u32x4 *)(slot_ + C::PANEL + rbw * C::UNIT + ((c0 ^ swz<C::CH>(rbw)) << 4)); \
+            if constexpr (PREC != 1) {                                                             \
+                FA[blk][1] = *(const u32x4 *)(slot_ + ra * C::UNIT + ((c1 ^ swz<C::CH>(ra)) << 4)); \
+                FB[blk][1] = *(const u32x4 *)(slot_ + C::PANEL + rbw * C::UNIT + ((c1 ^ swz<C::CH>(rbw)) << 4)); \
+            }                                                                                      \
+        }                                                                                          \
+    } while (0)
+#define BNA_MFMA(FA, FB)                                                                           \
+    do {                                                                                           \
+        if constexpr (PREC == 0) {                                                                 \
+            _Pragma("unroll") for (int q = 0; q < 8; q++)                                          \
+                _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                   \
+                    _Pragma("unroll") for (int cb = 0; cb < 4; cb++) {                             \
+                        const float av = __builtin_bit_cast(f32x4_t, FA[rb][q >> 2])[q & 3];       \
+                        const float bv = __builtin_bit_cast(f32x4_t, FB[cb][q >> 2])[q & 3];       \
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[rb][cb], 0, 0, 0); \
+                    }                                                                              \
+        } else {                                                                                   \
+            _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                       \
+                _Pragma("unroll") for (int cb = 0; cb < 4; cb++)                                   \
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                         \
+                        __builtin_bit_cast(bf16x8_t, FA[rb][0]), __builtin_bit_cast(bf16x8_t, FB[cb][0]), \
+                        acc[rb][cb], 0, 0, 0);                                                     \
+            if constexpr (PREC == 2) {                                                             \
+                _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                   \
+                    _Pragma("unroll") for (int cb = 0; cb < 4; cb++)                               \
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                     \
+                            __builtin_bit_cast(bf16x8_t, FA[rb][0]), __builtin_bit_cast(bf16x8_t, FB[cb][1]), \
+                            acc[rb][cb], 0, 0, 0);                                                 \
+                _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                   \
+                    _Pragma("unroll") for (int cb = 0; cb < 4; cb++)                               \
+                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                     \
+                            __builtin_bit_cast(bf16x8_t, FA[rb][1]), __builtin_bit_cast(bf16x8_t, FB[cb][0]), \
+                            acc[rb][cb], 0, 0, 0);                                                 \
+            }                                                                                      \
+        }                                                                                          \
+    } while (0)
 
-    const int lrow = lane & 31;
-    const int half = lane >> 5;
+    // Prologue: stages 0..3 in flight; wait for stage 0, read its fragments.
+    BNA_DMA(0);
+    BNA_DMA(1);
+    BNA_DMA(2);
+    BNA_DMA(3);
+    BNA_WAIT_VMCNT(3 * DMA_PER_STAGE);
+    __builtin_amdgcn_s_barrier();
+    BNA_LDFRAG(fa0, fb0, 0);
 
-    for (int kt = 0; kt < nk; kt++) {
-        const int cur = kt & 1;
-        // the last stage reloads itself so the loop body has no branch; the
-        // copy lands in the idle buffer and is never read
-        BNA_GLOAD(kt + 1 < nk ? kt + 1 : kt);
-        const unsigned char *base = smem + cur * C::STAGE;
-        if constexpr (PREC == 0) {
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                float4 af[4], bfv[4];
-                const int c = half * 4 + cc;  // lane half h covers k = 16h .. 16h+15
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-                    af[rb] = *(const float4 *)(base + lds_off<PREC>(0, 0, wr0 + rb * 32 + lrow, c));
-#pragma unroll
-                for (int cb = 0; cb < 4; cb++)
-                    bfv[cb] = *(const float4 *)(base + lds_off<PREC>(1, 0, wc0 + cb * 32 + lrow, c));
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-#pragma unroll
-                    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                        for (int cb = 0; cb < 4; cb++)
-                            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[rb][q], bfv[cb][q], acc[rb][cb], 0, 0, 0);
-            }
-        } else {
-#pragma unroll
-            for (int kk = 0; kk < 2; kk++) {
-                const int c = half * 2 + kk;  // lane half h covers k = 16h .. 16h+15
-                bf16x8_t ah[4], bh[4], al[4], bl[4];
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++) {
-                    ah[rb] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(base + lds_off<PREC>(0, 0, wr0 + rb * 32 + lrow, c)));
-                    if constexpr (PREC == 2)
-                        al[rb] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(base + lds_off<PREC>(0, 1, wr0 + rb * 32 + lrow, c)));
-                }
-#pragma unroll
-                for (int cb = 0; cb < 4; cb++) {
-                    bh[cb] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(base + lds_off<PREC>(1, 0, wc0 + cb * 32 + lrow, c)));
-                    if constexpr (PREC == 2)
-                        bl[cb] = __builtin_bit_cast(bf16x8_t, *(const uint4 *)(base + lds_off<PREC>(1, 1, wc0 + cb * 32 + lrow, c)));
-                }
-#pragma unroll
-                for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                    for (int cb = 0; cb < 4; cb++)
-                        acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bh[cb], acc[rb][cb], 0, 0, 0);
-                if constexpr (PREC == 2) {
-#pragma unroll
-                    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                        for (int cb = 0; cb < 4; cb++)
-                            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[rb], bl[cb], acc[rb][cb], 0, 0, 0);
-#pragma unroll
-                    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-                        for (int cb = 0; cb < 4; cb++)
-                            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[rb], bh[cb], acc[rb][cb], 0, 0, 0);
-                }
-            }
-        }
-        BNA_LSTORE(cur ^ 1);
-        __syncthreads();
+    // Iteration kt: fragments of stage kt are in set (kt & 1).  Wait for stage
+    // kt + 1 (two later stages may stay in flight), barrier (every wave has its
+    // stage-kt fragments, so slot kt % 4 is free), refill that slot with stage
+    // kt + 4, read stage kt + 1's fragments, then multiply stage kt.  L_pad is a
+    // multiple of 32, so nk is even and the loop runs whole pairs of stages.
+    for (int kt = 0; kt < nk; kt += 2) {
+        BNA_WAIT_VMCNT(2 * DMA_PER_STAGE);
+        __builtin_amdgcn_s_barrier();
+        BNA_DMA(kt + 4);
+        BNA_LDFRAG(fa1, fb1, kt + 1);
+        BNA_MFMA(fa0, fb0);
+        BNA_WAIT_VMCNT(2 * DMA_PER_STAGE);
+        __builtin_amdgcn_s_barrier();
+        BNA_DMA(kt + 5);
+        BNA_LDFRAG(fa0, fb0, kt + 2);
+        BNA_MFMA(fa1, fb1);
     }
-#undef BNA_GLOAD
-#undef BNA_LSTORE
+    // drain the ring before the epilogue reuses the LDS
+    BNA_WAIT_VMCNT(0);
+    __syncthreads();
+#undef BNA_DMA
+#undef BNA_LDFRAG
+#undef BNA_MFMA
 
     // ------------------------------------------------------------ epilogue
     EpiCtx ec;
@@ -354,7 +394,8 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
         int *rowpre = (int *)(colbits + TILE * 8);   // [256][8] slot of each bitmap word
         int *colpre = rowpre + TILE * 8;             // [256][8]
-        int *wsum = colpre + TILE * 8;               // [4]
+        KeptPair *lists = (KeptPair *)(colpre + TILE * 8);  // [4 waves][1024]
+        int *wsum = (int *)(lists + 4 * 1024);       // [4]
         int *wsum2 = wsum + 4;                       // [4]
         unsigned long long *sbase = (unsigned long long *)(wsum2 + 4);
         for (int k = tid; k < TILE * 8 * 2; k += CT_THREADS) rowbits[k] = 0;
@@ -405,7 +446,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         }
         if (total == 0) return;
         EmitOut eo{sd.entries, sd.cands, sd.cand_count, sd.cand_cap, base, total, fits};
-        BNA_FOR_BLOCKS(epi_emit, ec, rowbits, colbits, rowpre, colpre, eo);
+        BNA_FOR_BLOCKS(epi_emit, ec, rowbits, colbits, rowpre, colpre, eo, lists + wave * 1024);
     }
 }
 
@@ -413,16 +454,19 @@ template <int PREC, int HIST>
 static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, int64_t n_items, hipStream_t st)
 {
     using C = CorrCfg<PREC>;
-    constexpr size_t epi = (size_t)TILE * 8 * 4 * 4 + 64;
+    constexpr size_t epi = (size_t)TILE * 8 * 4 * 4 + 4 * 1024 * 8 + 64;
     constexpr size_t hst = (size_t)HIST_BINS * 4;
-    constexpr size_t need = 2 * (size_t)C::STAGE;
-    constexpr size_t shm = need > epi ? (need > hst ? need : hst) : (epi > hst ? epi : hst);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)corr_tile_kernel<PREC, HIST>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    constexpr size_t ring = (size_t)NSLOT * C::SLOT;
+    constexpr size_t shm = ring > epi ? (ring > hst ? ring : hst) : (epi > hst ? epi : hst);
+    static int attr_dev = -1;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (attr_dev != dev) {
+        e = hipFuncSetAttribute((const void *)corr_tile_kernel<PREC, HIST>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        attr_set = true;
+        attr_dev = dev;
     }
     const int64_t per_xcd = (n_items + 7) / 8;
     const int64_t grid = per_xcd * 8;
