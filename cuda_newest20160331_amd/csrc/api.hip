@@ -34,6 +34,7 @@ struct bna_ctx {
     bna_timings tm{};
     hipEvent_t ev[8] = {};
     std::map<std::string, DevBuf> ws;
+    int grid = 256;  // persistent correlation workgroups: CUs rounded up to a multiple of 8
     // tile table cache
     int tt_nb = -1;
     int64_t tt_count = 0;
@@ -195,6 +196,9 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
             return fail(nullptr, BNA_ERR_HIP, m);
         }
     }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        ctx->grid = (ncu + 7) / 8 * 8;
     if (hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) hbm_budget = (size_t)(0.9 * (double)fr);
@@ -470,6 +474,7 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     // capacities: start from what the workspace already holds
     int64_t ecap = std::max<int64_t>(1 << 20, (int64_t)(ctx->ws["entries"].bytes / sizeof(Entry)));
     int64_t ccap = std::max<int64_t>(1 << 16, (int64_t)(ctx->ws["cands"].bytes / sizeof(Candidate)));
+    int64_t ocap = std::max<int64_t>(1 << 16, (int64_t)(ctx->ws["kept_ovf"].bytes / sizeof(KeptPair)));
 
     SubjDesc h{};
     h.a0 = s->panel;
@@ -490,6 +495,7 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     h.frag_off = frag_off;
     h.entry_count = ctr;
     h.cand_count = ctr + 1;
+    h.ovf_count = ctr + 4;
     h.hist = nullptr;
 
     unsigned long long hc[8];
@@ -498,8 +504,12 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     for (int attempt = 0; attempt < 3; attempt++) {
         Entry *entries;
         Candidate *cands;
+        KeptPair *ovf;
         if ((rc = ws_get_t(ctx, "entries", (size_t)ecap, &entries))) return rc;
         if ((rc = ws_get_t(ctx, "cands", (size_t)ccap, &cands))) return rc;
+        if ((rc = ws_get_t(ctx, "kept_ovf", (size_t)ocap, &ovf))) return rc;
+        h.ovf = ovf;
+        h.ovf_cap = ocap;
         h.entries = entries;
         h.entry_cap = ecap;
         h.cands = cands;
@@ -515,18 +525,19 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ms_corr += ms;
-        const bool e_over = (int64_t)hc[0] > ecap, c_over = (int64_t)hc[1] > ccap;
-        if (!e_over && !c_over) break;
+        const bool e_over = (int64_t)hc[0] > ecap, c_over = (int64_t)hc[1] > ccap, o_over = (int64_t)hc[4] > ocap;
+        if (!e_over && !c_over && !o_over) break;
         if (attempt == 2) return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
         if (e_over) ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
         if (c_over) ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
+        if (o_over) ocap = (int64_t)hc[4];
     }
     const int64_t ncand = (int64_t)hc[1];
 
     // FP64 recheck
     HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
     if (ncand > 0)
-        HIPCHK(ctx, launch_recheck(s->z64, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
+        HIPCHK(ctx, launch_recheck(s->z64, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
                                    ctr + 2, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
 

@@ -6,6 +6,12 @@
 
 namespace bna {
 
+// A kept pair of a correlation tile (tile row, tile column, r).
+struct KeptPair {
+    uint16_t lr, lc;
+    float r;
+};
+
 // Correlation tile edge (voxels) and the K padding of the operand panels.
 constexpr int TILE = 256;
 constexpr int KPAD = 32;
@@ -40,9 +46,9 @@ struct SubjDesc {
     int32_t N, N_pad, L_pad, nb;
     float keep_thr;      // entry kept iff r > keep_thr
     float sure_thr;      // kept entry is sure iff r > sure_thr, else a candidate
-    int32_t *frag_orig;  // N x (nb + 1)
-    int32_t *frag_live;  // N x (nb + 1)
-    int64_t *frag_off;   // N x (nb + 1)
+    int32_t *frag_orig;  // (nb + 1) x N, fragment-major: [k * N + row]
+    int32_t *frag_live;  // (nb + 1) x N
+    int64_t *frag_off;   // (nb + 1) x N
     Entry *entries;      // compacted lists
     unsigned long long *entry_count; // running total (may exceed capacity)
     int64_t entry_cap;
@@ -50,6 +56,9 @@ struct SubjDesc {
     unsigned long long *cand_count;
     int64_t cand_cap;
     unsigned long long *hist; // histogram pass: HIST_BINS counters
+    struct KeptPair *ovf;     // kept-pair overflow pool for dense tiles
+    unsigned long long *ovf_count;
+    int64_t ovf_cap;          // entries in the pool
 };
 
 constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
@@ -81,7 +90,7 @@ hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const 
                        int64_t n_items, hipStream_t st);
 
 // FP64 recheck of candidates.
-hipError_t launch_recheck(const double *z64, int L, int nb, const Candidate *cands,
+hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
                           int32_t *frag_live, Entry *entries, unsigned long long *stats,
                           hipStream_t st);

@@ -38,11 +38,14 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) void gvoid;
 typedef __attribute__((address_space(3))) void lvoid;
 
 constexpr int CT_THREADS = 256;
 constexpr int NSLOT = 4;  // LDS ring of k16 stages
+// Epilogue LDS: bitmaps (16 KB), offsets (2 KB), scan partials, then the lists.
+constexpr int EPI_LIST_OFF = 20 * 1024;
 
 template <int PREC>
 struct CorrCfg {
@@ -98,17 +101,24 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
 struct EpiCtx {
     int N, gi0, gj0, wr0, wc0, half, lrow;
     bool diag;
-    float keep, sure;
+    float keep;
 };
 
-struct EmitOut {
-    Entry *entries;
-    Candidate *cands;
-    unsigned long long *cand_count;
-    int64_t cand_cap;
-    int64_t base;
-    int total;
-    bool fits;
+// Per-wave kept-pair list: the first LIST_LDS entries in LDS, the rest (dense
+// tiles only) in a global overflow area reserved on first need.
+constexpr int LIST_LDS = 2048;
+constexpr int WAVE_MAX = 128 * 128;  // a wave's whole sub-tile
+
+// Records are stored packed in one 64-bit word (tile row | tile column << 16
+// in the low half, r bits in the high half), with explicit LDS and global
+// address spaces.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(1))) uint64_t glb_u64;
+struct WaveList {
+    lds_u64 *lds;
+    glb_u64 *ovf;    // null until reserved
+    int n;           // entries so far (wave-uniform)
+    bool lost;       // the overflow pool was exhausted: the host enlarges it and reruns
 };
 
 __device__ __forceinline__ bool epi_in(const EpiCtx &c, int lr, int lc) {
@@ -130,79 +140,42 @@ __device__ __forceinline__ void epi_hist(const f32x16_t &a, const EpiCtx &c, uin
     }
 }
 
+// The single pass over the accumulators: marks the block's kept pairs in both
+// bitmaps and appends them to the wave's list at ballot-compacted positions.
 template <int RB, int CB>
-__device__ __forceinline__ void epi_mark(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits, uint32_t *colbits) {
-    const int lc = c.wc0 + CB * 32 + c.lrow;
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
-        if (a[e] > c.keep && epi_in(c, lr, lc)) {
-            atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
-            atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
-        }
-    }
-}
-
-// Kept pairs of one block are appended to the wave's LDS list (tile row,
-// tile column, r) with a ballot per element slot; the list is then walked
-// lane-parallel to compute both slots and write the entries.  Only compares
-// and ballots touch the accumulators, which keeps register pressure low.
-struct KeptPair {
-    uint16_t lr, lc;
-    float r;
-};
-
-template <int RB, int CB>
-__device__ __forceinline__ int epi_collect(const f32x16_t &a, const EpiCtx &c, KeptPair *list)
+__device__ __forceinline__ void epi_mark(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits, uint32_t *colbits,
+                                         WaveList &wl, KeptPair *ovf_pool, unsigned long long *ovf_count,
+                                         int64_t ovf_cap)
 {
+    if (wl.ovf == nullptr && !wl.lost && wl.n + 1024 > LIST_LDS) {
+        // this block could pass the LDS capacity: reserve the wave's worst case
+        unsigned long long base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(ovf_count, (unsigned long long)(WAVE_MAX - LIST_LDS));
+        base = __shfl(base, 0, 64);
+        if ((int64_t)(base + (WAVE_MAX - LIST_LDS)) <= ovf_cap)
+            wl.ovf = (glb_u64 *)(ovf_pool + base);
+        else
+            wl.lost = true;
+    }
     const int lc = c.wc0 + CB * 32 + c.lrow;
     const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
-    int n = 0;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
         const float v = a[e];
         const bool kp = v > c.keep && epi_in(c, lr, lc);
         const uint64_t m = __ballot(kp);
-        if (kp) list[n + __popcll(m & lt)] = KeptPair{(uint16_t)lr, (uint16_t)lc, v};
-        n += __popcll(m);
-    }
-    return n;
-}
-
-__device__ __forceinline__ void epi_emit_list(const KeptPair *list, int n, const EpiCtx &c, const uint32_t *rowbits,
-                                              const uint32_t *colbits, const int *rowpre, const int *colpre,
-                                              const EmitOut &o)
-{
-    for (int k = (int)(threadIdx.x & 63); k < n; k += 64) {
-        const KeptPair kp = list[k];
-        const int lr = kp.lr, lc = kp.lc;
-        const float v = kp.r;
-        const int rw = lr * 8 + (lc >> 5), cw = lc * 8 + (lr >> 5);
-        const int64_t p = o.base + rowpre[rw] + __popc(rowbits[rw] & ((1u << (lc & 31)) - 1u));
-        const int64_t q = o.base + o.total + colpre[cw] + __popc(colbits[cw] & ((1u << (lr & 31)) - 1u));
-        if (o.fits) {
-            o.entries[p] = Entry{c.gj0 + lc, v};
-            o.entries[q] = Entry{c.gi0 + lr, v};
+        if (kp) {
+            atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
+            atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
+            const int k = wl.n + __popcll(m & lt);
+            const uint64_t rec = (uint64_t)((uint32_t)lr | ((uint32_t)lc << 16)) |
+                                 ((uint64_t)__builtin_bit_cast(uint32_t, v) << 32);
+            if (k < LIST_LDS) wl.lds[k] = rec;
+            else if (wl.ovf) wl.ovf[k - LIST_LDS] = rec;
         }
-        if (!(v > c.sure)) {
-            const unsigned long long ci = atomicAdd(o.cand_count, 1ull);
-            if ((int64_t)ci < o.cand_cap) o.cands[ci] = Candidate{c.gi0 + lr, c.gj0 + lc, p, q, v, 0};
-        }
+        wl.n += __popcll(m);
     }
-}
-
-template <int RB, int CB>
-__device__ __forceinline__ void epi_emit(const f32x16_t &a, const EpiCtx &c, const uint32_t *rowbits,
-                                         const uint32_t *colbits, const int *rowpre, const int *colpre,
-                                         const EmitOut &o, KeptPair *list)
-{
-    const int n = epi_collect<RB, CB>(a, c, list);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    epi_emit_list(list, n, c, rowbits, colbits, rowpre, colpre, o);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
 }
 
 #define BNA_FOR_BLOCKS(F, ...)                                                                     \
@@ -216,7 +189,6 @@ __device__ __forceinline__ void epi_emit(const f32x16_t &a, const EpiCtx &c, con
         F<3, 0>(acc[3][0], __VA_ARGS__); F<3, 1>(acc[3][1], __VA_ARGS__);                          \
         F<3, 2>(acc[3][2], __VA_ARGS__); F<3, 3>(acc[3][3], __VA_ARGS__);                          \
     } while (0)
-
 
 template <int PREC, int HIST>
 __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc *__restrict__ subjs,
@@ -378,7 +350,6 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     ec.lrow = lrow;
     ec.diag = (I == J);
     ec.keep = sd.keep_thr;
-    ec.sure = sd.sure_thr;
 
     if constexpr (HIST) {
         uint32_t *hist = (uint32_t *)smem;
@@ -392,61 +363,77 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     } else {
         uint32_t *rowbits = (uint32_t *)smem;        // [256][8] tile-row bitmaps
         uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
-        int *rowpre = (int *)(colbits + TILE * 8);   // [256][8] slot of each bitmap word
-        int *colpre = rowpre + TILE * 8;             // [256][8]
-        KeptPair *lists = (KeptPair *)(colpre + TILE * 8);  // [4 waves][1024]
-        int *wsum = (int *)(lists + 4 * 1024);       // [4]
+        int *rowoff = (int *)(colbits + TILE * 8);   // [256] first slot of each tile row
+        int *coloff = rowoff + TILE;                 // [256] first slot of each tile column
+        int *wsum = coloff + TILE;                   // [4]
         int *wsum2 = wsum + 4;                       // [4]
         unsigned long long *sbase = (unsigned long long *)(wsum2 + 4);
+        KeptPair *lists = (KeptPair *)(smem + EPI_LIST_OFF);  // [4 waves][LIST_LDS]
         for (int k = tid; k < TILE * 8 * 2; k += CT_THREADS) rowbits[k] = 0;
         __syncthreads();
 
-        BNA_FOR_BLOCKS(epi_mark, ec, rowbits, colbits);
+        WaveList wl{(lds_u64 *)(lists + wave * LIST_LDS), nullptr, 0, false};
+        BNA_FOR_BLOCKS(epi_mark, ec, rowbits, colbits, wl, sd.ovf, sd.ovf_count, sd.ovf_cap);
         __syncthreads();
 
-        int rw[8], cw[8];
         int rc = 0, cc = 0;
 #pragma unroll
         for (int w = 0; w < 8; w++) {
-            rw[w] = __popc(rowbits[tid * 8 + w]);
-            cw[w] = __popc(colbits[tid * 8 + w]);
-            rc += rw[w];
-            cc += cw[w];
+            rc += __popc(rowbits[tid * 8 + w]);
+            cc += __popc(colbits[tid * 8 + w]);
         }
         int total = 0, total2 = 0;
         const int ro = block_excl_scan(rc, wsum, total);
         const int co = block_excl_scan(cc, wsum2, total2);
-        {
-            int a = ro, b2 = co;
-#pragma unroll
-            for (int w = 0; w < 8; w++) {
-                rowpre[tid * 8 + w] = a;
-                colpre[tid * 8 + w] = b2;
-                a += rw[w];
-                b2 += cw[w];
-            }
-        }
+        rowoff[tid] = ro;
+        coloff[tid] = co;
         if (tid == 0) *sbase = atomicAdd(sd.entry_count, (unsigned long long)(2 * total));
         __syncthreads();
         const int64_t base = (int64_t)*sbase;
         const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
 
-        const int nb1 = sd.nb + 1;
-        if (ec.gi0 + tid < ec.N) {
-            const int64_t f = (int64_t)(ec.gi0 + tid) * nb1 + (J + 1);
+        // fragment tables, [k][row] so that a tile's writes are contiguous
+        const int N = sd.N;
+        if (ec.gi0 + tid < N) {
+            const int64_t f = (int64_t)(J + 1) * N + ec.gi0 + tid;
             sd.frag_orig[f] = rc;
             sd.frag_live[f] = rc;
             sd.frag_off[f] = base + ro;
         }
-        if (ec.gj0 + tid < ec.N) {
-            const int64_t f = (int64_t)(ec.gj0 + tid) * nb1 + I;
+        if (ec.gj0 + tid < N) {
+            const int64_t f = (int64_t)I * N + ec.gj0 + tid;
             sd.frag_orig[f] = cc;
             sd.frag_live[f] = cc;
             sd.frag_off[f] = base + total + co;
         }
-        if (total == 0) return;
-        EmitOut eo{sd.entries, sd.cands, sd.cand_count, sd.cand_cap, base, total, fits};
-        BNA_FOR_BLOCKS(epi_emit, ec, rowbits, colbits, rowpre, colpre, eo, lists + wave * 1024);
+        if (wl.n == 0 || wl.lost) return;
+        if (wl.n > LIST_LDS) __builtin_amdgcn_s_waitcnt(0);  // overflow entries stored by this wave
+        const float sure = sd.sure_thr;
+        for (int k = lane; k < wl.n; k += 64) {
+            uint64_t raw;
+            if (k < LIST_LDS)
+                raw = wl.lds[k];
+            else
+                raw = __builtin_nontemporal_load(wl.ovf + (k - LIST_LDS));
+            const uint32_t lo32 = (uint32_t)raw;
+            const int lr = lo32 & 0xffff, lc = lo32 >> 16;
+            const float v = __builtin_bit_cast(float, (uint32_t)(raw >> 32));
+            int p = rowoff[lr], q = coloff[lc];
+            const int rwd = lc >> 5, cwd = lr >> 5;
+            for (int w = 0; w < rwd; w++) p += __popc(rowbits[lr * 8 + w]);
+            for (int w = 0; w < cwd; w++) q += __popc(colbits[lc * 8 + w]);
+            p += __popc(rowbits[lr * 8 + rwd] & ((1u << (lc & 31)) - 1u));
+            q += __popc(colbits[lc * 8 + cwd] & ((1u << (lr & 31)) - 1u));
+            const int64_t ps = base + p, qs = base + total + q;
+            if (fits) {
+                sd.entries[ps] = Entry{ec.gj0 + lc, v};
+                sd.entries[qs] = Entry{ec.gi0 + lr, v};
+            }
+            if (!(v > sure)) {
+                const unsigned long long ci = atomicAdd(sd.cand_count, 1ull);
+                if ((int64_t)ci < sd.cand_cap) sd.cands[ci] = Candidate{ec.gi0 + lr, ec.gj0 + lc, ps, qs, v, 0};
+            }
+        }
     }
 }
 
@@ -454,7 +441,7 @@ template <int PREC, int HIST>
 static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, int64_t n_items, hipStream_t st)
 {
     using C = CorrCfg<PREC>;
-    constexpr size_t epi = (size_t)TILE * 8 * 4 * 4 + 4 * 1024 * 8 + 64;
+    constexpr size_t epi = (size_t)EPI_LIST_OFF + 4 * LIST_LDS * sizeof(KeptPair);
     constexpr size_t hst = (size_t)HIST_BINS * 4;
     constexpr size_t ring = (size_t)NSLOT * C::SLOT;
     constexpr size_t shm = ring > epi ? (ring > hst ? ring : hst) : (epi > hst ? epi : hst);

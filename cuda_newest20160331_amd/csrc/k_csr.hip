@@ -19,7 +19,7 @@ namespace bna {
 // (lane q adds lane q + s for s = 32, 16, ..., 1).  Edge iff r64 > thr.
 // stats[0]: max |r_mfma - r64| as double bits, stats[1]: decisions that differ
 // from the MFMA value's own (r > thr) decision.
-__global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__ z64, int L, int nb,
+__global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__ z64, int N, int L, int nb,
                                                       const Candidate *__restrict__ cands,
                                                       const unsigned long long *__restrict__ cand_count,
                                                       int64_t cand_cap, double thr,
@@ -50,9 +50,8 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
             } else {
                 entries[cd.slot_row].col = -1;
                 entries[cd.slot_col].col = -1;
-                const int nb1 = nb + 1;
-                atomicSub(&frag_live[(int64_t)cd.i * nb1 + (cd.j / TILE) + 1], 1);
-                atomicSub(&frag_live[(int64_t)cd.j * nb1 + (cd.i / TILE)], 1);
+                atomicSub(&frag_live[(int64_t)((cd.j / TILE) + 1) * N + cd.i], 1);
+                atomicSub(&frag_live[(int64_t)(cd.i / TILE) * N + cd.j], 1);
             }
             const double err = fabs((double)cd.r - p);
             atomicMax(&stats[0], (unsigned long long)__builtin_bit_cast(unsigned long long, err));
@@ -61,20 +60,16 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
     }
 }
 
-// One wave per row: sum of the live fragment counts.
+// One thread per row: sum of the live fragment counts (fragment-major tables,
+// so consecutive threads read consecutive words).
 __global__ __launch_bounds__(256) void rowlen_kernel(const int32_t *__restrict__ frag_live, int N, int nb,
                                                      int64_t *__restrict__ rowlen)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= N) return;
-    const int nb1 = nb + 1;
-    const int32_t *f = frag_live + row * nb1;
     int64_t s = 0;
-    for (int k = lane; k < nb1; k += 64) s += f[k];
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane == 0) rowlen[row] = s;
+    for (int k = 0; k <= nb; k++) s += frag_live[(int64_t)k * N + row];
+    rowlen[row] = s;
 }
 
 // Single-workgroup exclusive scan over N row lengths (N is at most a few
@@ -110,9 +105,10 @@ __global__ __launch_bounds__(SCAN_T) void scan_kernel(const int64_t *__restrict_
     }
 }
 
-// One wave per row: lane k takes fragment k (in chunks of 64), a wave scan of
-// the live counts gives each fragment its destination, and the lane copies the
-// fragment's entries, skipping rejected ones.
+// One thread per row walks its nb + 1 fragments in order and copies their
+// entries, skipping the ones the recheck rejected.  Fragment k of row i lists
+// neighbours in tile k (k <= I(i)) or tile k - 1 (k > I(i)), each ascending,
+// so the copies come out column-sorted.
 __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ frag_orig,
                                                     const int32_t *__restrict__ frag_live,
                                                     const int64_t *__restrict__ frag_off,
@@ -120,35 +116,22 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
                                                     const int64_t *__restrict__ R64, int N, int nb,
                                                     int32_t *__restrict__ Cout, float *__restrict__ Vout)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= N) return;
-    const int nb1 = nb + 1;
-    int64_t run = R64[row];
-    for (int k0 = 0; k0 < nb1; k0 += 64) {
-        const int k = k0 + lane;
-        int live = 0, orig = 0;
-        int64_t off = 0;
-        if (k < nb1) {
-            const int64_t f = row * nb1 + k;
-            live = frag_live[f];
-            orig = frag_orig[f];
-            off = frag_off[f];
-        }
-        int inc = live;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            int t = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += t;
-        }
-        const int tot = __shfl(inc, 63, 64);
-        int64_t dst = run + inc - live;
+    int64_t dst = R64[row];
+    for (int k = 0; k <= nb; k++) {
+        const int64_t f = (int64_t)k * N + row;
+        const int orig = frag_orig[f];
+        if (orig == 0) continue;
+        const int live = frag_live[f];
+        const int64_t off = frag_off[f];
         if (live == orig) {
             for (int e = 0; e < orig; e++) {
                 const Entry en = entries[off + e];
                 Cout[dst + e] = en.col;
                 if (Vout) Vout[dst + e] = en.r;
             }
+            dst += orig;
         } else {
             for (int e = 0; e < orig; e++) {
                 const Entry en = entries[off + e];
@@ -159,25 +142,23 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
                 }
             }
         }
-        run += tot;
     }
 }
 
-hipError_t launch_recheck(const double *z64, int L, int nb, const Candidate *cands,
+hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
                           int32_t *frag_live, Entry *entries, unsigned long long *stats, hipStream_t st)
 {
     // enough waves to cover the device; candidates are grid-strided
-    hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, z64, L, nb, cands, cand_count, cand_cap,
+    hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, z64, N, L, nb, cands, cand_count, cand_cap,
                        thr, frag_live, entries, stats);
     return hipGetLastError();
 }
 
 hipError_t launch_rowlen(const int32_t *frag_live, int N, int nb, int64_t *rowlen, hipStream_t st)
 {
-    const int64_t threads = (int64_t)N * 64;
-    hipLaunchKernelGGL(rowlen_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, frag_live, N,
-                       nb, rowlen);
+    hipLaunchKernelGGL(rowlen_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, frag_live, N, nb,
+                       rowlen);
     return hipGetLastError();
 }
 
@@ -191,8 +172,7 @@ hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, cons
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C, float *V,
                         hipStream_t st)
 {
-    const int64_t threads = (int64_t)N * 64;
-    hipLaunchKernelGGL(place_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, frag_orig,
+    hipLaunchKernelGGL(place_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, frag_orig,
                        frag_live, frag_off, entries, R64, N, nb, C, V);
     return hipGetLastError();
 }
