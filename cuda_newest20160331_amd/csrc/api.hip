@@ -499,15 +499,35 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     h.hist = nullptr;
 
     unsigned long long hc[8];
-    float ms_corr = 0.f;
+    float ms_corr = 0.f, ms_re = 0.f;
     int64_t launches = 0;
-    for (int attempt = 0; attempt < 3; attempt++) {
+    int64_t nnz = 0;
+
+    // row lengths, scan and placement buffers
+    int64_t *rowlen, *R64;
+    if ((rc = ws_get_t(ctx, "rowlen", (size_t)N, &rowlen))) return rc;
+    if ((rc = ws_get_t(ctx, "R64", (size_t)N + 1, &R64))) return rc;
+    int32_t *orig_rm, *live_rm;
+    int64_t *off_rm;
+    if ((rc = ws_get_t(ctx, "frag_orig_rm", nfrag, &orig_rm))) return rc;
+    if ((rc = ws_get_t(ctx, "frag_live_rm", nfrag, &live_rm))) return rc;
+    if ((rc = ws_get_t(ctx, "frag_off_rm", nfrag, &off_rm))) return rc;
+    int32_t *R32 = nullptr;
+    if ((rc = out_alloc(ctx, (void **)&R32, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
+
+    // One pass of correlation, recheck, row lengths and scan, then one host
+    // read of the counters: when any list overflowed its capacity, grow it and
+    // run the pass again (the recheck skips slots past the capacity).
+    for (int attempt = 0;; attempt++) {
         Entry *entries;
         Candidate *cands;
         KeptPair *ovf;
-        if ((rc = ws_get_t(ctx, "entries", (size_t)ecap, &entries))) return rc;
-        if ((rc = ws_get_t(ctx, "cands", (size_t)ccap, &cands))) return rc;
-        if ((rc = ws_get_t(ctx, "kept_ovf", (size_t)ocap, &ovf))) return rc;
+        if ((rc = ws_get_t(ctx, "entries", (size_t)ecap, &entries)) ||
+            (rc = ws_get_t(ctx, "cands", (size_t)ccap, &cands)) ||
+            (rc = ws_get_t(ctx, "kept_ovf", (size_t)ocap, &ovf))) {
+            (void)hipFree(R32);
+            return rc;
+        }
         h.ovf = ovf;
         h.ovf_cap = ocap;
         h.entries = entries;
@@ -520,39 +540,31 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, ntiles, ctx->stream));
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        HIPCHK(ctx, launch_recheck(s->z64, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
+                                   h.entry_cap, ctr + 2, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        HIPCHK(ctx, launch_frag_rows(frag_orig, frag_live, frag_off, N, nb, orig_rm, live_rm, off_rm, rowlen,
+                                     ctx->stream));
+        HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(&nnz, R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ms_corr += ms;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
+        ms_re += ms;
         const bool e_over = (int64_t)hc[0] > ecap, c_over = (int64_t)hc[1] > ccap, o_over = (int64_t)hc[4] > ocap;
         if (!e_over && !c_over && !o_over) break;
-        if (attempt == 2) return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
+        if (attempt == 2) {
+            (void)hipFree(R32);
+            return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
+        }
         if (e_over) ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
         if (c_over) ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
         if (o_over) ocap = (int64_t)hc[4];
     }
     const int64_t ncand = (int64_t)hc[1];
-
-    // FP64 recheck
-    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-    if (ncand > 0)
-        HIPCHK(ctx, launch_recheck(s->z64, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
-                                   ctr + 2, ctx->stream));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-
-    // rows, scan, placement
-    int64_t *rowlen, *R64;
-    if ((rc = ws_get_t(ctx, "rowlen", (size_t)N, &rowlen))) return rc;
-    if ((rc = ws_get_t(ctx, "R64", (size_t)N + 1, &R64))) return rc;
-    int32_t *R32 = nullptr;
-    if ((rc = out_alloc(ctx, (void **)&R32, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
-    HIPCHK(ctx, launch_rowlen(frag_live, N, nb, rowlen, ctx->stream));
-    HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, ctx->stream));
-    int64_t nnz = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&nnz, R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (nnz > 0x7fffffffLL) {
         (void)hipFree(R32);
         return fail(ctx, BNA_ERR_OVERFLOW, "nnz exceeds int32 (" + std::to_string(nnz) + ")");
@@ -568,13 +580,12 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         (void)hipFree(Cd);
         return rc;
     }
-    HIPCHK(ctx, launch_place(frag_orig, frag_live, frag_off, h.entries, R64, N, nb, Cd, Vd, ctx->stream));
+    HIPCHK(ctx, launch_place(orig_rm, live_rm, off_rm, h.entries, R64, N, nb, Cd, Vd, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 
-    float ms_re = 0.f, ms_csr = 0.f, ms_tot = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms_re, ctx->ev[4], ctx->ev[5]));
-    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[5], ctx->ev[6]));
+    float ms_csr = 0.f, ms_tot = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[4], ctx->ev[6]));
     HIPCHK(ctx, hipEventElapsedTime(&ms_tot, timing_from_norm ? ctx->ev[0] : ctx->ev[2], ctx->ev[6]));
     ctx->tm.corr_ms = ms_corr;
     ctx->tm.recheck_ms = ms_re;

@@ -90,14 +90,18 @@ hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const 
                        int64_t n_items, hipStream_t st);
 
 // FP64 recheck of candidates.
+// Slots at or past entry_cap (overflowed lists) are skipped.
 hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
-                          int32_t *frag_live, Entry *entries, unsigned long long *stats,
+                          int32_t *frag_live, Entry *entries, int64_t entry_cap, unsigned long long *stats,
                           hipStream_t st);
 
 // Row lengths, exclusive scan into R, placement into C/V.
-hipError_t launch_rowlen(const int32_t *frag_live, int N, int nb, int64_t *rowlen, hipStream_t st);
+// Fragment tables [k][row] -> [row][k], plus row lengths (sums of live counts).
+hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, const int64_t *off_km, int N, int nb,
+                            int32_t *orig_rm, int32_t *live_rm, int64_t *off_rm, int64_t *rowlen, hipStream_t st);
 hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, hipStream_t st);
+// frag_* here are the row-major tables.
 hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C,
                         float *V, hipStream_t st);

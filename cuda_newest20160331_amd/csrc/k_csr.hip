@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
                                                       const unsigned long long *__restrict__ cand_count,
                                                       int64_t cand_cap, double thr,
                                                       int32_t *__restrict__ frag_live, Entry *__restrict__ entries,
-                                                      unsigned long long *__restrict__ stats)
+                                                      int64_t entry_cap, unsigned long long *__restrict__ stats)
 {
     const int64_t n = (int64_t)min((unsigned long long)cand_cap, *cand_count);
     const int lane = threadIdx.x & 63;
@@ -34,7 +34,9 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
         const Candidate cd = cands[c];
         const double *zi = z64 + (int64_t)cd.i * L;
         const double *zj = z64 + (int64_t)cd.j * L;
+        if (cd.slot_row >= entry_cap || cd.slot_col >= entry_cap) continue;  // lists overflowed: rerun follows
         double p = 0.0;
+#pragma unroll 4
         for (int l = lane; l < L; l += 64) p = __builtin_fma(zi[l], zj[l], p);
 #pragma unroll
         for (int s = 32; s >= 1; s >>= 1) {
@@ -60,16 +62,60 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
     }
 }
 
-// One thread per row: sum of the live fragment counts (fragment-major tables,
-// so consecutive threads read consecutive words).
-__global__ __launch_bounds__(256) void rowlen_kernel(const int32_t *__restrict__ frag_live, int N, int nb,
-                                                     int64_t *__restrict__ rowlen)
+// Fragment tables are written fragment-major ([k][row]) by the tile kernel,
+// where each tile's writes are contiguous.  This pass turns them row-major
+// ([row][k]) through LDS, 64 x 64 at a time, so the placement reads a row's
+// fragments contiguously, and adds each row's live counts into rowlen (which
+// must start at zero; integer atomics, so the sum is exact).
+constexpr int TR = 64;
+__global__ __launch_bounds__(256) void frag_rows_kernel(const int32_t *__restrict__ orig_km,
+                                                        const int32_t *__restrict__ live_km,
+                                                        const int64_t *__restrict__ off_km, int N, int nb1,
+                                                        int32_t *__restrict__ orig_rm, int32_t *__restrict__ live_rm,
+                                                        int64_t *__restrict__ off_rm,
+                                                        unsigned long long *__restrict__ rowlen)
 {
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= N) return;
-    int64_t s = 0;
-    for (int k = 0; k <= nb; k++) s += frag_live[(int64_t)k * N + row];
-    rowlen[row] = s;
+    __shared__ int64_t t64[TR][TR + 1];
+    __shared__ int32_t t32[TR][TR + 1];
+    const int k0 = blockIdx.x * TR, r0 = blockIdx.y * TR;
+    const int tx = threadIdx.x & (TR - 1), ty = threadIdx.x / TR;  // 64 x 4
+    // offsets
+    for (int k = ty; k < TR; k += 4) {
+        const int gk = k0 + k, gr = r0 + tx;
+        t64[k][tx] = (gk < nb1 && gr < N) ? off_km[(int64_t)gk * N + gr] : 0;
+    }
+    __syncthreads();
+    for (int r = ty; r < TR; r += 4) {
+        const int gr = r0 + r, gk = k0 + tx;
+        if (gr < N && gk < nb1) off_rm[(int64_t)gr * nb1 + gk] = t64[tx][r];
+    }
+    __syncthreads();
+    // original counts
+    for (int k = ty; k < TR; k += 4) {
+        const int gk = k0 + k, gr = r0 + tx;
+        t32[k][tx] = (gk < nb1 && gr < N) ? orig_km[(int64_t)gk * N + gr] : 0;
+    }
+    __syncthreads();
+    for (int r = ty; r < TR; r += 4) {
+        const int gr = r0 + r, gk = k0 + tx;
+        if (gr < N && gk < nb1) orig_rm[(int64_t)gr * nb1 + gk] = t32[tx][r];
+    }
+    __syncthreads();
+    // live counts, and their per-row partial sums
+    for (int k = ty; k < TR; k += 4) {
+        const int gk = k0 + k, gr = r0 + tx;
+        t32[k][tx] = (gk < nb1 && gr < N) ? live_km[(int64_t)gk * N + gr] : 0;
+    }
+    __syncthreads();
+    for (int r = ty; r < TR; r += 4) {
+        const int gr = r0 + r, gk = k0 + tx;
+        const int v = t32[tx][r];
+        if (gr < N && gk < nb1) live_rm[(int64_t)gr * nb1 + gk] = v;
+        int s = v;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+        if (tx == 0 && gr < N && s) atomicAdd(&rowlen[gr], (unsigned long long)s);
+    }
 }
 
 // Single-workgroup exclusive scan over N row lengths (N is at most a few
@@ -105,10 +151,11 @@ __global__ __launch_bounds__(SCAN_T) void scan_kernel(const int64_t *__restrict_
     }
 }
 
-// One thread per row walks its nb + 1 fragments in order and copies their
-// entries, skipping the ones the recheck rejected.  Fragment k of row i lists
-// neighbours in tile k (k <= I(i)) or tile k - 1 (k > I(i)), each ascending,
-// so the copies come out column-sorted.
+// One wave per row: lane k takes fragment k (in chunks of 64) of the row-major
+// tables, a wave scan of the live counts gives each fragment its destination,
+// and the lane copies the fragment's entries, skipping rejected ones.
+// Fragment k of row i lists neighbours in tile k (k <= I(i)) or tile k - 1
+// (k > I(i)), each ascending, so the copies come out column-sorted.
 __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ frag_orig,
                                                     const int32_t *__restrict__ frag_live,
                                                     const int64_t *__restrict__ frag_off,
@@ -116,22 +163,37 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
                                                     const int64_t *__restrict__ R64, int N, int nb,
                                                     int32_t *__restrict__ Cout, float *__restrict__ Vout)
 {
-    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (row >= N) return;
-    int64_t dst = R64[row];
-    for (int k = 0; k <= nb; k++) {
-        const int64_t f = (int64_t)k * N + row;
-        const int orig = frag_orig[f];
-        if (orig == 0) continue;
-        const int live = frag_live[f];
-        const int64_t off = frag_off[f];
+    const int nb1 = nb + 1;
+    int64_t run = R64[row];
+    for (int k0 = 0; k0 < nb1; k0 += 64) {
+        const int k = k0 + lane;
+        int live = 0, orig = 0;
+        int64_t off = 0;
+        if (k < nb1) {
+            const int64_t f = row * nb1 + k;
+            orig = frag_orig[f];
+            if (orig) {
+                live = frag_live[f];
+                off = frag_off[f];
+            }
+        }
+        int inc = live;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            int t = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += t;
+        }
+        const int tot = __shfl(inc, 63, 64);
+        int64_t dst = run + inc - live;
         if (live == orig) {
             for (int e = 0; e < orig; e++) {
                 const Entry en = entries[off + e];
                 Cout[dst + e] = en.col;
                 if (Vout) Vout[dst + e] = en.r;
             }
-            dst += orig;
         } else {
             for (int e = 0; e < orig; e++) {
                 const Entry en = entries[off + e];
@@ -142,23 +204,30 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
                 }
             }
         }
+        run += tot;
     }
 }
 
 hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
-                          int32_t *frag_live, Entry *entries, unsigned long long *stats, hipStream_t st)
+                          int32_t *frag_live, Entry *entries, int64_t entry_cap, unsigned long long *stats,
+                          hipStream_t st)
 {
     // enough waves to cover the device; candidates are grid-strided
     hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, z64, N, L, nb, cands, cand_count, cand_cap,
-                       thr, frag_live, entries, stats);
+                       thr, frag_live, entries, entry_cap, stats);
     return hipGetLastError();
 }
 
-hipError_t launch_rowlen(const int32_t *frag_live, int N, int nb, int64_t *rowlen, hipStream_t st)
+hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, const int64_t *off_km, int N, int nb,
+                            int32_t *orig_rm, int32_t *live_rm, int64_t *off_rm, int64_t *rowlen, hipStream_t st)
 {
-    hipLaunchKernelGGL(rowlen_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, frag_live, N, nb,
-                       rowlen);
+    hipError_t e = hipMemsetAsync(rowlen, 0, sizeof(int64_t) * (size_t)N, st);
+    if (e != hipSuccess) return e;
+    const int nb1 = nb + 1;
+    hipLaunchKernelGGL(frag_rows_kernel, dim3((unsigned)((nb1 + TR - 1) / TR), (unsigned)((N + TR - 1) / TR)),
+                       dim3(256), 0, st, orig_km, live_km, off_km, N, nb1, orig_rm, live_rm, off_rm,
+                       (unsigned long long *)rowlen);
     return hipGetLastError();
 }
 
@@ -172,7 +241,8 @@ hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, cons
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C, float *V,
                         hipStream_t st)
 {
-    hipLaunchKernelGGL(place_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, frag_orig,
+    const int64_t threads = (int64_t)N * 64;
+    hipLaunchKernelGGL(place_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, frag_orig,
                        frag_live, frag_off, entries, R64, N, nb, C, V);
     return hipGetLastError();
 }
