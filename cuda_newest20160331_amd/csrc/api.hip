@@ -44,7 +44,8 @@ struct bna_series {
     int N = 0, L = 0, N_pad = 0, L_pad = 0, layout = 0;
     float *x = nullptr;
     bool own_x = false;
-    double *z64 = nullptr, *mu = nullptr, *sd = nullptr;
+    float *xv = nullptr;  // time-major input only: the series voxel-major, row stride L_pad
+    double *mu = nullptr, *sd = nullptr;
     uint8_t *valid = nullptr;
     unsigned char *panel = nullptr;  // k-blocked MFMA operands of prec_ready
     size_t panel_bytes = 0;
@@ -278,13 +279,13 @@ int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, in
 
 static void series_free_norm(bna_ctx *ctx, bna_series *s)
 {
-    const size_t NL = (size_t)s->N * s->L;
-    dev_free(ctx, s->z64, NL * 8);
+    dev_free(ctx, s->xv, (size_t)s->N * s->L_pad * 4);
     dev_free(ctx, s->mu, (size_t)s->N * 8);
     dev_free(ctx, s->sd, (size_t)s->N * 8);
     dev_free(ctx, s->valid, (size_t)s->N);
     dev_free(ctx, s->panel, s->panel_bytes);
-    s->z64 = s->mu = s->sd = nullptr;
+    s->xv = nullptr;
+    s->mu = s->sd = nullptr;
     s->valid = nullptr;
     s->panel = nullptr;
     s->panel_bytes = 0;
@@ -306,10 +307,10 @@ int bna_series_destroy(bna_ctx *ctx, bna_series *s)
 static int series_normalize(bna_ctx *ctx, bna_series *s, int prec)
 {
     if (prec < BNA_FP32 || prec > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bad precision");
-    const size_t NL = (size_t)s->N * s->L;
     int rc;
-    if (!s->z64) {
-        if ((rc = dev_alloc(ctx, (void **)&s->z64, NL * 8))) return rc;
+    if (!s->mu) {
+        if (s->layout == BNA_TIME_MAJOR && (rc = dev_alloc(ctx, (void **)&s->xv, (size_t)s->N * s->L_pad * 4)))
+            return rc;
         if ((rc = dev_alloc(ctx, (void **)&s->mu, (size_t)s->N * 8))) return rc;
         if ((rc = dev_alloc(ctx, (void **)&s->sd, (size_t)s->N * 8))) return rc;
         if ((rc = dev_alloc(ctx, (void **)&s->valid, (size_t)s->N))) return rc;
@@ -325,7 +326,7 @@ static int series_normalize(bna_ctx *ctx, bna_series *s, int prec)
         if ((rc = dev_alloc(ctx, (void **)&s->panel, pb))) return rc;
         s->panel_bytes = pb;
     }
-    HIPCHK(ctx, launch_znorm(s->x, s->layout, s->N, s->L, s->N_pad, s->L_pad, s->mu, s->sd, s->z64, s->valid,
+    HIPCHK(ctx, launch_znorm(s->x, s->layout, s->N, s->L, s->N_pad, s->L_pad, s->mu, s->sd, s->xv, s->valid,
                              prec, s->panel, ctx->stream));
     s->prec_ready = prec;
     return BNA_OK;
@@ -349,9 +350,21 @@ int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision)
 int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *mu, double *sd)
 {
     if (!ctx || !s) return fail(ctx, BNA_ERR_ARG, "bna_series_download: NULL argument");
-    if (!s->z64) return fail(ctx, BNA_ERR_ARG, "bna_series_download: series not normalised");
+    if (!s->mu) return fail(ctx, BNA_ERR_ARG, "bna_series_download: series not normalised");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (z64) HIPCHK(ctx, hipMemcpyAsync(z64, s->z64, (size_t)s->N * s->L * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (z64) {
+        // z is not kept resident; it is materialised from the series for the copy
+        const size_t zb = (size_t)s->N * s->L * 8;
+        double *zd = nullptr;
+        int rc = dev_alloc(ctx, (void **)&zd, zb);
+        if (rc) return rc;
+        hipError_t e = launch_zmat(s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->N, s->L, s->mu, s->sd, zd,
+                                   ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(z64, zd, zb, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        dev_free(ctx, zd, zb);
+        if (e != hipSuccess) return fail(ctx, BNA_ERR_HIP, std::string("bna_series_download: ") + hipGetErrorString(e));
+    }
     if (mu) HIPCHK(ctx, hipMemcpyAsync(mu, s->mu, (size_t)s->N * 8, hipMemcpyDeviceToHost, ctx->stream));
     if (sd) HIPCHK(ctx, hipMemcpyAsync(sd, s->sd, (size_t)s->N * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -504,9 +517,10 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     int64_t nnz = 0;
 
     // row lengths, scan and placement buffers
-    int64_t *rowlen, *R64;
+    int64_t *rowlen, *R64, *bsum;
     if ((rc = ws_get_t(ctx, "rowlen", (size_t)N, &rowlen))) return rc;
     if ((rc = ws_get_t(ctx, "R64", (size_t)N + 1, &R64))) return rc;
+    if ((rc = ws_get_t(ctx, "scan_bsum", (size_t)scan_blocks(N) + 1, &bsum))) return rc;
     int32_t *orig_rm, *live_rm;
     int64_t *off_rm;
     if ((rc = ws_get_t(ctx, "frag_orig_rm", nfrag, &orig_rm))) return rc;
@@ -540,12 +554,12 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, ntiles, ctx->stream));
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        HIPCHK(ctx, launch_recheck(s->z64, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
+        HIPCHK(ctx, launch_recheck(s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->mu, s->sd, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
                                    h.entry_cap, ctr + 2, ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
         HIPCHK(ctx, launch_frag_rows(frag_orig, frag_live, frag_off, N, nb, orig_rm, live_rm, off_rm, rowlen,
                                      ctx->stream));
-        HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, ctx->stream));
+        HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, bsum, ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(&nnz, R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

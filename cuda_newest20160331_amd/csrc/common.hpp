@@ -79,10 +79,14 @@ __host__ __device__ inline float bf16_to_f32(uint16_t h) {
 // fp32 z[16] = 64, bf16 hi[16] = 32, bf16x3 hi[16] lo[16] = 64.
 __host__ __device__ inline int panel_unit(int prec) { return prec == 1 ? 32 : 64; }
 
-// z-normalisation (kernel a): mean, norm, z64 and the k-blocked MFMA operand panel.
+// z-normalisation (kernel a): mean, norm, validity and the k-blocked MFMA operand
+// panel; for time-major input (layout 0) also xv, the series voxel-major with row
+// stride L_pad.
 hipError_t launch_znorm(const float *x, int layout, int N, int L, int N_pad, int L_pad, double *mu,
-                        double *sd, double *z64, uint8_t *valid, int prec, unsigned char *panel,
-                        hipStream_t st);
+                        double *sd, float *xv, uint8_t *valid, int prec, unsigned char *panel, hipStream_t st);
+// z64 [N][L] from a voxel-major series with row stride ld.
+hipError_t launch_zmat(const float *xs, int64_t ld, int N, int L, const double *mu, const double *sd, double *z64,
+                       hipStream_t st);
 
 // Correlation + threshold + compaction over a tile table (kernel b + first half of c).
 // prec: 0 FP32, 1 BF16, 2 BF16X3.  hist_mode: 1 = histogram epilogue.
@@ -91,7 +95,9 @@ hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const 
 
 // FP64 recheck of candidates.
 // Slots at or past entry_cap (overflowed lists) are skipped.
-hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candidate *cands,
+// z is recomputed from the voxel-major series xs (row stride ld), mu and sd.
+hipError_t launch_recheck(const float *xs, int64_t ld, const double *mu, const double *sd, int N, int L, int nb,
+                          const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
                           int32_t *frag_live, Entry *entries, int64_t entry_cap, unsigned long long *stats,
                           hipStream_t st);
@@ -100,7 +106,9 @@ hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candida
 // Fragment tables [k][row] -> [row][k], plus row lengths (sums of live counts).
 hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, const int64_t *off_km, int N, int nb,
                             int32_t *orig_rm, int32_t *live_rm, int64_t *off_rm, int64_t *rowlen, hipStream_t st);
-hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, hipStream_t st);
+// bsum: scan_blocks(N) + 1 int64 of scratch.
+hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, int64_t *bsum, hipStream_t st);
+int scan_blocks(int N);
 // frag_* here are the row-major tables.
 hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C,

@@ -14,12 +14,16 @@
 
 namespace bna {
 
-// One wave per candidate.  r64 is evaluated in the FP64 oracle's order: lane q
+// One wave per candidate.  z[l] = (x[l] - mean) / norm is recomputed in double,
+// the expression the normalisation evaluates, so the values equal it bit for
+// bit.  r64 is evaluated in the FP64 oracle's order: lane q
 // accumulates l = q, q + 64, ... with fused multiply-add, then a halving tree
 // (lane q adds lane q + s for s = 32, 16, ..., 1).  Edge iff r64 > thr.
 // stats[0]: max |r_mfma - r64| as double bits, stats[1]: decisions that differ
 // from the MFMA value's own (r > thr) decision.
-__global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__ z64, int N, int L, int nb,
+__global__ __launch_bounds__(256) void recheck_kernel(const float *__restrict__ xs, int64_t ld,
+                                                      const double *__restrict__ mu,
+                                                      const double *__restrict__ sd, int N, int L, int nb,
                                                       const Candidate *__restrict__ cands,
                                                       const unsigned long long *__restrict__ cand_count,
                                                       int64_t cand_cap, double thr,
@@ -28,16 +32,23 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
 {
     const int64_t n = (int64_t)min((unsigned long long)cand_cap, *cand_count);
     const int lane = threadIdx.x & 63;
+    double wmax = 0.0;       // this wave's max |r_mfma - r64| (lane 0)
+    unsigned long long wflips = 0;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t c = wave; c < n; c += nwaves) {
         const Candidate cd = cands[c];
-        const double *zi = z64 + (int64_t)cd.i * L;
-        const double *zj = z64 + (int64_t)cd.j * L;
         if (cd.slot_row >= entry_cap || cd.slot_col >= entry_cap) continue;  // lists overflowed: rerun follows
+        const float *xi = xs + (int64_t)cd.i * ld;
+        const float *xj = xs + (int64_t)cd.j * ld;
+        const double mi = mu[cd.i], si = sd[cd.i], mj = mu[cd.j], sj = sd[cd.j];
         double p = 0.0;
 #pragma unroll 4
-        for (int l = lane; l < L; l += 64) p = __builtin_fma(zi[l], zj[l], p);
+        for (int l = lane; l < L; l += 64) {
+            const double zi = ((double)xi[l] - mi) / si;
+            const double zj = ((double)xj[l] - mj) / sj;
+            p = __builtin_fma(zi, zj, p);
+        }
 #pragma unroll
         for (int s = 32; s >= 1; s >>= 1) {
             double o = __shfl_down(p, s, 64);
@@ -55,10 +66,13 @@ __global__ __launch_bounds__(256) void recheck_kernel(const double *__restrict__
                 atomicSub(&frag_live[(int64_t)((cd.j / TILE) + 1) * N + cd.i], 1);
                 atomicSub(&frag_live[(int64_t)(cd.i / TILE) * N + cd.j], 1);
             }
-            const double err = fabs((double)cd.r - p);
-            atomicMax(&stats[0], (unsigned long long)__builtin_bit_cast(unsigned long long, err));
-            if (edge != raw) atomicAdd(&stats[1], 1ull);
+            wmax = fmax(wmax, fabs((double)cd.r - p));
+            wflips += (edge != raw);
         }
+    }
+    if (lane == 0 && (wmax > 0.0 || wflips)) {
+        atomicMax(&stats[0], (unsigned long long)__builtin_bit_cast(unsigned long long, wmax));
+        if (wflips) atomicAdd(&stats[1], wflips);
     }
 }
 
@@ -118,36 +132,100 @@ __global__ __launch_bounds__(256) void frag_rows_kernel(const int32_t *__restric
     }
 }
 
-// Single-workgroup exclusive scan over N row lengths (N is at most a few
-// hundred thousand): each thread scans a contiguous run, then the run totals
-// are scanned across the block.
-constexpr int SCAN_T = 1024;
-__global__ __launch_bounds__(SCAN_T) void scan_kernel(const int64_t *__restrict__ rowlen, int N,
-                                                      int64_t *__restrict__ R64, int32_t *__restrict__ R32)
+// Exclusive scan of the N row lengths into R, in three passes: per-block sums
+// (SCAN_E elements per block), a single-block scan of those sums, then each
+// block scans its own elements from its offset.  R[N] is the total.
+constexpr int SCAN_T = 256;
+constexpr int SCAN_PER = 8;
+constexpr int SCAN_E = SCAN_T * SCAN_PER;
+
+__device__ __forceinline__ int64_t block_scan_excl64(int64_t v, int64_t *sh, int64_t &total)
 {
-    __shared__ int64_t part[SCAN_T];
-    const int t = threadIdx.x;
-    const int per = (N + SCAN_T - 1) / SCAN_T;
-    const int b = t * per, e = min(N, b + per);
-    int64_t s = 0;
-    for (int k = b; k < e; k++) s += rowlen[k];
-    part[t] = s;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int64_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+    }
+    if (lane == 63) sh[w] = inc;
     __syncthreads();
-    for (int d = 1; d < SCAN_T; d <<= 1) {
-        int64_t v = (t >= d) ? part[t - d] : 0;
+    int64_t pre = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_T / 64; k++) {
+        const int64_t x = sh[k];
+        if (k < w) pre += x;
+        total += x;
+    }
+    __syncthreads();
+    return pre + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_sums_kernel(const int64_t *__restrict__ rowlen, int N,
+                                                           int64_t *__restrict__ bsum)
+{
+    __shared__ int64_t sh[SCAN_T / 64];
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_E;
+    int64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const int64_t k = b0 + q * SCAN_T + threadIdx.x;
+        if (k < N) s += rowlen[k];
+    }
+    int64_t total;
+    block_scan_excl64(s, sh, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_top_kernel(int64_t *__restrict__ bsum, int nblocks)
+{
+    __shared__ int64_t sh[SCAN_T / 64];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblocks; b0 += SCAN_T) {
+        const int k = b0 + threadIdx.x;
+        const int64_t v = k < nblocks ? bsum[k] : 0;
+        int64_t total;
+        const int64_t ex = block_scan_excl64(v, sh, total);
+        if (k < nblocks) bsum[k] = carry + ex;
         __syncthreads();
-        part[t] += v;
+        if (threadIdx.x == 0) carry += total;
         __syncthreads();
     }
-    int64_t run = part[t] - s;
-    for (int k = b; k < e; k++) {
-        R64[k] = run;
-        if (R32) R32[k] = (int32_t)run;
-        run += rowlen[k];
+    if (threadIdx.x == 0) bsum[nblocks] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_down_kernel(const int64_t *__restrict__ rowlen, int N,
+                                                           const int64_t *__restrict__ bsum, int nblocks,
+                                                           int64_t *__restrict__ R64, int32_t *__restrict__ R32)
+{
+    __shared__ int64_t sh[SCAN_T / 64];
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_E;
+    // thread t owns elements b0 + t * SCAN_PER .. + SCAN_PER - 1 (contiguous run)
+    int64_t v[SCAN_PER];
+    int64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const int64_t k = b0 + (int64_t)threadIdx.x * SCAN_PER + q;
+        v[q] = k < N ? rowlen[k] : 0;
+        s += v[q];
     }
-    if (t == SCAN_T - 1) {
-        R64[N] = part[t];
-        if (R32) R32[N] = (int32_t)part[t];
+    int64_t total;
+    int64_t run = bsum[blockIdx.x] + block_scan_excl64(s, sh, total);
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const int64_t k = b0 + (int64_t)threadIdx.x * SCAN_PER + q;
+        if (k < N) {
+            R64[k] = run;
+            if (R32) R32[k] = (int32_t)run;
+        }
+        run += v[q];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        R64[N] = bsum[nblocks];
+        if (R32) R32[N] = (int32_t)bsum[nblocks];
     }
 }
 
@@ -208,13 +286,15 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
     }
 }
 
-hipError_t launch_recheck(const double *z64, int N, int L, int nb, const Candidate *cands,
+hipError_t launch_recheck(const float *xs, int64_t ld, const double *mu, const double *sd, int N, int L, int nb,
+                          const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
                           int32_t *frag_live, Entry *entries, int64_t entry_cap, unsigned long long *stats,
                           hipStream_t st)
 {
     // enough waves to cover the device; candidates are grid-strided
-    hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, z64, N, L, nb, cands, cand_count, cand_cap,
+    hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, xs, ld, mu, sd, N, L, nb, cands, cand_count,
+                       cand_cap,
                        thr, frag_live, entries, entry_cap, stats);
     return hipGetLastError();
 }
@@ -231,11 +311,16 @@ hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, cons
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, hipStream_t st)
+hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, int64_t *bsum, hipStream_t st)
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(SCAN_T), 0, st, rowlen, N, R64, R32);
+    const int nblocks = (N + SCAN_E - 1) / SCAN_E;
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(nblocks), dim3(SCAN_T), 0, st, rowlen, N, bsum);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(SCAN_T), 0, st, bsum, nblocks);
+    hipLaunchKernelGGL(scan_down_kernel, dim3(nblocks), dim3(SCAN_T), 0, st, rowlen, N, bsum, nblocks, R64, R32);
     return hipGetLastError();
 }
+
+int scan_blocks(int N) { return (N + SCAN_E - 1) / SCAN_E; }
 
 hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C, float *V,

@@ -7,117 +7,210 @@
 // The file is compiled without floating-point contraction, so every operation
 // rounds as written and the double results equal the CPU restatement's.
 //
-// One thread per voxel.  With time-major input (the reference's BOLD[l*N+i])
-// consecutive threads read consecutive voxels of one time point, so every load
-// is coalesced; with voxel-major input each thread streams its own row.  The
-// loads of a pass are independent of each other and are unrolled so many are in
-// flight while the running sum advances.
+// A workgroup of 256 threads owns 64 voxels and walks the series in chunks of 64
+// time points.  Every chunk is loaded by all four waves with coalesced 256-byte
+// rows (one time point of 64 voxels for time-major input, 64 time points of one
+// voxel for voxel-major input) into an LDS tile [t][v]; the next chunk's loads
+// are in flight while wave 0 adds the current one into the 64 sequential sums.
+// Three passes: mean, sum of squares, then z.
 //
-// Outputs: z64 [N][L] (voxel-major) for the FP64 recheck, and the k-blocked
-// MFMA operand panel of the requested precision, zero-padded to N_pad x L_pad:
-// one record per (16-point block kb, voxel row r) at ((kb * N_pad) + r) * unit
-// bytes holding fp32 z[16] (unit 64), bf16 hi[16] (unit 32) or bf16 hi[16]
-// followed by lo[16] (unit 64), with hi = bf16(z) and lo = bf16(z - hi).  Each
-// thread writes whole records, and neighbouring threads write neighbouring
-// records.  A voxel whose mean or norm is not finite, or whose norm is 0 (a
-// constant series), is marked invalid and its operand row is NaN, so every r it
-// takes part in is NaN and fails every comparison downstream.
+// Pass 3 gives each thread one 16-point block of one voxel and writes
+//  - the k-blocked MFMA operand panel of the requested precision, zero-padded to
+//    N_pad x L_pad: one record per (16-point block kb, voxel row r) at
+//    ((kb * N_pad) + r) * unit bytes holding fp32 z[16] (unit 64), bf16 hi[16]
+//    (unit 32) or bf16 hi[16] followed by lo[16] (unit 64), with hi = bf16(z)
+//    and lo = bf16(z - hi); a wave writes 64 neighbouring records;
+//  - for time-major input, xv: the raw series voxel-major with row stride L_pad,
+//    from which the FP64 recheck recomputes z bit-for-bit.
+// A voxel whose mean or norm is not finite, or whose norm is 0 (a constant
+// series), is marked invalid and its operand row is NaN, so every r it takes
+// part in is NaN and fails every comparison downstream.
 #include "common.hpp"
 
 namespace bna {
 
-constexpr int ZN_T = 64;
+constexpr int ZV = 64;         // voxels per workgroup
+constexpr int ZC = 64;         // time points per chunk (four 16-point blocks)
+constexpr int ZT = 256;        // threads
+constexpr int ZLD = ZV + 1;    // LDS row stride (floats)
+constexpr int ZPER = ZC * ZV / ZT;  // loads per thread per chunk
 
 typedef unsigned int zu32x4 __attribute__((ext_vector_type(4)));
+typedef float zf32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ double zn_x(const float *__restrict__ x, int layout, int N, int L, int i, int l) {
-    return layout ? (double)x[(int64_t)i * L + l] : (double)x[(int64_t)l * N + i];
+// Loads chunk t0 into registers.  Time-major: thread (w, lane) takes time point
+// t0 + w + 4q of voxel v0 + lane.  Voxel-major: it takes time point t0 + lane of
+// voxel v0 + w + 4q.  Out-of-range elements read as 0 and are never summed.
+__device__ __forceinline__ void zn_load(const float *__restrict__ x, int layout, int N, int L, int v0, int t0,
+                                        float (&r)[ZPER])
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < ZPER; q++) {
+        const int a = w + 4 * q;
+        const int t = layout ? t0 + lane : t0 + a;
+        const int v = layout ? v0 + a : v0 + lane;
+        r[q] = (t < L && v < N) ? (layout ? x[(int64_t)v * L + t] : x[(int64_t)t * N + v]) : 0.0f;
+    }
 }
 
-__global__ __launch_bounds__(ZN_T) void znorm_kernel(const float *__restrict__ x, int layout, int N, int L,
-                                                     int N_pad, int L_pad, double *__restrict__ mu,
-                                                     double *__restrict__ sd, double *__restrict__ z64,
-                                                     uint8_t *__restrict__ valid, int prec,
-                                                     unsigned char *__restrict__ panel)
+__device__ __forceinline__ void zn_store(float *tile, int layout, const float (&r)[ZPER])
 {
-    const int i = blockIdx.x * ZN_T + threadIdx.x;  // < N_pad (grid spans N_pad)
-    const bool live = i < N;
-    const int unit = panel_unit(prec);
-
-    double s1 = 0.0, s2 = 0.0;
-    if (live) {
-#pragma unroll 8
-        for (int l = 0; l < L; l++) s1 += zn_x(x, layout, N, L, i, l);
-        s1 /= L;
-#pragma unroll 8
-        for (int l = 0; l < L; l++) {
-            const double d = zn_x(x, layout, N, L, i, l) - s1;
-            const double p = d * d;
-            s2 += p;
-        }
-        s2 = sqrt(s2);
-    }
-    const bool ok = live && isfinite(s1) && isfinite(s2) && s2 > 0.0;
-    if (live) {
-        mu[i] = s1;
-        sd[i] = s2;
-        valid[i] = ok ? 1 : 0;
-    }
-    const bool row_ok = ok || !live;  // padding rows are valid zeros
-
-    for (int kb = 0; kb < L_pad / 16; kb++) {
-        double z[16];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int l = kb * 16 + q;
-            z[q] = (live && l < L) ? (zn_x(x, layout, N, L, i, l) - s1) / s2 : 0.0;
+    for (int q = 0; q < ZPER; q++) {
+        const int a = w + 4 * q;
+        if (layout) tile[lane * ZLD + a] = r[q];
+        else tile[a * ZLD + lane] = r[q];
+    }
+}
+
+__global__ __launch_bounds__(ZT) void znorm_kernel(const float *__restrict__ x, int layout, int N, int L,
+                                                   int N_pad, int L_pad, double *__restrict__ mu,
+                                                   double *__restrict__ sd, float *__restrict__ xv,
+                                                   uint8_t *__restrict__ valid, int prec,
+                                                   unsigned char *__restrict__ panel)
+{
+    __shared__ float tile[2][ZC * ZLD];
+    __shared__ double s_mu[ZV], s_sd[ZV];
+    __shared__ int s_ok[ZV];
+    const int v0 = blockIdx.x * ZV;  // < N_pad (grid spans N_pad)
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nc = (L + ZC - 1) / ZC;
+
+    // passes 0 and 1: sequential sums in wave 0
+    double s = 0.0, mean = 0.0;
+    for (int pass = 0; pass < 2; pass++) {
+        float r[ZPER];
+        zn_load(x, layout, N, L, v0, 0, r);
+        s = 0.0;
+        for (int c = 0; c < nc; c++) {
+            float *tb = tile[c & 1];
+            zn_store(tb, layout, r);
+            __syncthreads();
+            if (c + 1 < nc) zn_load(x, layout, N, L, v0, (c + 1) * ZC, r);
+            if (w == 0) {
+                const int n = min(ZC, L - c * ZC);
+                if (pass == 0) {
+                    for (int t = 0; t < n; t++) s += (double)tb[t * ZLD + lane];
+                } else {
+                    for (int t = 0; t < n; t++) {
+                        const double d = (double)tb[t * ZLD + lane] - mean;
+                        const double p = d * d;
+                        s += p;
+                    }
+                }
+            }
         }
+        if (pass == 0) mean = s / L;
+        __syncthreads();  // wave 0 is done with both tiles before they are refilled
+    }
+    if (w == 0) {
+        const double norm = sqrt(s);
+        const int i = v0 + lane;
+        const bool live = i < N;
+        const bool ok = live && isfinite(mean) && isfinite(norm) && norm > 0.0;
         if (live) {
+            mu[i] = mean;
+            sd[i] = norm;
+            valid[i] = ok ? 1 : 0;
+        }
+        s_mu[lane] = mean;
+        s_sd[lane] = norm;
+        s_ok[lane] = ok || !live;  // padding rows are valid zeros
+    }
+
+    // pass 2: z, operand records and the voxel-major copy
+    const int unit = panel_unit(prec);
+    const int nkb = L_pad / 16;
+    float r[ZPER];
+    zn_load(x, layout, N, L, v0, 0, r);
+    const int ncp = (L_pad + ZC - 1) / ZC;  // chunks covering the padded length
+    for (int c = 0; c < ncp; c++) {
+        float *tb = tile[c & 1];
+        zn_store(tb, layout, r);
+        __syncthreads();
+        if (c + 1 < ncp) zn_load(x, layout, N, L, v0, (c + 1) * ZC, r);
+        const int kb = c * 4 + w;
+        if (kb < nkb) {
+            const int i = v0 + lane;
+            const bool live = i < N;
+            const double m = s_mu[lane], sdv = s_sd[lane];
+            const bool row_ok = s_ok[lane];
+            float xs[16];
+            double z[16];
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int l = kb * 16 + q;
-                if (l < L) z64[(int64_t)i * L + l] = z[q];
+                xs[q] = tb[(w * 16 + q) * ZLD + lane];
+                z[q] = (live && l < L) ? ((double)xs[q] - m) / sdv : 0.0;
             }
-        }
-        unsigned char *rec = panel + ((int64_t)kb * N_pad + i) * unit;
-        if (prec == 0) {
-            zu32x4 w[4];
+            if (xv && live) {
+                zf32x4 *dst = (zf32x4 *)(xv + (int64_t)i * L_pad + kb * 16);
 #pragma unroll
-            for (int q = 0; q < 16; q++)
-                w[q >> 2][q & 3] = __builtin_bit_cast(uint32_t, row_ok ? (float)z[q] : __builtin_nanf(""));
+                for (int c4 = 0; c4 < 4; c4++)
+                    dst[c4] = zf32x4{xs[4 * c4], xs[4 * c4 + 1], xs[4 * c4 + 2], xs[4 * c4 + 3]};
+            }
+            unsigned char *rec = panel + ((int64_t)kb * N_pad + i) * unit;
+            if (prec == 0) {
+                zu32x4 wv[4];
 #pragma unroll
-            for (int c = 0; c < 4; c++) *(zu32x4 *)(rec + 16 * c) = w[c];
-        } else {
-            uint32_t h[8], w[8];
+                for (int q = 0; q < 16; q++)
+                    wv[q >> 2][q & 3] = __builtin_bit_cast(uint32_t, row_ok ? (float)z[q] : __builtin_nanf(""));
 #pragma unroll
-            for (int q = 0; q < 16; q += 2) {
-                uint16_t h0 = 0x7fc0u, h1 = 0x7fc0u, w0 = 0x7fc0u, w1 = 0x7fc0u;
-                if (row_ok) {
-                    h0 = f32_to_bf16_rn((float)z[q]);
-                    h1 = f32_to_bf16_rn((float)z[q + 1]);
-                    w0 = f32_to_bf16_rn((float)(z[q] - (double)bf16_to_f32(h0)));
-                    w1 = f32_to_bf16_rn((float)(z[q + 1] - (double)bf16_to_f32(h1)));
+                for (int c4 = 0; c4 < 4; c4++) *(zu32x4 *)(rec + 16 * c4) = wv[c4];
+            } else {
+                uint32_t h[8], lo[8];
+#pragma unroll
+                for (int q = 0; q < 16; q += 2) {
+                    uint16_t h0 = 0x7fc0u, h1 = 0x7fc0u, w0 = 0x7fc0u, w1 = 0x7fc0u;
+                    if (row_ok) {
+                        h0 = f32_to_bf16_rn((float)z[q]);
+                        h1 = f32_to_bf16_rn((float)z[q + 1]);
+                        w0 = f32_to_bf16_rn((float)(z[q] - (double)bf16_to_f32(h0)));
+                        w1 = f32_to_bf16_rn((float)(z[q + 1] - (double)bf16_to_f32(h1)));
+                    }
+                    h[q >> 1] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+                    lo[q >> 1] = (uint32_t)w0 | ((uint32_t)w1 << 16);
                 }
-                h[q >> 1] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-                w[q >> 1] = (uint32_t)w0 | ((uint32_t)w1 << 16);
-            }
-            *(zu32x4 *)(rec + 0) = zu32x4{h[0], h[1], h[2], h[3]};
-            *(zu32x4 *)(rec + 16) = zu32x4{h[4], h[5], h[6], h[7]};
-            if (prec == 2) {
-                *(zu32x4 *)(rec + 32) = zu32x4{w[0], w[1], w[2], w[3]};
-                *(zu32x4 *)(rec + 48) = zu32x4{w[4], w[5], w[6], w[7]};
+                *(zu32x4 *)(rec + 0) = zu32x4{h[0], h[1], h[2], h[3]};
+                *(zu32x4 *)(rec + 16) = zu32x4{h[4], h[5], h[6], h[7]};
+                if (prec == 2) {
+                    *(zu32x4 *)(rec + 32) = zu32x4{lo[0], lo[1], lo[2], lo[3]};
+                    *(zu32x4 *)(rec + 48) = zu32x4{lo[4], lo[5], lo[6], lo[7]};
+                }
             }
         }
+    }
+}
+
+// z64[i][l] = (x[i][l] - mu[i]) / sd[i] from the voxel-major series (row stride
+// ld), the same expression pass 2 of the normalisation evaluates.
+__global__ __launch_bounds__(256) void zmat_kernel(const float *__restrict__ xs, int64_t ld, int N, int L,
+                                                   const double *__restrict__ mu, const double *__restrict__ sd,
+                                                   double *__restrict__ z64)
+{
+    const int64_t n = (int64_t)N * L;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
+        const int64_t i = k / L, l = k - i * L;
+        z64[k] = ((double)xs[i * ld + l] - mu[i]) / sd[i];
     }
 }
 
 hipError_t launch_znorm(const float *x, int layout, int N, int L, int N_pad, int L_pad, double *mu,
-                        double *sd, double *z64, uint8_t *valid, int prec, unsigned char *panel,
-                        hipStream_t st)
+                        double *sd, float *xv, uint8_t *valid, int prec, unsigned char *panel, hipStream_t st)
 {
-    if (N_pad % ZN_T != 0 || L_pad % 32 != 0 || prec < 0 || prec > 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(znorm_kernel, dim3(N_pad / ZN_T), dim3(ZN_T), 0, st, x, layout, N, L, N_pad, L_pad, mu,
-                       sd, z64, valid, prec, panel);
+    if (N_pad % ZV != 0 || L_pad % 32 != 0 || prec < 0 || prec > 2 || L < 1) return hipErrorInvalidValue;
+    if (layout == 0 && !xv) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(znorm_kernel, dim3(N_pad / ZV), dim3(ZT), 0, st, x, layout, N, L, N_pad, L_pad, mu, sd,
+                       layout == 0 ? xv : nullptr, valid, prec, panel);
+    return hipGetLastError();
+}
+
+hipError_t launch_zmat(const float *xs, int64_t ld, int N, int L, const double *mu, const double *sd, double *z64,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(zmat_kernel, dim3(2048), dim3(256), 0, st, xs, ld, N, L, mu, sd, z64);
     return hipGetLastError();
 }
 
