@@ -36,7 +36,8 @@ def log(msg: str) -> None:
 
 
 def time_path(ctx, series, N, L, prec, thr, steps, warmup):
-    """Returns (ms per step by host clock, mean corr-kernel ms, last graph stats)."""
+    """Returns (ms per step by host clock, mean corr-kernel ms, last graph stats,
+    mean per-stage ms from the library's HIP events)."""
     last = None
     for _ in range(warmup):
         series.normalize(bna.PRECISIONS[prec])
@@ -44,17 +45,23 @@ def time_path(ctx, series, N, L, prec, thr, steps, warmup):
         last = (g.nnz, g.n_candidates, g.n_flips, g.band, g.max_err)
         series.free_device(g)
     ctx._L.bna_synchronize(ctx._h)
-    corr = []
+    corr, stages = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         series.normalize(bna.PRECISIONS[prec])
+        zn = ctx.timings()["znorm_ms"]
         g = series.build_device(thr=thr, precision=prec)
-        corr.append(ctx.timings()["corr_ms"])
+        t = ctx.timings()
+        t["znorm_ms"] = zn
+        corr.append(t["corr_ms"])
+        stages.append(t)
         last = (g.nnz, g.n_candidates, g.n_flips, g.band, g.max_err)
         series.free_device(g)
     ctx._L.bna_synchronize(ctx._h)
     t1 = time.perf_counter()
-    return (t1 - t0) * 1e3 / steps, float(np.mean(corr)), last
+    keys = ("znorm_ms", "corr_ms", "recheck_ms", "select_ms", "csr_ms", "refine_ms")
+    st_ms = {k: round(float(np.mean([d[k] for d in stages])), 4) for k in keys}
+    return (t1 - t0) * 1e3 / steps, float(np.mean(corr)), last, st_ms
 
 
 def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float):
@@ -117,10 +124,10 @@ def main() -> int:
     with bna.Context(0) as ctx, ctx.series(x) as s:
         for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"])):
             log(f"bench: {prec} warmup={a.warmup} steps={a.steps}")
-            ms, corr_ms, st = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
-            res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st}
+            ms, corr_ms, st, st_ms = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
+            res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st, "stages_ms": st_ms}
             log(f"bench: {prec} {ms:.3f} ms/step, corr kernel {corr_ms:.3f} ms, nnz={st[0]} "
-                f"cand={st[1]} flips={st[2]} band={st[3]:.3g} max_err={st[4]:.3g}")
+                f"cand={st[1]} flips={st[2]} band={st[3]:.3g} max_err={st[4]:.3g} stages={st_ms}")
 
     b = res["bf16x3"]
     value = M1 / (b["ms"] / 1e3)
@@ -148,6 +155,7 @@ def main() -> int:
         "band_pairs_rechecked": b["stats"][1],
         "max_abs_err_vs_fp64": b["stats"][4],
         "p99_subject_latency_ms": None,
+        "stages_ms": b["stages_ms"],
     }
     if "fp32" in res:
         f = res["fp32"]
@@ -156,6 +164,7 @@ def main() -> int:
         line["speedup_vs_fp32"] = value / fval
         line["fp32_roofline_frac"] = flops / (f["corr_ms"] / 1e3) / 1e12 / PEAK["fp32"]
         line["same_csr_as_fp32"] = f["stats"][0] == b["stats"][0]
+        line["fp32_stages_ms"] = f["stages_ms"]
     if not a.no_cpu:
         log("bench: CPU baseline")
         line["cpu_baseline"] = cpu_baseline(x, N, L, thr, a.cpu_seconds)

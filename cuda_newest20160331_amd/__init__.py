@@ -37,7 +37,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("thr_type", C.c_int), ("thr", C.c_double), ("weighted", C.c_int),
         ("precision", C.c_int), ("recheck", C.c_int), ("ref_compat_eps", C.c_int),
-        ("tile", C.c_int),
+        ("tile", C.c_int), ("refine", C.c_int),
     ]
 
 
@@ -53,7 +53,8 @@ class CSR(C.Structure):
 class Timings(C.Structure):
     _fields_ = [
         ("znorm_ms", C.c_double), ("corr_ms", C.c_double), ("recheck_ms", C.c_double),
-        ("csr_ms", C.c_double), ("select_ms", C.c_double), ("total_ms", C.c_double),
+        ("csr_ms", C.c_double), ("refine_ms", C.c_double), ("select_ms", C.c_double),
+        ("total_ms", C.c_double),
         ("corr_launches", C.c_int64),
     ]
 
@@ -132,7 +133,7 @@ def density_to_k(N: int, spa_percent: float) -> int:
 
 def make_opts(thr: float = 0.0, *, density: float | None = None, weighted: bool = True,
               precision: int | str = BF16X3, recheck: bool = True,
-              ref_compat_eps: bool = False) -> Opts:
+              ref_compat_eps: bool = False, refine: bool = True) -> Opts:
     o = Opts()
     lib().bna_opts_default(C.byref(o))
     if isinstance(precision, str):
@@ -145,6 +146,7 @@ def make_opts(thr: float = 0.0, *, density: float | None = None, weighted: bool 
     o.precision = int(precision)
     o.recheck = int(bool(recheck))
     o.ref_compat_eps = int(bool(ref_compat_eps))
+    o.refine = int(bool(refine))
     return o
 
 

@@ -151,6 +151,7 @@ void bna_opts_default(bna_opts *o)
     o->weighted = 1;
     o->precision = BNA_BF16X3;
     o->recheck = 1;
+    o->refine = 1;
 }
 
 const char *bna_status_string(int s)
@@ -460,12 +461,64 @@ static float f_ceil_above(double t)  // a float >= t, one ulp above the nearest
 }
 
 // ------------------------------------------------------------ build
-static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, double thr_eff, bna_csr *out,
-                           bool timing_from_norm)
+// Histogram bin edge b of the tile kernel's r histogram over [-1, 1].
+static double hist_edge(int b) { return -1.0 + (double)b / (HIST_BINS / 2); }
+
+// Density bracket.  H(b) = number of pairs whose MFMA r falls in bins >= b.  The
+// kernel's bin of v is floor((v + 1) * HIST_BINS / 2) in float arithmetic, so
+//   #(v >= edge(b + 1)) <= H(b) <= #(v >= edge(b - 1)).
+// With |r_mfma - r64| <= band and b the largest bin with H(b) >= k, the k-th
+// largest r64, r*, satisfies  s_lo + band <= r* < s_hi - band  for
+//   s_lo = edge(b - 1) - 2 band,  s_hi = edge(b + 2) + 2 band:
+// then every pair with MFMA r >= s_hi has r64 > r* (a sure edge), every pair with
+// MFMA r < s_lo has r64 < r* (never an edge), and all ties at r* lie in between.
+struct DensityBracket {
+    double s_lo, s_hi;
+    int64_t emit_max, cand_max, valid_pairs;
+};
+static DensityBracket density_bracket(const unsigned long long *hist, int64_t k, double band)
+{
+    std::vector<int64_t> H(HIST_BINS + 2, 0);
+    for (int b = HIST_BINS - 1; b >= 0; b--) H[b] = H[b + 1] + (int64_t)hist[b];
+    DensityBracket d{};
+    d.valid_pairs = H[0];
+    int b = 0;
+    for (int t = HIST_BINS - 1; t >= 0; t--)
+        if (H[t] >= k) {
+            b = t;
+            break;
+        }
+    d.s_lo = b - 1 <= 0 ? -INFINITY : hist_edge(b - 1) - 2.0 * band;
+    d.s_hi = b + 2 >= HIST_BINS ? INFINITY : hist_edge(b + 2) + 2.0 * band;
+    // #(v >= s_lo) <= H(b_lo) for edge(b_lo + 1) <= s_lo; #(v >= s_hi) >= H(b_hi) for edge(b_hi - 1) >= s_hi
+    int b_lo = 0, b_hi = HIST_BINS;
+    if (std::isfinite(d.s_lo)) b_lo = std::max(0, std::min(HIST_BINS - 1, (int)std::floor((d.s_lo + 1.0) * (HIST_BINS / 2)) - 1));
+    if (std::isfinite(d.s_hi)) b_hi = std::max(0, std::min(HIST_BINS, (int)std::ceil((d.s_hi + 1.0) * (HIST_BINS / 2)) + 1));
+    d.emit_max = H[b_lo];
+    d.cand_max = H[b_lo] - H[b_hi];
+    return d;
+}
+
+static double decode_desc_key(unsigned long long k)
+{
+    uint64_t u = ~(uint64_t)k;
+    const uint64_t bits = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+    double r;
+    memcpy(&r, &bits, sizeof r);
+    return r;
+}
+
+// One construction.  Threshold mode (k < 0): edge iff r > thr_eff, with pairs
+// within the error band of thr_eff rechecked in FP64.  Density mode (k >= 0):
+// exactly k edges, chosen by a histogram pass, a bracketed emission pass, an
+// FP64 recheck of the bracket and an exact radix select.
+static int build_graph(bna_ctx *ctx, bna_series *s, const bna_opts *o, double thr_eff, int64_t k_density,
+                       bna_csr *out, bool timing_from_norm)
 {
     const int N = s->N, nb = s->N_pad / TILE, nb1 = nb + 1;
     const int prec = o->precision;
     const bool recheck = o->recheck != 0;
+    const bool density = k_density >= 0;
     const double band = band_for(prec, s->L_pad);
 
     TileItem *tiles = nullptr;
@@ -479,7 +532,7 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     if ((rc = ws_get_t(ctx, "frag_orig", nfrag, &frag_orig))) return rc;
     if ((rc = ws_get_t(ctx, "frag_live", nfrag, &frag_live))) return rc;
     if ((rc = ws_get_t(ctx, "frag_off", nfrag, &frag_off))) return rc;
-    unsigned long long *ctr;  // [0] entries, [1] candidates, [2] max err bits, [3] flips
+    unsigned long long *ctr;  // [0] entries, [1] candidates, [2] max err bits, [3] flips, [4] overflow pool
     if ((rc = ws_get_t(ctx, "counters", 8, &ctr))) return rc;
     SubjDesc *dsd;
     if ((rc = ws_get_t(ctx, "subjdesc", 1, &dsd))) return rc;
@@ -496,13 +549,6 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     h.N_pad = s->N_pad;
     h.L_pad = s->L_pad;
     h.nb = nb;
-    if (recheck) {
-        h.keep_thr = f_below(thr_eff - band);
-        h.sure_thr = f_ceil_above(thr_eff + band);
-    } else {
-        h.keep_thr = f_floor(thr_eff);
-        h.sure_thr = h.keep_thr;
-    }
     h.frag_orig = frag_orig;
     h.frag_live = frag_live;
     h.frag_off = frag_off;
@@ -511,9 +557,54 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     h.ovf_count = ctr + 4;
     h.hist = nullptr;
 
-    unsigned long long hc[8];
-    float ms_corr = 0.f, ms_re = 0.f;
+    float ms_corr = 0.f, ms_re = 0.f, ms_sel = 0.f;
     int64_t launches = 0;
+    SelState *sel = nullptr;
+    double *r64c = nullptr;
+    const double band_used = recheck ? band : 0.0;
+
+    if (density && k_density < 1)
+        return fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
+    if (density) {
+        // pass 1: histogram of the MFMA r over every valid pair
+        unsigned long long *hist;
+        if ((rc = ws_get_t(ctx, "hist", HIST_BINS, &hist))) return rc;
+        if ((rc = ws_get_t(ctx, "selstate", 1, &sel))) return rc;
+        h.hist = hist;
+        h.keep_thr = INFINITY;
+        h.sure_thr = INFINITY;
+        HIPCHK(ctx, hipMemsetAsync(hist, 0, HIST_BINS * sizeof(unsigned long long), ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(dsd, &h, sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        HIPCHK(ctx, launch_corr(prec, 1, dsd, tiles, ntiles, ctx->stream));
+        launches++;
+        HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        std::vector<unsigned long long> hh(HIST_BINS);
+        HIPCHK(ctx, hipMemcpyAsync(hh.data(), hist, HIST_BINS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        ms_corr += ms;
+        h.hist = nullptr;
+        const DensityBracket br = density_bracket(hh.data(), k_density, band_used);
+        if (k_density < 1 || k_density > br.valid_pairs)
+            return fail(ctx, BNA_ERR_DENSITY, "density: " + std::to_string(k_density) + " edges requested, " +
+                                                  std::to_string(br.valid_pairs) + " valid pairs");
+        h.keep_thr = std::isfinite(br.s_lo) ? f_below(br.s_lo) : -INFINITY;  // emit v >= s_lo
+        h.sure_thr = std::isfinite(br.s_hi) ? f_below(br.s_hi) : INFINITY;   // candidate v < s_hi
+        ecap = std::max<int64_t>(ecap, 2 * br.emit_max + 4096);
+        ccap = std::max<int64_t>(ccap, br.cand_max + 4096);
+    } else if (recheck) {
+        h.keep_thr = f_below(thr_eff - band);
+        h.sure_thr = f_ceil_above(thr_eff + band);
+    } else {
+        h.keep_thr = f_floor(thr_eff);
+        h.sure_thr = h.keep_thr;
+    }
+
+    unsigned long long hc[8];
+    SelState hs{};
     int64_t nnz = 0;
 
     // row lengths, scan and placement buffers
@@ -529,16 +620,18 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     int32_t *R32 = nullptr;
     if ((rc = out_alloc(ctx, (void **)&R32, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
 
-    // One pass of correlation, recheck, row lengths and scan, then one host
-    // read of the counters: when any list overflowed its capacity, grow it and
-    // run the pass again (the recheck skips slots past the capacity).
+    // One pass of correlation, recheck (and selection), row lengths and scan,
+    // then one host read of the counters: when any list overflowed its
+    // capacity, grow it and run the pass again (the recheck and the selection
+    // skip slots past the capacity).
     for (int attempt = 0;; attempt++) {
         Entry *entries;
         Candidate *cands;
         KeptPair *ovf;
         if ((rc = ws_get_t(ctx, "entries", (size_t)ecap, &entries)) ||
             (rc = ws_get_t(ctx, "cands", (size_t)ccap, &cands)) ||
-            (rc = ws_get_t(ctx, "kept_ovf", (size_t)ocap, &ovf))) {
+            (rc = ws_get_t(ctx, "kept_ovf", (size_t)ocap, &ovf)) ||
+            (density && (rc = ws_get_t(ctx, "cand_r64", (size_t)ccap, &r64c)))) {
             (void)hipFree(R32);
             return rc;
         }
@@ -554,9 +647,18 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, ntiles, ctx->stream));
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        HIPCHK(ctx, launch_recheck(s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->mu, s->sd, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff, frag_live, h.entries,
-                                   h.entry_cap, ctr + 2, ctx->stream));
+        const float *xs = s->xv ? s->xv : s->x;
+        const int64_t ld = s->xv ? s->L_pad : s->L;
+        const int mode = density ? (recheck ? 1 : 2) : 0;
+        HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff,
+                                   frag_live, h.entries, h.entry_cap, ctr + 2, mode, r64c, ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        if (density) {
+            HIPCHK(ctx, launch_select(sel, ctr, h.cands, r64c, h.cand_cap, k_density, N, frag_live, h.entries,
+                                      h.entry_cap, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(&hs, sel, sizeof(SelState), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
         HIPCHK(ctx, launch_frag_rows(frag_orig, frag_live, frag_off, N, nb, orig_rm, live_rm, off_rm, rowlen,
                                      ctx->stream));
         HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, bsum, ctx->stream));
@@ -568,6 +670,8 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         ms_corr += ms;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
         ms_re += ms;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+        ms_sel += ms;
         const bool e_over = (int64_t)hc[0] > ecap, c_over = (int64_t)hc[1] > ccap, o_over = (int64_t)hc[4] > ocap;
         if (!e_over && !c_over && !o_over) break;
         if (attempt == 2) {
@@ -579,6 +683,16 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
         if (o_over) ocap = (int64_t)hc[4];
     }
     const int64_t ncand = (int64_t)hc[1];
+    double r_star = thr_eff;
+    if (density) {
+        if (hs.err || nnz != 2 * k_density) {
+            (void)hipFree(R32);
+            return fail(ctx, BNA_ERR_DENSITY,
+                        "density selection inconsistent (err " + std::to_string(hs.err) + ", nnz " +
+                            std::to_string(nnz) + ", k " + std::to_string(k_density) + ")");
+        }
+        r_star = decode_desc_key(hs.kmax);
+    }
     if (nnz > 0x7fffffffLL) {
         (void)hipFree(R32);
         return fail(ctx, BNA_ERR_OVERFLOW, "nnz exceeds int32 (" + std::to_string(nnz) + ")");
@@ -596,13 +710,20 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     }
     HIPCHK(ctx, launch_place(orig_rm, live_rm, off_rm, h.entries, R64, N, nb, Cd, Vd, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    if (Vd && o->refine && nnz > 0)
+        HIPCHK(ctx, launch_refine(R32, Cd, Vd, s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->mu, s->sd, N, s->L,
+                                  ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 
-    float ms_csr = 0.f, ms_tot = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[4], ctx->ev[6]));
-    HIPCHK(ctx, hipEventElapsedTime(&ms_tot, timing_from_norm ? ctx->ev[0] : ctx->ev[2], ctx->ev[6]));
+    float ms_csr = 0.f, ms_ref = 0.f, ms_tot = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[5], ctx->ev[6]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms_ref, ctx->ev[6], ctx->ev[7]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms_tot, timing_from_norm ? ctx->ev[0] : ctx->ev[1], ctx->ev[7]));
+    ctx->tm.refine_ms = ms_ref;
     ctx->tm.corr_ms = ms_corr;
     ctx->tm.recheck_ms = ms_re;
+    ctx->tm.select_ms = ms_sel;
     ctx->tm.csr_ms = ms_csr;
     ctx->tm.total_ms = ms_tot;
     ctx->tm.corr_launches = launches;
@@ -614,12 +735,12 @@ static int build_threshold(bna_ctx *ctx, bna_series *s, const bna_opts *o, doubl
     out->C = Cd;
     out->V = Vd;
     out->on_device = 1;
-    out->thr_used = thr_eff;
+    out->thr_used = r_star;
     const double M1 = (double)N * (N - 1) / 2.0;
     out->density_pct = 100.0 * (double)nnz / M1 / 2.0;
     out->n_candidates = ncand;
-    out->n_flips = (int64_t)hc[3];
-    out->band = recheck ? band : 0.0;
+    out->n_flips = density ? 0 : (int64_t)hc[3];
+    out->band = band_used;
     double me = 0.0;
     memcpy(&me, &hc[2], sizeof(double));
     out->max_err = me;
@@ -645,10 +766,15 @@ int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *o, bna_csr
     if (o->thr_type == BNA_THR_R) {
         double thr = o->thr;
         if (o->ref_compat_eps) thr -= 1e-6;
-        int rc = build_threshold(ctx, s, o, thr, out, normed);
+        int rc = build_graph(ctx, s, o, thr, -1, out, normed);
+        if (rc) return rc;
+    } else if (o->thr_type == BNA_THR_DENSITY) {
+        if (!(o->thr >= 0.0 && o->thr <= 100.0)) return fail(ctx, BNA_ERR_ARG, "density must be in [0, 100] percent");
+        const int64_t k = bna_density_to_k(s->N, (float)o->thr);
+        int rc = build_graph(ctx, s, o, 0.0, k, out, normed);
         if (rc) return rc;
     } else {
-        return fail(ctx, BNA_ERR_ARG, "density threshold: not available in this build");
+        return fail(ctx, BNA_ERR_ARG, "bad thr_type");
     }
     if (normed) {
         float ms = 0.f;

@@ -63,6 +63,18 @@ struct SubjDesc {
 
 constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
 
+// Radix-select state for the density threshold (device memory).
+struct SelState {
+    unsigned long long key_hi, key_lo;  // threshold key fixed so far (r part, pair part)
+    long long rank;                     // rank of the threshold key among keys matching it
+    int all;                            // 1: the threshold key is final
+    int err;                            // nonzero: counts inconsistent, nothing selected
+    unsigned int done, pad;             // workgroups finished in the current pass
+    unsigned long long kmax;            // largest kept r key (gives r*)
+    long long n_sure;                   // sure edges (pairs emitted above the bracket)
+    unsigned long long hist[256];
+};
+
 // bf16 round-to-nearest-even from float bits; NaN stays a quiet NaN.
 __host__ __device__ inline uint16_t f32_to_bf16_rn(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -96,11 +108,17 @@ hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const 
 // FP64 recheck of candidates.
 // Slots at or past entry_cap (overflowed lists) are skipped.
 // z is recomputed from the voxel-major series xs (row stride ld), mu and sd.
+// mode 0: decide r64 > thr; mode 1: write r64 to r64_out and to both entry slots
+// (density selection decides later); mode 2: as 1 with the MFMA r taken as is.
 hipError_t launch_recheck(const float *xs, int64_t ld, const double *mu, const double *sd, int N, int L, int nb,
                           const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
                           int32_t *frag_live, Entry *entries, int64_t entry_cap, unsigned long long *stats,
-                          hipStream_t st);
+                          int mode, double *r64_out, hipStream_t st);
+// Exact-k selection among rechecked candidates (ctr: [0] entry slots, [1] candidates).
+hipError_t launch_select(SelState *st, const unsigned long long *ctr, const Candidate *cands, const double *r64,
+                         int64_t cand_cap, int64_t k, int N, int32_t *frag_live, Entry *entries, int64_t entry_cap,
+                         hipStream_t stream);
 
 // Row lengths, exclusive scan into R, placement into C/V.
 // Fragment tables [k][row] -> [row][k], plus row lengths (sums of live counts).
@@ -109,6 +127,9 @@ hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, cons
 // bsum: scan_blocks(N) + 1 int64 of scratch.
 hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, int64_t *bsum, hipStream_t st);
 int scan_blocks(int N);
+// Recomputes every value of a weighted CSR in double from the voxel-major series.
+hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
+                         const double *mu, const double *sd, int N, int L, hipStream_t st);
 // frag_* here are the row-major tables.
 hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C,

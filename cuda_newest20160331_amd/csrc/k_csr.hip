@@ -28,7 +28,8 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float *__restrict__ 
                                                       const unsigned long long *__restrict__ cand_count,
                                                       int64_t cand_cap, double thr,
                                                       int32_t *__restrict__ frag_live, Entry *__restrict__ entries,
-                                                      int64_t entry_cap, unsigned long long *__restrict__ stats)
+                                                      int64_t entry_cap, unsigned long long *__restrict__ stats,
+                                                      int mode, double *__restrict__ r64_out)
 {
     const int64_t n = (int64_t)min((unsigned long long)cand_cap, *cand_count);
     const int lane = threadIdx.x & 63;
@@ -39,6 +40,10 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float *__restrict__ 
     for (int64_t c = wave; c < n; c += nwaves) {
         const Candidate cd = cands[c];
         if (cd.slot_row >= entry_cap || cd.slot_col >= entry_cap) continue;  // lists overflowed: rerun follows
+        if (mode == 2) {
+            if (lane == 0) r64_out[c] = (double)cd.r;
+            continue;
+        }
         const float *xi = xs + (int64_t)cd.i * ld;
         const float *xj = xs + (int64_t)cd.j * ld;
         const double mi = mu[cd.i], si = sd[cd.i], mj = mu[cd.j], sj = sd[cd.j];
@@ -54,7 +59,12 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float *__restrict__ 
             double o = __shfl_down(p, s, 64);
             if (lane < s) p = p + o;
         }
-        if (lane == 0) {
+        if (lane == 0 && mode == 1) {
+            r64_out[c] = p;
+            entries[cd.slot_row].r = (float)p;
+            entries[cd.slot_col].r = (float)p;
+            wmax = fmax(wmax, fabs((double)cd.r - p));
+        } else if (lane == 0) {
             const bool edge = p > thr;
             const bool raw = (double)cd.r > thr;
             if (edge) {
@@ -286,16 +296,121 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
     }
 }
 
+// Value refinement of a weighted CSR: every kept pair's value is recomputed as
+// V = (float) sum_l z_i[l] z_j[l] with z = (x - mean) * (1 / norm) in double,
+// so |V - r64| stays within half a float ulp of r plus ~1e-15, whatever the MFMA
+// accumulation did.  One wave per row i takes the entries j > i, four at a time:
+// each 16-lane group owns one entry, reads row j in 16-byte pieces (row i's
+// pieces come from cache), sums in double and writes the value to entry (i, j)
+// and to its mirror (j, i), which the group finds in row j's sorted columns.
+constexpr int RG = 16;  // lanes per entry
+
+__device__ __forceinline__ int64_t group_find_col(const int32_t *__restrict__ C, int64_t lo, int64_t hi, int key,
+                                                  int gl, uint64_t gmask, int gbase)
+{
+    // narrow [lo, hi) by sampling RG evenly spaced entries until one load covers it
+    while (hi - lo > RG) {
+        const int64_t step = (hi - lo + RG - 1) / RG;
+        const int64_t p = lo + (int64_t)gl * step;
+        const bool le = p < hi && C[p] <= key;
+        const uint64_t m = (__ballot(le) & gmask) >> gbase;
+        const int t = m ? 63 - __builtin_clzll(m) : 0;
+        lo = lo + (int64_t)t * step;
+        hi = min(hi, lo + step);
+    }
+    const int64_t p = lo + gl;
+    const uint64_t m = (__ballot(p < hi && C[p] == key) & gmask) >> gbase;
+    return m ? lo + __builtin_ctzll(m) : -1;
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void refine_kernel(const int32_t *__restrict__ R, const int32_t *__restrict__ C,
+                                                     float *__restrict__ V, const float *__restrict__ xs,
+                                                     int64_t ld, const double *__restrict__ mu,
+                                                     const double *__restrict__ sd, int N, int L)
+{
+    const int lane = threadIdx.x & 63;
+    const int g = lane / RG, gl = lane % RG, gbase = g * RG;
+    const uint64_t gmask = 0xffffull << gbase;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N) return;
+    const int64_t r0 = R[i], r1 = R[i + 1];
+    if (r0 == r1) return;
+    // first entry with column > i
+    int64_t e0 = r1;
+    for (int64_t b = r0; b < r1; b += 64) {
+        const uint64_t m = __ballot(b + lane < r1 && C[b + lane] > (int)i);
+        if (m) {
+            e0 = b + __builtin_ctzll(m);
+            break;
+        }
+    }
+    const float *xi = xs + i * ld;
+    const double mi = mu[i], ii = 1.0 / sd[i];
+    // every group runs the same number of trips so the ballots stay wave-wide
+    const int64_t trips = (r1 - e0 + 3) / 4;
+    for (int64_t t = 0; t < trips; t++) {
+        const int64_t e = e0 + 4 * t + g;
+        const bool act = e < r1;
+        const int j = act ? C[e] : (int)i;
+        const float *xj = xs + (int64_t)j * ld;
+        const double mj = mu[j], ij = 1.0 / sd[j];
+        double p = 0.0;
+        if (act) {
+            if constexpr (VEC) {
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                const int L4 = L >> 2;
+#pragma unroll 4
+                for (int q = gl; q < L4; q += RG) {
+                    const f4 a = *(const f4 *)(xi + 4 * q);
+                    const f4 b = *(const f4 *)(xj + 4 * q);
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+                        p = __builtin_fma(((double)a[c] - mi) * ii, ((double)b[c] - mj) * ij, p);
+                }
+                for (int l = 4 * L4 + gl; l < L; l += RG)
+                    p = __builtin_fma(((double)xi[l] - mi) * ii, ((double)xj[l] - mj) * ij, p);
+            } else {
+#pragma unroll 4
+                for (int l = gl; l < L; l += RG)
+                    p = __builtin_fma(((double)xi[l] - mi) * ii, ((double)xj[l] - mj) * ij, p);
+            }
+        }
+#pragma unroll
+        for (int s = RG / 2; s >= 1; s >>= 1) p += __shfl_xor(p, s, 64);
+        const int64_t q = act ? group_find_col(C, R[j], R[j + 1], (int)i, gl, gmask, gbase)
+                              : group_find_col(C, 0, 0, 0, gl, gmask, gbase);
+        if (act && gl == 0) {
+            const float v = (float)p;
+            V[e] = v;
+            if (q >= 0) V[q] = v;
+        }
+    }
+}
+
+hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
+                         const double *mu, const double *sd, int N, int L, hipStream_t st)
+{
+    const int64_t threads = (int64_t)N * 64;
+    const dim3 grid((unsigned)((threads + 255) / 256));
+    // 16-byte loads need 16-byte aligned rows
+    if (ld % 4 == 0 && ((uintptr_t)xs & 15) == 0)
+        hipLaunchKernelGGL(refine_kernel<true>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L);
+    else
+        hipLaunchKernelGGL(refine_kernel<false>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L);
+    return hipGetLastError();
+}
+
 hipError_t launch_recheck(const float *xs, int64_t ld, const double *mu, const double *sd, int N, int L, int nb,
                           const Candidate *cands,
                           const unsigned long long *cand_count, int64_t cand_cap, double thr,
                           int32_t *frag_live, Entry *entries, int64_t entry_cap, unsigned long long *stats,
-                          hipStream_t st)
+                          int mode, double *r64_out, hipStream_t st)
 {
     // enough waves to cover the device; candidates are grid-strided
     hipLaunchKernelGGL(recheck_kernel, dim3(2048), dim3(256), 0, st, xs, ld, mu, sd, N, L, nb, cands, cand_count,
                        cand_cap,
-                       thr, frag_live, entries, entry_cap, stats);
+                       thr, frag_live, entries, entry_cap, stats, mode, r64_out);
     return hipGetLastError();
 }
 

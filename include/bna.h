@@ -71,9 +71,11 @@ typedef struct bna_opts {
     int recheck;        /* 1: FP64 recheck of pairs within the error band (exact edge set) */
     int ref_compat_eps; /* 1: edge iff r > thr - 1e-6 (W/CuCorMat-2mm-fuse/CorMat-2mm-fuse.cu:13,158) */
     int tile;           /* reserved, 0 */
+    int refine;         /* weighted: recompute every kept value in double after the
+                           edge set is fixed, so |V - r64| <= half a float ulp + 1e-15 */
 } bna_opts;
 
-/* Fills the defaults: r threshold 0, weighted, BF16X3, recheck on. */
+/* Fills the defaults: r threshold 0, weighted, BF16X3, recheck on, refine on. */
 void bna_opts_default(bna_opts *o);
 
 /* A CSR graph.  R/C/V are host arrays when on_device == 0 and device pointers
@@ -104,6 +106,7 @@ typedef struct bna_timings {
     double corr_ms;    /* MFMA tile kernel(s), all passes */
     double recheck_ms;
     double csr_ms;     /* scan + placement */
+    double refine_ms;  /* value refinement */
     double select_ms;  /* density selection */
     double total_ms;   /* first to last event of the build */
     int64_t corr_launches;

@@ -1,8 +1,9 @@
 """GPU parity: libbna on the MI355X against the CPU oracle, through the C ABI.
 
 Bar: edge sets and R/C bit-exact against the FP64 oracle; z64, mean and norm
-bit-exact; weighted values within the error band the library reports (pairs
-the FP64 recheck saw carry (float) r64 exactly); degree bit-exact; eigenvector
+bit-exact; weighted values within 1e-6 absolute of r64 (the library recomputes
+each kept value in double once the edge set is fixed; without that refinement
+they stay within the error band it reports); degree bit-exact; eigenvector
 centrality within 1e-5 relative L2.
 """
 import os
@@ -15,6 +16,9 @@ import cuda_newest20160331_amd as bna
 import orc
 
 pytestmark = pytest.mark.gpu
+
+# Weighted values: |V - r64| <= 1e-6 absolute against the FP64 oracle.
+V_TOL = 1e-6
 
 
 def say(msg):
@@ -49,7 +53,7 @@ def compare(g, og, weighted=True):
     assert np.array_equal(g.C, og.C)
     if weighted:
         d = np.abs(g.V.astype(np.float64) - og.V64)
-        assert d.max(initial=0.0) <= max(g.band, 1e-7) + 1e-7, f"max |V - r64| = {d.max()}"
+        assert d.max(initial=0.0) <= V_TOL, f"max |V - r64| = {d.max()}"
         return float(d.max(initial=0.0))
     assert g.V is None
     return 0.0
@@ -163,6 +167,18 @@ def test_recheck_off_reports_flips(ctx):
         assert abs(r - 0.4) < 2e-2
 
 
+def test_refine_off_values_within_band(ctx):
+    say("refine off")
+    x, z64, valid = series_and_oracle(2500, 300, 17)
+    og = orc.build_threshold(z64, valid, 0.42)
+    for prec in ("bf16x3", "fp32"):
+        g = ctx.build_csr(x, thr=0.42, precision=prec, refine=False)
+        assert np.array_equal(g.R, og.R) and np.array_equal(g.C, og.C)
+        d = np.abs(g.V.astype(np.float64) - og.V64)
+        say(f"  {prec}: MFMA values, max |V - r64| = {d.max():.3g}, band {g.band:.3g}")
+        assert d.max() <= g.band
+
+
 def test_degree_and_ec_parity(ctx):
     say("degree + EC")
     x, z64, valid = series_and_oracle(3000, 200, 31)
@@ -223,3 +239,15 @@ def test_c3_full_size_properties(ctx):
         r[i] = -np.inf
         exp = np.nonzero(r > 0.4)[0]
         assert np.array_equal(a.C[a.R[i]:a.R[i + 1]], exp), f"row {i}"
+    # every weighted value against the oracle's r64 (upper-triangle entries)
+    rows = np.repeat(np.arange(N, dtype=np.int32), np.diff(a.R))
+    up = a.C > rows
+    ii = np.ascontiguousarray(rows[up])
+    jj = np.ascontiguousarray(a.C[up])
+    r = np.empty(len(ii))
+    orc.olib().orc_corr_pairs(z64.ctypes.data, L, ii.ctypes.data, jj.ctypes.data, len(ii), r.ctypes.data)
+    for name, g in (("bf16x3", a), ("fp32", b)):
+        d = np.abs(g.V[up].astype(np.float64) - r)
+        say(f"  {name}: max |V - r64| over all {len(ii)} edges = {d.max():.3g} "
+            f"(> 1e-6: {int(np.sum(d > 1e-6))}), band {g.band:.3g}")
+        assert d.max() <= V_TOL
