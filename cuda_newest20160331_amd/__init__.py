@@ -29,7 +29,7 @@ EXPORTS = [
     "bna_build_csr", "bna_series_create", "bna_series_destroy", "bna_series_normalize",
     "bna_build_csr_series", "bna_csr_to_host", "bna_series_download", "bna_degree",
     "bna_eigcent", "bna_density_to_k", "bna_synth_factor_model", "bna_write_csr",
-    "bna_read_csr", "bna_write_nm",
+    "bna_read_csr", "bna_write_nm", "bna_nifti_mask", "bna_nifti_bold", "bna_free",
 ]
 
 
@@ -98,6 +98,11 @@ def lib() -> C.CDLL:
         "bna_write_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
         "bna_read_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
         "bna_write_nm": (i32, [C.c_char_p, fp, i32]),
+        "bna_nifti_mask": (i32, [C.c_char_p, C.c_float, C.POINTER(C.POINTER(C.c_uint8)),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+        "bna_nifti_bold": (i32, [C.c_char_p, C.POINTER(C.c_uint8), i64, i32,
+                                 C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_int)]),
+        "bna_free": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -115,6 +120,58 @@ class BnaError(RuntimeError):
 
 def _f32p(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def nifti_mask(path: str, mask_thr: float) -> tuple[np.ndarray, int]:
+    """Voxel selection of a NIfTI mask (mask >= mask_thr): (selected uint8[total], N)."""
+    p = C.POINTER(C.c_uint8)()
+    total, n = C.c_int64(), C.c_int()
+    rc = lib().bna_nifti_mask(str(path).encode(), mask_thr, C.byref(p), C.byref(total), C.byref(n))
+    if rc:
+        raise BnaError(rc, f"bna_nifti_mask({path})")
+    try:
+        sel = np.ctypeslib.as_array(p, shape=(total.value,)).copy()
+    finally:
+        lib().bna_free(C.cast(p, C.c_void_p))
+    return sel, n.value
+
+
+def nifti_bold(path: str, selected: np.ndarray) -> np.ndarray:
+    """Selected voxels of a NIfTI series, time-major (L, N) float32."""
+    selected = np.ascontiguousarray(selected, np.uint8)
+    N = int(selected.sum())
+    p = C.POINTER(C.c_float)()
+    L = C.c_int()
+    rc = lib().bna_nifti_bold(str(path).encode(), selected.ctypes.data_as(C.POINTER(C.c_uint8)),
+                              selected.size, N, C.byref(p), C.byref(L))
+    if rc:
+        raise BnaError(rc, f"bna_nifti_bold({path})")
+    try:
+        x = np.ctypeslib.as_array(p, shape=(L.value, N)).copy()
+    finally:
+        lib().bna_free(C.cast(p, C.c_void_p))
+    return x
+
+
+def read_csr(path: str) -> "Graph":
+    """A .csr file as a host Graph (V is None for a binary file)."""
+    h = CSR()
+    rc = lib().bna_read_csr(str(path).encode(), C.byref(h))
+    if rc:
+        raise BnaError(rc, f"bna_read_csr({path})")
+    try:
+        return Graph(h)
+    finally:
+        lib().bna_csr_free(C.byref(h))
+
+
+def read_nm(path: str) -> np.ndarray:
+    """A .nm node map: int32 N then float32[N]."""
+    raw = np.fromfile(path, dtype=np.int32)
+    n = int(raw[0])
+    if raw.size != n + 1:
+        raise ValueError(f"{path}: {raw.size - 1} values for N={n}")
+    return raw[1:].view(np.float32).copy()
 
 
 def synth_factor_model(N: int, L: int, K: int, phi: float, seed: int,

@@ -188,3 +188,127 @@ extern "C" int bna_write_nm(const char *path, const float *v, int N)
     ok = (fclose(fp) == 0) && ok;
     return ok ? BNA_OK : BNA_ERR_IO;
 }
+
+// ------------------------------------------------------------------ NIfTI
+// NIfTI-1 the way the reference reads it (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:
+// 141-238): the first 352 bytes taken as short[176], spatial dims hdr[21..23],
+// time points hdr[24], datatype hdr[35] (mask: 2 = uint8, 16 = float), bits per
+// value hdr[36] (series: 32 = float, 64 = double); voxel data from byte 352 on.
+namespace {
+constexpr int HDR_BYTES = 352;
+
+int read_hdr(FILE *fp, int16_t h[HDR_BYTES / 2])
+{
+    return fread(h, 1, HDR_BYTES, fp) == (size_t)HDR_BYTES ? BNA_OK : BNA_ERR_FORMAT;
+}
+
+int64_t file_bytes(FILE *fp)
+{
+    if (fseek(fp, 0, SEEK_END) != 0) return -1;
+    const long n = ftell(fp);
+    if (fseek(fp, HDR_BYTES, SEEK_SET) != 0) return -1;
+    return n;
+}
+}  // namespace
+
+extern "C" {
+
+int bna_nifti_mask(const char *mask_path, float mask_thr, uint8_t **selected, int64_t *total_voxels, int *N)
+{
+    if (!mask_path || !selected || !total_voxels || !N) return BNA_ERR_ARG;
+    *selected = nullptr;
+    FILE *fp = fopen(mask_path, "rb");
+    if (!fp) return BNA_ERR_IO;
+    int16_t h[HDR_BYTES / 2];
+    int rc = read_hdr(fp, h);
+    const int64_t total = (int64_t)h[21] * h[22] * h[23];
+    const int dt = h[35];
+    const int esize = dt == 2 ? 1 : (dt == 16 ? 4 : 0);
+    uint8_t *sel = nullptr;
+    int64_t n = 0;
+    if (rc == BNA_OK && (total <= 0 || esize == 0)) rc = BNA_ERR_FORMAT;
+    if (rc == BNA_OK && file_bytes(fp) < HDR_BYTES + total * esize) rc = BNA_ERR_FORMAT;
+    if (rc == BNA_OK) {
+        std::vector<unsigned char> raw((size_t)(total * esize));
+        sel = (uint8_t *)malloc((size_t)total);
+        if (!sel) rc = BNA_ERR_NOMEM;
+        else if (fread(raw.data(), 1, raw.size(), fp) != raw.size()) rc = BNA_ERR_FORMAT;
+        else
+            for (int64_t k = 0; k < total; k++) {
+                float m;
+                if (dt == 2) m = (float)raw[(size_t)k];
+                else memcpy(&m, &raw[(size_t)k * 4], 4);
+                sel[k] = m >= mask_thr;  // CorMat.cpp:177
+                n += sel[k];
+            }
+    }
+    fclose(fp);
+    if (rc == BNA_OK && n > 0x7fffffff) rc = BNA_ERR_OVERFLOW;
+    if (rc != BNA_OK) {
+        free(sel);
+        return rc;
+    }
+    *selected = sel;
+    *total_voxels = total;
+    *N = (int)n;
+    return BNA_OK;
+}
+
+int bna_nifti_bold(const char *path, const uint8_t *selected, int64_t total_voxels, int N, float **x, int *L)
+{
+    if (!path || !selected || !x || !L || total_voxels <= 0 || N < 0) return BNA_ERR_ARG;
+    *x = nullptr;
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return BNA_ERR_IO;
+    int16_t h[HDR_BYTES / 2];
+    int rc = read_hdr(fp, h);
+    const int Lr = h[24];
+    const int bits = h[36];
+    const int esize = bits == 32 ? 4 : (bits == 64 ? 8 : 0);
+    float *out = nullptr;
+    if (rc == BNA_OK && (Lr < 1 || esize == 0)) rc = BNA_ERR_FORMAT;
+    if (rc == BNA_OK && (int64_t)h[21] * h[22] * h[23] != total_voxels) rc = BNA_ERR_FORMAT;
+    if (rc == BNA_OK && file_bytes(fp) < HDR_BYTES + (int64_t)Lr * total_voxels * esize) rc = BNA_ERR_FORMAT;
+    if (rc == BNA_OK) {
+        out = (float *)malloc(sizeof(float) * (size_t)std::max<int64_t>((int64_t)Lr * N, 1));
+        if (!out) rc = BNA_ERR_NOMEM;
+    }
+    if (rc == BNA_OK) {
+        // one time point (a whole volume) at a time: BOLD[l*N + i] = In[l*total + k]
+        // for the selected voxels k in ascending order (CorMat.cpp:212-219)
+        std::vector<unsigned char> vol((size_t)(total_voxels * esize));
+        for (int l = 0; l < Lr && rc == BNA_OK; l++) {
+            if (fread(vol.data(), 1, vol.size(), fp) != vol.size()) {
+                rc = BNA_ERR_FORMAT;
+                break;
+            }
+            float *row = out + (int64_t)l * N;
+            int64_t i = 0;
+            for (int64_t k = 0; k < total_voxels; k++)
+                if (selected[k]) {
+                    if (esize == 4) {
+                        float v;
+                        memcpy(&v, &vol[(size_t)k * 4], 4);
+                        row[i++] = v;
+                    } else {
+                        double v;
+                        memcpy(&v, &vol[(size_t)k * 8], 8);
+                        row[i++] = (float)v;
+                    }
+                }
+            if (i != N) rc = BNA_ERR_ARG;
+        }
+    }
+    fclose(fp);
+    if (rc != BNA_OK) {
+        free(out);
+        return rc;
+    }
+    *x = out;
+    *L = Lr;
+    return BNA_OK;
+}
+
+void bna_free(void *p) { free(p); }
+
+}  // extern "C"

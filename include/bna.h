@@ -185,6 +185,20 @@ int bna_read_csr(const char *path, bna_csr *out);
 /* .nm node maps (Degree.cpp:95-99, weighted-net.cu:305-322): int32 N, float[N]. */
 int bna_write_nm(const char *path, const float *v, int N);
 
+/* NIfTI-1 ingest as CUCorMat does it (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:141-238):
+ * the 352-byte header read as short[176] (dims hdr[21..23], time points hdr[24],
+ * datatype hdr[35], bits hdr[36]), data from byte 352.
+ * bna_nifti_mask: mask of uint8 (datatype 2) or float (16); voxel k is selected
+ * iff mask[k] >= mask_thr (CorMat.cpp:177).  selected gets total_voxels bytes.
+ * bna_nifti_bold: float (32-bit) or double (64-bit) series of the same volume
+ * size; x gets the selected voxels in ascending order, time-major x[l*N + i]
+ * (CorMat.cpp:212-219; doubles narrowed to float).  Stricter than the
+ * reference: a short file or a volume-size mismatch is BNA_ERR_FORMAT.
+ * Buffers are released with bna_free. */
+int bna_nifti_mask(const char *mask_path, float mask_thr, uint8_t **selected, int64_t *total_voxels, int *N);
+int bna_nifti_bold(const char *path, const uint8_t *selected, int64_t total_voxels, int N, float **x, int *L);
+void bna_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""The command-line tools end to end on the GPU: NIfTI directory in, .csr / .nm /
+_evalue.txt files out, checked against the oracle on the ingested series.
+
+File names follow CorMat.cpp:294-350 (`<stem>_<N>_spa%.3f%%_cor%.3f[_weighted].csr`
+under weighted/ or unweighted/), Degree.cpp:93 (`_deg.nm`) and
+weighted-net.cu:300-301 (`_ec.nm`, `_evalue.txt`).
+"""
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cuda_newest20160331_amd as bna
+import orc
+from nifti import write_nii
+
+pytestmark = pytest.mark.gpu
+BIN = Path(bna.__file__).resolve().parent / "bin"
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run(tool, *args):
+    say(f"  run {tool} {' '.join(map(str, args))}")
+    r = subprocess.run([str(BIN / tool), *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+@pytest.fixture(scope="module")
+def subjects(tmp_path_factory):
+    d = tmp_path_factory.mktemp("bold")
+    X, Y, Z = 16, 16, 10
+    rng = np.random.default_rng(11)
+    mask = (rng.random((X, Y, Z)) < 0.85).astype(np.uint8)
+    write_nii(d / "mask.nii", mask)
+    out = {}
+    for name, L, dt, seed in (("subjA", 150, np.float32, 5), ("subjB", 96, np.float64, 6)):
+        ser = bna.synth_factor_model(X * Y * Z, L, 8, 0.7, seed)  # (L, voxels)
+        vol = ser.T.reshape(X, Y, Z, L, order="F").astype(dt)
+        write_nii(d / f"{name}.nii", vol)
+        out[name] = L
+    return d, mask, out
+
+
+def ingest(d, name):
+    sel, N = bna.nifti_mask(str(d / "mask.nii"), 0.5)
+    return bna.nifti_bold(str(d / f"{name}.nii"), sel), N
+
+
+def test_cormat_r_threshold_weighted_files(subjects):
+    d, mask, Ls = subjects
+    say("cli: r thresholds, weighted")
+    run("bna_cormat", d, d / "mask.nii", 0.5, "n", "n", "r", 0.5, 0.45)
+    for name in Ls:
+        x, N = ingest(d, name)
+        z64, _, _, _, valid = orc.znorm(x, 0)
+        M1 = N * (N - 1) / 2
+        for thr in (0.45, 0.5):
+            og = orc.build_threshold(z64, valid, np.float32(thr))
+            spa = np.float32(100.0 * og.nnz / M1 / 2)
+            f = d / "weighted" / f"{name}_{N}_spa{spa:.3f}%_cor{np.float32(thr):.3f}_weighted.csr"
+            assert f.exists(), sorted(p.name for p in (d / "weighted").iterdir())
+            g = bna.read_csr(f)
+            assert np.array_equal(g.R, og.R) and np.array_equal(g.C, og.C)
+            assert np.abs(g.V.astype(np.float64) - og.V64).max(initial=0) <= 1e-6
+
+
+def test_cormat_density_binary_and_maskless_form(subjects):
+    d, mask, Ls = subjects
+    say("cli: density, binary, mask path left out")
+    run("bna_cormat", "--binary", d, 0.5, "n", "n", "s", 2.0)
+    for name in Ls:
+        x, N = ingest(d, name)
+        z64, _, _, _, valid = orc.znorm(x, 0)
+        k = orc.density_k(N, np.float32(2.0))
+        og = orc.build_density(z64, valid, k)
+        f = d / "unweighted" / f"{name}_{N}_spa{np.float32(2.0):.3f}%_cor{np.float32(og.r_star):.3f}.csr"
+        assert f.exists(), sorted(p.name for p in (d / "unweighted").iterdir())
+        g = bna.read_csr(f)
+        assert g.V is None
+        assert np.array_equal(g.R, og.R) and np.array_equal(g.C, og.C)
+
+
+def test_degree_and_ec_tools(subjects):
+    d, mask, Ls = subjects
+    say("cli: degree + EC over the weighted and unweighted outputs")
+    for sub in ("weighted", "unweighted"):
+        if not (d / sub).exists() or not list((d / sub).glob("*.csr")):
+            pytest.skip("construction outputs missing")
+        run("bna_degree", d / sub)
+        run("bna_ec", d / sub)
+        iso = []
+        for f in sorted((d / sub).glob("*.csr")):
+            g = bna.read_csr(f)
+            R64 = g.R.astype(np.int64)
+            deg = bna.read_nm(str(f)[:-4] + "_deg.nm")
+            assert np.array_equal(deg, orc.degree(R64, g.V, g.N))
+            iso.append(int(np.sum(np.diff(R64) == 0)))
+            ec = bna.read_nm(str(f)[:-4] + "_ec.nm")
+            oec, olam, _ = orc.eigcent(R64, g.C, g.V, g.N, out_mode=0 if g.V is not None else 1)
+            assert np.linalg.norm(ec - oec) <= 1e-5 * np.linalg.norm(oec)
+            lam = float(Path(str(f)[:-4] + "_evalue.txt").read_text())
+            assert abs(lam - olam) <= 1e-5 * abs(olam)  # printed with 6 significant digits
+        if sub == "unweighted":
+            lines = (d / sub / "isolated_v_mark.txt").read_text().split()
+            assert list(map(int, lines)) == iso

@@ -1,0 +1,95 @@
+"""NIfTI + mask ingest (CorMat.cpp:141-238) and the command-line tools' argument
+handling, without a GPU.
+
+Closed form for the ingest: voxel k of a volume stored x-fastest is selected iff
+mask[k] >= thr, selected voxels keep ascending k, and BOLD[l*N + i] is time point
+l of the i-th selected voxel (doubles narrowed to float).
+"""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cuda_newest20160331_amd as bna
+from nifti import write_nii
+
+BIN = Path(bna.__file__).resolve().parent / "bin"
+
+
+def _expected(data, mask, thr):
+    X, Y, Z, T = data.shape
+    flat = data.reshape(X * Y * Z, T, order="F")  # row k = voxel x + X*(y + Y*z)
+    sel = mask.reshape(-1, order="F").astype(np.float32) >= np.float32(thr)
+    return sel, np.ascontiguousarray(flat[sel].T.astype(np.float32))
+
+
+@pytest.mark.parametrize("mask_dtype", [np.uint8, np.float32])
+@pytest.mark.parametrize("data_dtype", [np.float32, np.float64])
+def test_nifti_ingest_matches_closed_form(tmp_path, mask_dtype, data_dtype):
+    rng = np.random.default_rng(4)
+    X, Y, Z, T = 7, 5, 4, 23
+    mask = (rng.random((X, Y, Z)) * 2).astype(mask_dtype) if mask_dtype == np.uint8 else \
+        rng.random((X, Y, Z)).astype(np.float32)
+    data = rng.standard_normal((X, Y, Z, T)).astype(data_dtype)
+    write_nii(tmp_path / "mask.nii", mask)
+    write_nii(tmp_path / "s1.nii", data)
+    thr = 0.5
+    sel, N = bna.nifti_mask(str(tmp_path / "mask.nii"), thr)
+    esel, ex = _expected(data, mask, thr)
+    assert N == int(esel.sum()) and np.array_equal(sel.astype(bool), esel)
+    x = bna.nifti_bold(str(tmp_path / "s1.nii"), sel)
+    assert x.shape == (T, N)
+    assert np.array_equal(x, ex)
+
+
+def test_nifti_ingest_rejects_bad_files(tmp_path):
+    rng = np.random.default_rng(1)
+    mask = np.ones((4, 4, 2), np.uint8)
+    write_nii(tmp_path / "mask.nii", mask)
+    sel, N = bna.nifti_mask(str(tmp_path / "mask.nii"), 0.5)
+    assert N == 32
+    # volume size differs from the mask's
+    write_nii(tmp_path / "other.nii", rng.standard_normal((4, 4, 3, 5)).astype(np.float32))
+    with pytest.raises(bna.BnaError):
+        bna.nifti_bold(str(tmp_path / "other.nii"), sel)
+    # truncated data
+    write_nii(tmp_path / "short.nii", rng.standard_normal((4, 4, 2, 5)).astype(np.float32))
+    b = (tmp_path / "short.nii").read_bytes()
+    (tmp_path / "short.nii").write_bytes(b[:-8])
+    with pytest.raises(bna.BnaError):
+        bna.nifti_bold(str(tmp_path / "short.nii"), sel)
+    # series of 8-bit values (neither float nor double)
+    write_nii(tmp_path / "u8.nii", np.ones((4, 4, 2, 5), np.uint8))
+    with pytest.raises(bna.BnaError):
+        bna.nifti_bold(str(tmp_path / "u8.nii"), sel)
+    # mask of double values (only uint8 and float masks are read)
+    write_nii(tmp_path / "m64.nii", np.ones((4, 4, 2), np.float64))
+    with pytest.raises(bna.BnaError):
+        bna.nifti_mask(str(tmp_path / "m64.nii"), 0.5)
+    with pytest.raises(bna.BnaError):
+        bna.nifti_mask(str(tmp_path / "missing.nii"), 0.5)
+
+
+def _run(tool, *args):
+    return subprocess.run([str(BIN / tool), *map(str, args)], capture_output=True, text=True, timeout=60)
+
+
+@pytest.mark.skipif(not (BIN / "bna_cormat").exists(), reason="tools not built")
+def test_cli_argument_handling(tmp_path):
+    r = _run("bna_cormat", tmp_path)
+    assert r.returncode == 1 and "usage" in r.stderr
+    write_nii(tmp_path / "mask.nii", np.ones((2, 2, 2), np.uint8))
+    # group averaging and the dense dump are refused before any device work
+    r = _run("bna_cormat", tmp_path, tmp_path / "mask.nii", 0.5, "yf", "n", "r", 0.3)
+    assert r.returncode == 2 and "averaging" in r.stderr
+    r = _run("bna_cormat", tmp_path, 0.5, "n", "y", "r", 0.3)
+    assert r.returncode == 2 and "cormat" in r.stderr
+    r = _run("bna_cormat", tmp_path, 0.5, "n", "n", "q", 0.3)
+    assert r.returncode == 1 and "threshold type" in r.stderr
+    r = _run("bna_cormat", "--precision=fp64", tmp_path, 0.5, "n", "n", "r", 0.3)
+    assert r.returncode == 1 and "precision" in r.stderr
+    r = _run("bna_degree", tmp_path / "nope")
+    assert r.returncode == 1
+    r = _run("bna_ec")
+    assert r.returncode == 1 and "usage" in r.stderr
