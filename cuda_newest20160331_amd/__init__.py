@@ -30,6 +30,7 @@ EXPORTS = [
     "bna_build_csr_series", "bna_csr_to_host", "bna_series_download", "bna_degree",
     "bna_eigcent", "bna_density_to_k", "bna_synth_factor_model", "bna_write_csr",
     "bna_read_csr", "bna_write_nm", "bna_nifti_mask", "bna_nifti_bold", "bna_free",
+    "bna_build_csr_batch", "bna_trim",
 ]
 
 
@@ -103,6 +104,9 @@ def lib() -> C.CDLL:
         "bna_nifti_bold": (i32, [C.c_char_p, C.POINTER(C.c_uint8), i64, i32,
                                  C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_int)]),
         "bna_free": (None, [vp]),
+        "bna_build_csr_batch": (i32, [vp, C.POINTER(vp), C.POINTER(Opts), i32, C.POINTER(CSR),
+                                      C.POINTER(C.c_int)]),
+        "bna_trim": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -376,3 +380,34 @@ class Series:
             return self.to_host(g)
         finally:
             self.free_device(g)
+
+
+def build_batch_device(ctx: Context, series: list, opts: list) -> tuple[list, list]:
+    """bna_build_csr_batch over Series objects: (device CSRs, per-subject status)."""
+    n = len(series)
+    hs = (C.c_void_p * n)(*[s._h.value for s in series])
+    os_ = (Opts * n)(*opts)
+    out = (CSR * n)()
+    st = (C.c_int * n)()
+    rc = ctx._L.bna_build_csr_batch(ctx._h, hs, os_, n, out, st)
+    if rc and all(v == 0 for v in st):
+        ctx._check(rc, "bna_build_csr_batch")
+    return [out[q] for q in range(n)], list(st)
+
+
+def build_batch(ctx: Context, series: list, opts: list) -> list:
+    """Host Graphs (or a BnaError per failed subject) for a batch."""
+    gs, st = build_batch_device(ctx, series, opts)
+    res = []
+    for g, rc in zip(gs, st):
+        if rc:
+            res.append(BnaError(rc, "subject failed"))
+            continue
+        h = CSR()
+        ctx._check(ctx._L.bna_csr_to_host(ctx._h, C.byref(g), C.byref(h)), "bna_csr_to_host")
+        try:
+            res.append(Graph(h))
+        finally:
+            ctx._L.bna_csr_free(C.byref(h))
+            ctx._L.bna_csr_free(C.byref(g))
+    return res

@@ -508,41 +508,73 @@ static double decode_desc_key(unsigned long long k)
     return r;
 }
 
-// One construction.  Threshold mode (k < 0): edge iff r > thr_eff, with pairs
-// within the error band of thr_eff rechecked in FP64.  Density mode (k >= 0):
-// exactly k edges, chosen by a histogram pass, a bracketed emission pass, an
-// FP64 recheck of the bracket and an exact radix select.
-static int build_graph(bna_ctx *ctx, bna_series *s, const bna_opts *o, double thr_eff, int64_t k_density,
-                       bna_csr *out, bool timing_from_norm)
+// ------------------------------------------------------------ batched build
+// One subject's construction inside a batch.  Threshold mode (k < 0): edge iff
+// r > thr_eff, with pairs within the error band of thr_eff rechecked in FP64.
+// Density mode (k >= 0): exactly k edges, from a histogram pass, a bracketed
+// emission pass, an FP64 recheck of the bracket and an exact radix select.
+// The tiles of every subject in a batch go through one histogram launch and
+// one emission launch; the per-subject tails follow on the same stream.
+struct Job {
+    bna_series *s = nullptr;
+    bna_opts o{};
+    double thr_eff = 0.0;
+    int64_t k = -1;  // density: edges wanted; -1: threshold mode
+    int rc = BNA_OK;
+    bool done = false;
+    double band = 0.0, band_used = 0.0;
+    std::string pre;  // workspace name prefix
+    SubjDesc h{};
+    unsigned long long *ctr = nullptr;  // [0] entries, [1] candidates, [2] max err, [3] flips, [4] pool
+    unsigned long long *hist = nullptr;
+    SelState *sel = nullptr;
+    double *r64c = nullptr;
+    int64_t ecap = 0, ccap = 0, ocap = 0;
+    int64_t *rowlen = nullptr, *R64 = nullptr, *bsum = nullptr, *off_rm = nullptr;
+    int32_t *orig_rm = nullptr, *live_rm = nullptr;
+    int32_t *R32 = nullptr, *Cd = nullptr;  // outputs, owned by the job until handed out
+    float *Vd = nullptr;
+    unsigned long long hc[8] = {};
+    SelState hs{};
+    int64_t nnz = 0;
+};
+
+template <typename T>
+static int job_ws(bna_ctx *ctx, Job &j, const char *name, size_t count, T **out)
 {
-    const int N = s->N, nb = s->N_pad / TILE, nb1 = nb + 1;
-    const int prec = o->precision;
-    const bool recheck = o->recheck != 0;
-    const bool density = k_density >= 0;
-    const double band = band_for(prec, s->L_pad);
+    const std::string n = j.pre + name;
+    return ws_get_t(ctx, n.c_str(), count, out);
+}
 
-    TileItem *tiles = nullptr;
-    int64_t ntiles = 0;
-    int rc = get_tile_table(ctx, nb, &tiles, &ntiles);
-    if (rc) return rc;
-
+static int job_prepare(bna_ctx *ctx, Job &j)
+{
+    bna_series *s = j.s;
+    const int N = s->N, nb = s->N_pad / TILE;
+    const size_t nfrag = (size_t)N * (nb + 1);
+    int rc;
     int32_t *frag_orig, *frag_live;
     int64_t *frag_off;
-    const size_t nfrag = (size_t)N * nb1;
-    if ((rc = ws_get_t(ctx, "frag_orig", nfrag, &frag_orig))) return rc;
-    if ((rc = ws_get_t(ctx, "frag_live", nfrag, &frag_live))) return rc;
-    if ((rc = ws_get_t(ctx, "frag_off", nfrag, &frag_off))) return rc;
-    unsigned long long *ctr;  // [0] entries, [1] candidates, [2] max err bits, [3] flips, [4] overflow pool
-    if ((rc = ws_get_t(ctx, "counters", 8, &ctr))) return rc;
-    SubjDesc *dsd;
-    if ((rc = ws_get_t(ctx, "subjdesc", 1, &dsd))) return rc;
-
+    if ((rc = job_ws(ctx, j, "frag_orig", nfrag, &frag_orig)) || (rc = job_ws(ctx, j, "frag_live", nfrag, &frag_live)) ||
+        (rc = job_ws(ctx, j, "frag_off", nfrag, &frag_off)) || (rc = job_ws(ctx, j, "counters", 8, &j.ctr)) ||
+        (rc = job_ws(ctx, j, "rowlen", (size_t)N, &j.rowlen)) || (rc = job_ws(ctx, j, "R64", (size_t)N + 1, &j.R64)) ||
+        (rc = job_ws(ctx, j, "scan_bsum", (size_t)scan_blocks(N) + 1, &j.bsum)) ||
+        (rc = job_ws(ctx, j, "frag_orig_rm", nfrag, &j.orig_rm)) ||
+        (rc = job_ws(ctx, j, "frag_live_rm", nfrag, &j.live_rm)) ||
+        (rc = job_ws(ctx, j, "frag_off_rm", nfrag, &j.off_rm)))
+        return rc;
+    if (j.k >= 0) {
+        if ((rc = job_ws(ctx, j, "hist", HIST_BINS, &j.hist)) || (rc = job_ws(ctx, j, "selstate", 1, &j.sel)))
+            return rc;
+    }
     // capacities: start from what the workspace already holds
-    int64_t ecap = std::max<int64_t>(1 << 20, (int64_t)(ctx->ws["entries"].bytes / sizeof(Entry)));
-    int64_t ccap = std::max<int64_t>(1 << 16, (int64_t)(ctx->ws["cands"].bytes / sizeof(Candidate)));
-    int64_t ocap = std::max<int64_t>(1 << 16, (int64_t)(ctx->ws["kept_ovf"].bytes / sizeof(KeptPair)));
-
-    SubjDesc h{};
+    auto held = [&](const char *n, size_t unit) { return (int64_t)(ctx->ws[j.pre + n].bytes / unit); };
+    j.ecap = std::max<int64_t>(1 << 20, held("entries", sizeof(Entry)));
+    j.ccap = std::max<int64_t>(1 << 16, held("cands", sizeof(Candidate)));
+    j.ocap = std::max<int64_t>(1 << 16, held("kept_ovf", sizeof(KeptPair)));
+    j.band = band_for(j.o.precision, s->L_pad);
+    j.band_used = j.o.recheck ? j.band : 0.0;
+    SubjDesc &h = j.h;
+    h = SubjDesc{};
     h.a0 = s->panel;
     h.a1 = nullptr;
     h.N = N;
@@ -552,118 +584,162 @@ static int build_graph(bna_ctx *ctx, bna_series *s, const bna_opts *o, double th
     h.frag_orig = frag_orig;
     h.frag_live = frag_live;
     h.frag_off = frag_off;
-    h.entry_count = ctr;
-    h.cand_count = ctr + 1;
-    h.ovf_count = ctr + 4;
+    h.entry_count = j.ctr;
+    h.cand_count = j.ctr + 1;
+    h.ovf_count = j.ctr + 4;
     h.hist = nullptr;
+    if (j.k < 0) {
+        if (j.o.recheck) {
+            h.keep_thr = f_below(j.thr_eff - j.band);
+            h.sure_thr = f_ceil_above(j.thr_eff + j.band);
+        } else {
+            h.keep_thr = f_floor(j.thr_eff);
+            h.sure_thr = h.keep_thr;
+        }
+    }
+    return out_alloc(ctx, (void **)&j.R32, sizeof(int32_t) * ((size_t)N + 1));
+}
 
+// Tile table of a set of jobs: each job's band-ordered upper triangle, tagged
+// with its index in the SubjDesc array.
+static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **dsd, TileItem **tiles,
+                        int64_t *ntiles)
+{
+    std::vector<SubjDesc> hd;
+    for (Job *j : js) hd.push_back(j->h);
+    int rc;
+    if ((rc = ws_get_t(ctx, "batch_subjdesc", hd.size(), dsd))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(*dsd, hd.data(), hd.size() * sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
+    if (js.size() == 1) return get_tile_table(ctx, js[0]->s->N_pad / TILE, tiles, ntiles);  // cached
+    std::vector<TileItem> t;
+    for (size_t q = 0; q < js.size(); q++) make_tile_table(js[q]->s->N_pad / TILE, (int)q, t);
+    if ((rc = ws_get_t(ctx, "batch_tiles", t.size(), tiles))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(*tiles, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
+    *ntiles = (int64_t)t.size();
+    return BNA_OK;
+}
+
+static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_norm)
+{
+    const int prec = jobs[0].o.precision;
+    for (Job &j : jobs)
+        if (j.o.precision != prec) return fail(ctx, BNA_ERR_ARG, "a batch shares one precision");
     float ms_corr = 0.f, ms_re = 0.f, ms_sel = 0.f;
     int64_t launches = 0;
-    SelState *sel = nullptr;
-    double *r64c = nullptr;
-    const double band_used = recheck ? band : 0.0;
+    int rc;
+    for (size_t q = 0; q < jobs.size(); q++) {
+        Job &j = jobs[q];
+        j.pre = jobs.size() == 1 ? std::string() : "b" + std::to_string(q) + ":";
+        if (j.k == 0) {
+            j.rc = fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
+            continue;
+        }
+        if ((j.rc = job_prepare(ctx, j))) return j.rc;
+    }
 
-    if (density && k_density < 1)
-        return fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
-    if (density) {
-        // pass 1: histogram of the MFMA r over every valid pair
-        unsigned long long *hist;
-        if ((rc = ws_get_t(ctx, "hist", HIST_BINS, &hist))) return rc;
-        if ((rc = ws_get_t(ctx, "selstate", 1, &sel))) return rc;
-        h.hist = hist;
-        h.keep_thr = INFINITY;
-        h.sure_thr = INFINITY;
-        HIPCHK(ctx, hipMemsetAsync(hist, 0, HIST_BINS * sizeof(unsigned long long), ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(dsd, &h, sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
+    // pass 1 (density subjects): histogram of the MFMA r over every valid pair
+    std::vector<Job *> dj;
+    for (Job &j : jobs)
+        if (j.k > 0 && j.rc == BNA_OK) dj.push_back(&j);
+    if (!dj.empty()) {
+        for (Job *j : dj) {
+            j->h.hist = j->hist;
+            j->h.keep_thr = INFINITY;
+            j->h.sure_thr = INFINITY;
+            HIPCHK(ctx, hipMemsetAsync(j->hist, 0, HIST_BINS * sizeof(unsigned long long), ctx->stream));
+        }
+        SubjDesc *dsd;
+        TileItem *tiles;
+        int64_t nt;
+        if ((rc = upload_batch(ctx, dj, &dsd, &tiles, &nt))) return rc;
         HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        HIPCHK(ctx, launch_corr(prec, 1, dsd, tiles, ntiles, ctx->stream));
+        HIPCHK(ctx, launch_corr(prec, 1, dsd, tiles, nt, ctx->stream));
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        std::vector<unsigned long long> hh(HIST_BINS);
-        HIPCHK(ctx, hipMemcpyAsync(hh.data(), hist, HIST_BINS * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                   ctx->stream));
+        std::vector<unsigned long long> hh(HIST_BINS * dj.size());
+        for (size_t q = 0; q < dj.size(); q++)
+            HIPCHK(ctx, hipMemcpyAsync(hh.data() + q * HIST_BINS, dj[q]->hist, HIST_BINS * sizeof(unsigned long long),
+                                       hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ms_corr += ms;
-        h.hist = nullptr;
-        const DensityBracket br = density_bracket(hh.data(), k_density, band_used);
-        if (k_density < 1 || k_density > br.valid_pairs)
-            return fail(ctx, BNA_ERR_DENSITY, "density: " + std::to_string(k_density) + " edges requested, " +
-                                                  std::to_string(br.valid_pairs) + " valid pairs");
-        h.keep_thr = std::isfinite(br.s_lo) ? f_below(br.s_lo) : -INFINITY;  // emit v >= s_lo
-        h.sure_thr = std::isfinite(br.s_hi) ? f_below(br.s_hi) : INFINITY;   // candidate v < s_hi
-        ecap = std::max<int64_t>(ecap, 2 * br.emit_max + 4096);
-        ccap = std::max<int64_t>(ccap, br.cand_max + 4096);
-    } else if (recheck) {
-        h.keep_thr = f_below(thr_eff - band);
-        h.sure_thr = f_ceil_above(thr_eff + band);
-    } else {
-        h.keep_thr = f_floor(thr_eff);
-        h.sure_thr = h.keep_thr;
+        for (size_t q = 0; q < dj.size(); q++) {
+            Job &j = *dj[q];
+            j.h.hist = nullptr;
+            const DensityBracket br = density_bracket(hh.data() + q * HIST_BINS, j.k, j.band_used);
+            if (j.k > br.valid_pairs) {
+                j.rc = fail(ctx, BNA_ERR_DENSITY, "density: " + std::to_string(j.k) + " edges requested, " +
+                                                      std::to_string(br.valid_pairs) + " valid pairs");
+                continue;
+            }
+            j.h.keep_thr = std::isfinite(br.s_lo) ? f_below(br.s_lo) : -INFINITY;  // emit v >= s_lo
+            j.h.sure_thr = std::isfinite(br.s_hi) ? f_below(br.s_hi) : INFINITY;   // candidate v < s_hi
+            j.ecap = std::max<int64_t>(j.ecap, 2 * br.emit_max + 4096);
+            j.ccap = std::max<int64_t>(j.ccap, br.cand_max + 4096);
+        }
     }
 
-    unsigned long long hc[8];
-    SelState hs{};
-    int64_t nnz = 0;
-
-    // row lengths, scan and placement buffers
-    int64_t *rowlen, *R64, *bsum;
-    if ((rc = ws_get_t(ctx, "rowlen", (size_t)N, &rowlen))) return rc;
-    if ((rc = ws_get_t(ctx, "R64", (size_t)N + 1, &R64))) return rc;
-    if ((rc = ws_get_t(ctx, "scan_bsum", (size_t)scan_blocks(N) + 1, &bsum))) return rc;
-    int32_t *orig_rm, *live_rm;
-    int64_t *off_rm;
-    if ((rc = ws_get_t(ctx, "frag_orig_rm", nfrag, &orig_rm))) return rc;
-    if ((rc = ws_get_t(ctx, "frag_live_rm", nfrag, &live_rm))) return rc;
-    if ((rc = ws_get_t(ctx, "frag_off_rm", nfrag, &off_rm))) return rc;
-    int32_t *R32 = nullptr;
-    if ((rc = out_alloc(ctx, (void **)&R32, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
-
-    // One pass of correlation, recheck (and selection), row lengths and scan,
-    // then one host read of the counters: when any list overflowed its
-    // capacity, grow it and run the pass again (the recheck and the selection
-    // skip slots past the capacity).
+    // pass 2: emission, recheck (and selection), row lengths and scan; then one
+    // host read of every job's counters.  A job whose lists overflowed grows them
+    // and runs again (the recheck and the selection skip slots past capacity).
     for (int attempt = 0;; attempt++) {
-        Entry *entries;
-        Candidate *cands;
-        KeptPair *ovf;
-        if ((rc = ws_get_t(ctx, "entries", (size_t)ecap, &entries)) ||
-            (rc = ws_get_t(ctx, "cands", (size_t)ccap, &cands)) ||
-            (rc = ws_get_t(ctx, "kept_ovf", (size_t)ocap, &ovf)) ||
-            (density && (rc = ws_get_t(ctx, "cand_r64", (size_t)ccap, &r64c)))) {
-            (void)hipFree(R32);
-            return rc;
+        std::vector<Job *> act;
+        for (Job &j : jobs)
+            if (j.rc == BNA_OK && !j.done) act.push_back(&j);
+        if (act.empty()) break;
+        if (attempt == 3) return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
+        for (Job *j : act) {
+            Entry *entries;
+            Candidate *cands;
+            KeptPair *ovf;
+            if ((rc = job_ws(ctx, *j, "entries", (size_t)j->ecap, &entries)) ||
+                (rc = job_ws(ctx, *j, "cands", (size_t)j->ccap, &cands)) ||
+                (rc = job_ws(ctx, *j, "kept_ovf", (size_t)j->ocap, &ovf)) ||
+                (j->k > 0 && (rc = job_ws(ctx, *j, "cand_r64", (size_t)j->ccap, &j->r64c))))
+                return rc;
+            j->h.ovf = ovf;
+            j->h.ovf_cap = j->ocap;
+            j->h.entries = entries;
+            j->h.entry_cap = j->ecap;
+            j->h.cands = cands;
+            j->h.cand_cap = j->ccap;
+            HIPCHK(ctx, hipMemsetAsync(j->ctr, 0, 8 * sizeof(unsigned long long), ctx->stream));
         }
-        h.ovf = ovf;
-        h.ovf_cap = ocap;
-        h.entries = entries;
-        h.entry_cap = ecap;
-        h.cands = cands;
-        h.cand_cap = ccap;
-        HIPCHK(ctx, hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(dsd, &h, sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
+        SubjDesc *dsd;
+        TileItem *tiles;
+        int64_t nt;
+        if ((rc = upload_batch(ctx, act, &dsd, &tiles, &nt))) return rc;
         HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, ntiles, ctx->stream));
+        HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, nt, ctx->stream));
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        const float *xs = s->xv ? s->xv : s->x;
-        const int64_t ld = s->xv ? s->L_pad : s->L;
-        const int mode = density ? (recheck ? 1 : 2) : 0;
-        HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, h.cands, ctr + 1, h.cand_cap, thr_eff,
-                                   frag_live, h.entries, h.entry_cap, ctr + 2, mode, r64c, ctx->stream));
-        HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        if (density) {
-            HIPCHK(ctx, launch_select(sel, ctr, h.cands, r64c, h.cand_cap, k_density, N, frag_live, h.entries,
-                                      h.entry_cap, ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(&hs, sel, sizeof(SelState), hipMemcpyDeviceToHost, ctx->stream));
+        for (Job *j : act) {
+            bna_series *s = j->s;
+            const int N = s->N, nb = s->N_pad / TILE;
+            const float *xs = s->xv ? s->xv : s->x;
+            const int64_t ld = s->xv ? s->L_pad : s->L;
+            const int mode = j->k > 0 ? (j->o.recheck ? 1 : 2) : 0;
+            HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, j->h.cands, j->ctr + 1, j->h.cand_cap,
+                                       j->thr_eff, j->h.frag_live, j->h.entries, j->h.entry_cap, j->ctr + 2, mode,
+                                       j->r64c, ctx->stream));
         }
+        HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        for (Job *j : act)
+            if (j->k > 0) {
+                HIPCHK(ctx, launch_select(j->sel, j->ctr, j->h.cands, j->r64c, j->h.cand_cap, j->k, j->s->N,
+                                          j->h.frag_live, j->h.entries, j->h.entry_cap, ctx->stream));
+                HIPCHK(ctx, hipMemcpyAsync(&j->hs, j->sel, sizeof(SelState), hipMemcpyDeviceToHost, ctx->stream));
+            }
         HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        HIPCHK(ctx, launch_frag_rows(frag_orig, frag_live, frag_off, N, nb, orig_rm, live_rm, off_rm, rowlen,
-                                     ctx->stream));
-        HIPCHK(ctx, launch_scan(rowlen, N, R64, R32, bsum, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(&nnz, R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        for (Job *j : act) {
+            const int N = j->s->N, nb = j->s->N_pad / TILE;
+            HIPCHK(ctx, launch_frag_rows(j->h.frag_orig, j->h.frag_live, j->h.frag_off, N, nb, j->orig_rm, j->live_rm,
+                                         j->off_rm, j->rowlen, ctx->stream));
+            HIPCHK(ctx, launch_scan(j->rowlen, N, j->R64, j->R32, j->bsum, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(j->hc, j->ctr, sizeof(j->hc), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(&j->nnz, j->R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        }
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
@@ -672,47 +748,51 @@ static int build_graph(bna_ctx *ctx, bna_series *s, const bna_opts *o, double th
         ms_re += ms;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
         ms_sel += ms;
-        const bool e_over = (int64_t)hc[0] > ecap, c_over = (int64_t)hc[1] > ccap, o_over = (int64_t)hc[4] > ocap;
-        if (!e_over && !c_over && !o_over) break;
-        if (attempt == 2) {
-            (void)hipFree(R32);
-            return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
+        for (Job *j : act) {
+            const unsigned long long *hc = j->hc;
+            const bool e_over = (int64_t)hc[0] > j->ecap, c_over = (int64_t)hc[1] > j->ccap,
+                       o_over = (int64_t)hc[4] > j->ocap;
+            if (!e_over && !c_over && !o_over) {
+                j->done = true;
+                continue;
+            }
+            if (e_over) j->ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
+            if (c_over) j->ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
+            if (o_over) j->ocap = (int64_t)hc[4];
         }
-        if (e_over) ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
-        if (c_over) ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
-        if (o_over) ocap = (int64_t)hc[4];
     }
-    const int64_t ncand = (int64_t)hc[1];
-    double r_star = thr_eff;
-    if (density) {
-        if (hs.err || nnz != 2 * k_density) {
-            (void)hipFree(R32);
-            return fail(ctx, BNA_ERR_DENSITY,
-                        "density selection inconsistent (err " + std::to_string(hs.err) + ", nnz " +
-                            std::to_string(nnz) + ", k " + std::to_string(k_density) + ")");
+
+    // placement (and value refinement) of every finished job
+    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    for (size_t q = 0; q < jobs.size(); q++) {
+        Job &j = jobs[q];
+        if (j.rc != BNA_OK) continue;
+        if (j.k > 0 && (j.hs.err || j.nnz != 2 * j.k)) {
+            j.rc = fail(ctx, BNA_ERR_DENSITY, "density selection inconsistent (err " + std::to_string(j.hs.err) +
+                                                  ", nnz " + std::to_string(j.nnz) + ", k " + std::to_string(j.k) + ")");
+            continue;
         }
-        r_star = decode_desc_key(hs.kmax);
+        if (j.nnz > 0x7fffffffLL) {
+            j.rc = fail(ctx, BNA_ERR_OVERFLOW, "nnz exceeds int32 (" + std::to_string(j.nnz) + ")");
+            continue;
+        }
+        if ((j.rc = out_alloc(ctx, (void **)&j.Cd, sizeof(int32_t) * (size_t)std::max<int64_t>(j.nnz, 1))))
+            continue;
+        if (j.o.weighted &&
+            (j.rc = out_alloc(ctx, (void **)&j.Vd, sizeof(float) * (size_t)std::max<int64_t>(j.nnz, 1))))
+            continue;
+        bna_series *s = j.s;
+        HIPCHK(ctx, launch_place(j.orig_rm, j.live_rm, j.off_rm, j.h.entries, j.R64, s->N, s->N_pad / TILE, j.Cd,
+                                 j.Vd, ctx->stream));
     }
-    if (nnz > 0x7fffffffLL) {
-        (void)hipFree(R32);
-        return fail(ctx, BNA_ERR_OVERFLOW, "nnz exceeds int32 (" + std::to_string(nnz) + ")");
-    }
-    int32_t *Cd = nullptr;
-    float *Vd = nullptr;
-    if ((rc = out_alloc(ctx, (void **)&Cd, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) {
-        (void)hipFree(R32);
-        return rc;
-    }
-    if (o->weighted && (rc = out_alloc(ctx, (void **)&Vd, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)))) {
-        (void)hipFree(R32);
-        (void)hipFree(Cd);
-        return rc;
-    }
-    HIPCHK(ctx, launch_place(orig_rm, live_rm, off_rm, h.entries, R64, N, nb, Cd, Vd, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
-    if (Vd && o->refine && nnz > 0)
-        HIPCHK(ctx, launch_refine(R32, Cd, Vd, s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->mu, s->sd, N, s->L,
-                                  ctx->stream));
+    for (size_t q = 0; q < jobs.size(); q++) {
+        Job &j = jobs[q];
+        bna_series *s = j.s;
+        if (j.rc == BNA_OK && j.Vd && j.o.refine && j.nnz > 0)
+            HIPCHK(ctx, launch_refine(j.R32, j.Cd, j.Vd, s->xv ? s->xv : s->x,
+                                      s->xv ? s->L_pad : s->L, s->mu, s->sd, s->N, s->L, ctx->stream));
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 
@@ -720,67 +800,141 @@ static int build_graph(bna_ctx *ctx, bna_series *s, const bna_opts *o, double th
     HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[5], ctx->ev[6]));
     HIPCHK(ctx, hipEventElapsedTime(&ms_ref, ctx->ev[6], ctx->ev[7]));
     HIPCHK(ctx, hipEventElapsedTime(&ms_tot, timing_from_norm ? ctx->ev[0] : ctx->ev[1], ctx->ev[7]));
-    ctx->tm.refine_ms = ms_ref;
     ctx->tm.corr_ms = ms_corr;
     ctx->tm.recheck_ms = ms_re;
     ctx->tm.select_ms = ms_sel;
     ctx->tm.csr_ms = ms_csr;
+    ctx->tm.refine_ms = ms_ref;
     ctx->tm.total_ms = ms_tot;
     ctx->tm.corr_launches = launches;
-
-    memset(out, 0, sizeof(*out));
-    out->N = N;
-    out->nnz = nnz;
-    out->R = R32;
-    out->C = Cd;
-    out->V = Vd;
-    out->on_device = 1;
-    out->thr_used = r_star;
-    const double M1 = (double)N * (N - 1) / 2.0;
-    out->density_pct = 100.0 * (double)nnz / M1 / 2.0;
-    out->n_candidates = ncand;
-    out->n_flips = density ? 0 : (int64_t)hc[3];
-    out->band = band_used;
-    double me = 0.0;
-    memcpy(&me, &hc[2], sizeof(double));
-    out->max_err = me;
     return BNA_OK;
+}
+
+// Hands a finished job's arrays to out; a failed job's arrays are freed.
+static void job_result(Job &j, bna_csr *out)
+{
+    bna_series *s = j.s;
+    memset(out, 0, sizeof(*out));
+    if (j.rc != BNA_OK) {
+        if (j.R32) (void)hipFree(j.R32);
+        if (j.Cd) (void)hipFree(j.Cd);
+        if (j.Vd) (void)hipFree(j.Vd);
+        j.R32 = j.Cd = nullptr;
+        j.Vd = nullptr;
+        return;
+    }
+    out->N = s->N;
+    out->nnz = j.nnz;
+    out->R = j.R32;
+    out->C = j.Cd;
+    out->V = j.Vd;
+    j.R32 = j.Cd = nullptr;
+    j.Vd = nullptr;
+    out->on_device = 1;
+    out->thr_used = j.k > 0 ? decode_desc_key(j.hs.kmax) : j.thr_eff;
+    const double M1 = (double)s->N * (s->N - 1) / 2.0;
+    out->density_pct = 100.0 * (double)j.nnz / M1 / 2.0;
+    out->n_candidates = (int64_t)j.hc[1];
+    out->n_flips = j.k > 0 ? 0 : (int64_t)j.hc[3];
+    out->band = j.band_used;
+    double me = 0.0;
+    memcpy(&me, &j.hc[2], sizeof(double));
+    out->max_err = me;
 }
 
 extern "C" {
 
+// Sets up a job from options; normalises the series when its operand panel is
+// not of the requested precision.
+static int job_init(bna_ctx *ctx, Job &j, bna_series *s, const bna_opts *o, bool *normed)
+{
+    if (o->precision < BNA_FP32 || o->precision > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bad precision");
+    j.s = s;
+    j.o = *o;
+    if (o->thr_type == BNA_THR_R) {
+        j.thr_eff = o->thr - (o->ref_compat_eps ? 1e-6 : 0.0);
+        j.k = -1;
+    } else if (o->thr_type == BNA_THR_DENSITY) {
+        if (!(o->thr >= 0.0 && o->thr <= 100.0)) return fail(ctx, BNA_ERR_ARG, "density must be in [0, 100] percent");
+        j.k = bna_density_to_k(s->N, (float)o->thr);
+    } else {
+        return fail(ctx, BNA_ERR_ARG, "bad thr_type");
+    }
+    if (s->prec_ready != o->precision) {
+        int rc = series_normalize(ctx, s, o->precision);
+        if (rc) return rc;
+        *normed = true;
+    }
+    return BNA_OK;
+}
+
 int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *o, bna_csr *out)
 {
     if (!ctx || !s || !o || !out) return fail(ctx, BNA_ERR_ARG, "bna_build_csr_series: NULL argument");
-    if (o->precision < BNA_FP32 || o->precision > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bad precision");
+    memset(out, 0, sizeof(*out));
     HIPCHK(ctx, hipSetDevice(ctx->device));
     memset(&ctx->tm, 0, sizeof(ctx->tm));
     bool normed = false;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    if (s->prec_ready != o->precision) {
-        int rc = series_normalize(ctx, s, o->precision);
-        if (rc) return rc;
-        normed = true;
-    }
+    std::vector<Job> jobs(1);
+    int rc = job_init(ctx, jobs[0], s, o, &normed);
+    if (rc) return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    if (o->thr_type == BNA_THR_R) {
-        double thr = o->thr;
-        if (o->ref_compat_eps) thr -= 1e-6;
-        int rc = build_graph(ctx, s, o, thr, -1, out, normed);
-        if (rc) return rc;
-    } else if (o->thr_type == BNA_THR_DENSITY) {
-        if (!(o->thr >= 0.0 && o->thr <= 100.0)) return fail(ctx, BNA_ERR_ARG, "density must be in [0, 100] percent");
-        const int64_t k = bna_density_to_k(s->N, (float)o->thr);
-        int rc = build_graph(ctx, s, o, 0.0, k, out, normed);
-        if (rc) return rc;
-    } else {
-        return fail(ctx, BNA_ERR_ARG, "bad thr_type");
-    }
+    rc = build_batch(ctx, jobs, normed);
+    if (rc == BNA_OK) rc = jobs[0].rc;
+    job_result(jobs[0], out);
+    if (rc) return rc;
     if (normed) {
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
         ctx->tm.znorm_ms = ms;
     }
+    return BNA_OK;
+}
+
+int bna_build_csr_batch(bna_ctx *ctx, bna_series *const *series, const bna_opts *opts, int n, bna_csr *out,
+                        int *status)
+{
+    if (!ctx || !series || !opts || !out || n < 1) return fail(ctx, BNA_ERR_ARG, "bna_build_csr_batch: bad argument");
+    for (int q = 0; q < n; q++) {
+        memset(&out[q], 0, sizeof(out[q]));
+        if (status) status[q] = BNA_OK;
+        if (!series[q]) return fail(ctx, BNA_ERR_ARG, "bna_build_csr_batch: NULL series");
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    memset(&ctx->tm, 0, sizeof(ctx->tm));
+    bool normed = false;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    std::vector<Job> jobs(n);
+    for (int q = 0; q < n; q++) {
+        int rc = job_init(ctx, jobs[q], series[q], &opts[q], &normed);
+        if (rc) return rc;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    int rc = build_batch(ctx, jobs, normed);
+    int first = BNA_OK;
+    for (int q = 0; q < n; q++) {
+        if (rc && jobs[q].rc == BNA_OK) jobs[q].rc = rc;
+        if (status) status[q] = jobs[q].rc;
+        if (first == BNA_OK) first = jobs[q].rc;
+        job_result(jobs[q], &out[q]);
+    }
+    if (normed && rc == BNA_OK) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->tm.znorm_ms = ms;
+    }
+    return rc ? rc : first;
+}
+
+int bna_trim(bna_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_trim: ctx is NULL");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &kv : ctx->ws) dev_free(ctx, kv.second.p, kv.second.bytes);
+    ctx->ws.clear();
+    ctx->tt_nb = -1;
+    ctx->tt_count = 0;
     return BNA_OK;
 }
 

@@ -144,6 +144,17 @@ int bna_series_destroy(bna_ctx *ctx, bna_series *s);
 int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision);
 /* Correlate + threshold + CSR on a normalised series; out->on_device = 1. */
 int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *opts, bna_csr *out);
+/* Several subjects in one pass of batched launches (the batched form of the
+ * reference's per-subject loop, CorMat.cpp:195-384): every subject's tiles go
+ * through one histogram launch (density subjects) and one emission launch of
+ * the tile kernel, then each subject's recheck, selection and CSR steps.
+ * opts[q] per subject, one precision for the batch; out[q] gets a device CSR
+ * and status[q] (may be NULL) that subject's status.  Returns BNA_OK when every
+ * subject succeeded, else the first failure. */
+int bna_build_csr_batch(bna_ctx *ctx, bna_series *const *series, const bna_opts *opts, int n, bna_csr *out,
+                        int *status);
+/* Releases the context's device workspace (it regrows on the next build). */
+int bna_trim(bna_ctx *ctx);
 /* Copies a device CSR to host arrays (out gets fresh host buffers). */
 int bna_csr_to_host(bna_ctx *ctx, const bna_csr *dev, bna_csr *out);
 /* Normalised series back to the host, for tests: z as double (N x L,
