@@ -30,7 +30,9 @@ EXPORTS = [
     "bna_build_csr_series", "bna_csr_to_host", "bna_series_download", "bna_degree",
     "bna_eigcent", "bna_density_to_k", "bna_synth_factor_model", "bna_write_csr",
     "bna_read_csr", "bna_write_nm", "bna_nifti_mask", "bna_nifti_bold", "bna_free",
-    "bna_build_csr_batch", "bna_trim",
+    "bna_build_csr_batch", "bna_trim", "bna_server_config_default", "bna_server_create",
+    "bna_server_destroy", "bna_submit", "bna_poll", "bna_wait", "bna_cancel", "bna_server_get_stats",
+    "bna_result_free", "bna_request_footprint", "bna_memory_info",
 ]
 
 
@@ -411,3 +413,141 @@ def build_batch(ctx: Context, series: list, opts: list) -> list:
             ctx._L.bna_csr_free(C.byref(h))
             ctx._L.bna_csr_free(C.byref(g))
     return res
+
+
+# ------------------------------------------------------------------ serving
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("workers", C.c_int),
+                ("max_batch", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("N", C.c_int), ("L", C.c_int), ("layout", C.c_int),
+                ("opts", Opts), ("want_degree", C.c_int), ("want_ec", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("error", C.c_char * 256), ("csr", CSR),
+                ("degree", C.POINTER(C.c_float)), ("ec", C.POINTER(C.c_double)),
+                ("lam", C.c_double), ("ec_iters", C.c_int), ("batch_size", C.c_int),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "submitted", "completed", "failed", "cancelled", "rounds", "queued",
+        "max_concurrent_rounds", "max_batch_seen", "max_reserved", "budget")]
+
+
+class Served:
+    """A finished request: the graph, optional degree / EC, and its timings."""
+
+    def __init__(self, r: Result, N: int):
+        self.status = r.status
+        self.error = r.error.decode(errors="replace")
+        self.graph = Graph(r.csr) if r.status == 0 else None
+        self.degree = np.ctypeslib.as_array(r.degree, shape=(N,)).copy() if r.degree else None
+        self.ec = np.ctypeslib.as_array(r.ec, shape=(N,)).copy() if r.ec else None
+        self.lam = r.lam
+        self.ec_iters = r.ec_iters
+        self.batch_size = r.batch_size
+        self.queue_ms, self.run_ms, self.latency_ms = r.queue_ms, r.run_ms, r.latency_ms
+
+
+_SERVER_SIGS = None
+
+
+def _server_lib():
+    global _SERVER_SIGS
+    L = lib()
+    if _SERVER_SIGS is None:
+        vp = C.c_void_p
+        sigs = {
+            "bna_server_config_default": (None, [C.POINTER(ServerConfig)]),
+            "bna_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(vp)]),
+            "bna_server_destroy": (C.c_int, [vp]),
+            "bna_submit": (C.c_int, [vp, C.POINTER(Request), C.POINTER(C.c_int64)]),
+            "bna_poll": (C.c_int, [vp, C.c_int64, C.POINTER(Result)]),
+            "bna_wait": (C.c_int, [vp, C.c_int64, C.POINTER(Result)]),
+            "bna_cancel": (C.c_int, [vp, C.c_int64]),
+            "bna_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+            "bna_result_free": (None, [C.POINTER(Result)]),
+            "bna_request_footprint": (C.c_int64, [C.c_int, C.c_int, C.POINTER(Opts)]),
+            "bna_memory_info": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _SERVER_SIGS = sigs
+    return L
+
+
+class Server:
+    """bna_server: submit series, collect graphs; rounds are batched on the GPU."""
+
+    BUSY, CANCELLED, NOT_FOUND = -9, -10, -8
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2, max_batch: int = 16):
+        self._L = _server_lib()
+        cfg = ServerConfig()
+        self._L.bna_server_config_default(C.byref(cfg))
+        cfg.device, cfg.hbm_budget, cfg.workers, cfg.max_batch = device, hbm_budget, workers, max_batch
+        self._h = C.c_void_p()
+        rc = self._L.bna_server_create(C.byref(cfg), C.byref(self._h))
+        if rc:
+            raise BnaError(rc, f"bna_server_create: {self._L.bna_last_error(None).decode()}")
+        self._N = {}
+
+    def close(self):
+        if self._h:
+            self._L.bna_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, x: np.ndarray, layout: int = TIME_MAJOR, degree: bool = False, ec: bool = False,
+               **kw) -> int:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        L, N = x.shape if layout == TIME_MAJOR else x.shape[::-1]
+        r = Request(x.ctypes.data, N, L, layout, make_opts(**kw), int(degree), int(ec))
+        rid = C.c_int64()
+        rc = self._L.bna_submit(self._h, C.byref(r), C.byref(rid))
+        if rc:
+            raise BnaError(rc, "bna_submit")
+        self._N[rid.value] = N
+        return rid.value
+
+    def _collect(self, fn, rid: int):
+        r = Result()
+        rc = fn(self._h, rid, C.byref(r))
+        if rc == self.BUSY or rc == self.NOT_FOUND:
+            return rc
+        try:
+            return Served(r, self._N.pop(rid))
+        finally:
+            self._L.bna_result_free(C.byref(r))
+
+    def poll(self, rid: int):
+        """A Served, or Server.BUSY / Server.NOT_FOUND."""
+        return self._collect(self._L.bna_poll, rid)
+
+    def wait(self, rid: int):
+        return self._collect(self._L.bna_wait, rid)
+
+    def cancel(self, rid: int) -> int:
+        return self._L.bna_cancel(self._h, rid)
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        self._L.bna_server_get_stats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+
+def request_footprint(N: int, L: int, **kw) -> int:
+    o = make_opts(**kw)
+    return int(_server_lib().bna_request_footprint(N, L, C.byref(o)))

@@ -240,6 +240,17 @@ int bna_get_timings(bna_ctx *ctx, bna_timings *t)
 
 int64_t bna_workspace_bytes(bna_ctx *ctx) { return ctx ? (int64_t)ctx->held : -1; }
 
+int bna_memory_info(bna_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
+{
+    if (!ctx || !free_bytes || !total_bytes) return fail(ctx, BNA_ERR_ARG, "bna_memory_info: NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(ctx, hipMemGetInfo(&fr, &tot));
+    *free_bytes = (int64_t)fr;
+    *total_bytes = (int64_t)tot;
+    return BNA_OK;
+}
+
 // ------------------------------------------------------------------ series
 int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, int x_on_device, bna_series **out)
 {

@@ -210,6 +210,67 @@ int bna_nifti_mask(const char *mask_path, float mask_thr, uint8_t **selected, in
 int bna_nifti_bold(const char *path, const uint8_t *selected, int64_t total_voxels, int N, float **x, int *L);
 void bna_free(void *p);
 
+/* ------------------------------------------------------------- serving */
+/* A request loop for many subjects on one GPU (no counterpart in the
+ * reference, which builds one subject at a time).  Requests are admitted in
+ * arrival order while their estimated device footprint fits the budget and are
+ * built in rounds (one bna_build_csr_batch per round) by worker threads that
+ * each own a context; rounds of different workers overlap on the device. */
+typedef struct bna_server bna_server;
+
+typedef struct bna_server_config {
+    int device;
+    size_t hbm_budget; /* bytes for admitted requests; 0 = 80 % of free device memory */
+    int workers;       /* rounds in flight at once (contexts), 1..8 */
+    int max_batch;     /* requests per round */
+} bna_server_config;
+
+typedef struct bna_request {
+    const float *x; /* copied at submit */
+    int N, L, layout;
+    bna_opts opts;
+    int want_degree, want_ec; /* EC: tol 1e-6, 10000 iterations, the reference output of the graph's kind */
+} bna_request;
+
+typedef struct bna_result {
+    int status;
+    char error[256];
+    bna_csr csr;      /* host arrays */
+    float *degree;    /* N floats or NULL */
+    double *ec;       /* N doubles or NULL */
+    double lambda;
+    int ec_iters;
+    int batch_size;   /* requests in the round it ran in */
+    double queue_ms;  /* submit -> round start */
+    double run_ms;    /* round start -> result ready */
+    double latency_ms;/* submit -> result ready */
+} bna_result;
+
+typedef struct bna_server_stats {
+    int64_t submitted, completed, failed, cancelled, rounds, queued;
+    int64_t max_concurrent_rounds, max_batch_seen;
+    int64_t max_reserved, budget; /* footprint bytes admitted at once, and the budget */
+} bna_server_stats;
+
+void bna_server_config_default(bna_server_config *c);
+int bna_server_create(const bna_server_config *cfg, bna_server **out);
+/* Cancels queued requests, waits for running rounds, frees uncollected results. */
+int bna_server_destroy(bna_server *srv);
+int bna_submit(bna_server *srv, const bna_request *req, int64_t *id);
+/* BNA_ERR_BUSY until the request is finished; then moves the result to out
+ * (release with bna_result_free) and forgets the id; returns its status. */
+int bna_poll(bna_server *srv, int64_t id, bna_result *out);
+int bna_wait(bna_server *srv, int64_t id, bna_result *out);
+/* Removes a queued request (BNA_ERR_BUSY once it runs). */
+int bna_cancel(bna_server *srv, int64_t id);
+int bna_server_get_stats(bna_server *srv, bna_server_stats *out);
+void bna_result_free(bna_result *r);
+/* Estimated device bytes of one request: series, operand panel, tables, edge
+ * lists and CSR (edges: the density target, or 2 % of pairs for an r threshold). */
+int64_t bna_request_footprint(int N, int L, const bna_opts *o);
+/* Free and total device memory of the context's device. */
+int bna_memory_info(bna_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,3 +93,22 @@ def test_cli_argument_handling(tmp_path):
     assert r.returncode == 1
     r = _run("bna_ec")
     assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_request_footprint_and_server_without_device():
+    a = bna.request_footprint(5000, 200, density=1.0)
+    b = bna.request_footprint(10000, 200, density=1.0)
+    c = bna.request_footprint(10000, 1200, density=1.0)
+    assert 0 < a < b < c
+    # series + operand panel + CSR at 1 % density are all inside the estimate
+    N, L = 45000, 240
+    est = bna.request_footprint(N, L, density=1.0)
+    assert est > 4 * N * L + 8 * 2 * 0.01 * N * (N - 1) / 2
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        with pytest.raises(bna.BnaError):
+            bna.Server()
