@@ -97,6 +97,74 @@ def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float):
     }
 
 
+def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
+    """The serving form: n_subj subjects with Poisson arrivals (mean `rate` per
+    second), N log-uniform in [5000, 150000], L uniform in [150, 1200], 1 %
+    density, a mix of weighted and binary requests, split-BF16.  Latency is
+    submit -> host result; each subject is then run alone through the same
+    server for its solo time."""
+    rng = np.random.default_rng(seed)
+    subs = []
+    for q in range(n_subj):
+        N = int(np.exp(rng.uniform(np.log(5000), np.log(150000))))
+        L = int(rng.integers(150, 1201))
+        subs.append((N, L, bool(rng.random() < 0.5)))
+    gaps = rng.exponential(1.0 / rate, size=n_subj)
+    log(f"serve: generating {n_subj} subjects (sum N*L = {sum(N * L for N, L, _ in subs):.3g})")
+    xs = [bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 5000 + q) for q, (N, L, _) in enumerate(subs)]
+    lat, batch = [], []
+    with bna.Server(workers=workers, max_batch=8) as srv:
+        # warm the kernels once
+        srv.wait(srv.submit(xs[0][:, :2000], density=1.0), graph=False)
+        t0 = time.perf_counter()
+        ids, t_next = [], t0
+        for q, (x, (N, L, weighted)) in enumerate(zip(xs, subs)):
+            t_next += gaps[q]
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids.append(srv.submit(x, density=1.0, weighted=weighted))
+            if q % 16 == 15:
+                log(f"serve: submitted {q + 1}")
+        res = [srv.wait(i, graph=False) for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        failed = [r.error for r in res if r.status != 0]
+        lat = np.array([r.latency_ms for r in res])
+        batch = [r.batch_size for r in res]
+        log(f"serve: stream done in {wall:.1f} s, {st['rounds']} rounds, failed {len(failed)}")
+        solo = []
+        for q, (x, (N, L, weighted)) in enumerate(zip(xs, subs)):
+            r = srv.wait(srv.submit(x, density=1.0, weighted=weighted), graph=False)
+            solo.append(r.latency_ms)
+        solo = np.array(solo)
+    ratio = lat / solo
+    worst = np.argsort(ratio)[::-1][:6]
+    log("serve: worst latency/solo: " + "; ".join(
+        f"N={subs[q][0]} L={subs[q][1]} {lat[q]:.0f}/{solo[q]:.0f} ms" for q in worst))
+    return {
+        "subjects": n_subj,
+        "arrival_rate_per_s": rate,
+        "N_range": [min(s[0] for s in subs), max(s[0] for s in subs)],
+        "L_range": [min(s[1] for s in subs), max(s[1] for s in subs)],
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "p99_latency_over_solo": float(np.percentile(ratio, 99)),
+        "max_latency_over_solo": float(ratio.max()),
+        "solo_p50_ms": float(np.percentile(solo, 50)),
+        "subjects_per_s": n_subj / wall,
+        "voxel_pairs_per_s": sum(N * (N - 1) // 2 for N, _, _ in subs) / wall,
+        "rounds": st["rounds"],
+        "max_batch": max(batch),
+        "max_concurrent_rounds": st["max_concurrent_rounds"],
+        "failed": len(failed),
+        "worst": [[subs[q][0], subs[q][1], round(float(lat[q]), 1), round(float(solo[q]), 1)] for q in worst],
+        "config": "density 1 %, split-BF16, Poisson arrivals, N log-uniform [5000, 150000], "
+                  "L uniform [150, 1200], half weighted / half binary, "
+                  f"{workers} worker contexts",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +176,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--serve-subjects", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=2.0)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--no-serve", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -165,6 +237,11 @@ def main() -> int:
         line["fp32_roofline_frac"] = flops / (f["corr_ms"] / 1e3) / 1e12 / PEAK["fp32"]
         line["same_csr_as_fp32"] = f["stats"][0] == b["stats"][0]
         line["fp32_stages_ms"] = f["stages_ms"]
+    if not a.no_serve:
+        log("bench: serving stream")
+        sv = serve_stream(a.serve_subjects, a.serve_rate, 0xB17A0000 + 5000, a.serve_workers)
+        line["serving"] = sv
+        line["p99_subject_latency_ms"] = sv["p99_ms"]
     if not a.no_cpu:
         log("bench: CPU baseline")
         line["cpu_baseline"] = cpu_baseline(x, N, L, thr, a.cpu_seconds)

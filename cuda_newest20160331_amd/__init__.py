@@ -442,10 +442,11 @@ class ServerStats(C.Structure):
 class Served:
     """A finished request: the graph, optional degree / EC, and its timings."""
 
-    def __init__(self, r: Result, N: int):
+    def __init__(self, r: Result, N: int, graph: bool = True):
         self.status = r.status
         self.error = r.error.decode(errors="replace")
-        self.graph = Graph(r.csr) if r.status == 0 else None
+        self.nnz = r.csr.nnz if r.status == 0 else 0
+        self.graph = Graph(r.csr) if (r.status == 0 and graph) else None
         self.degree = np.ctypeslib.as_array(r.degree, shape=(N,)).copy() if r.degree else None
         self.ec = np.ctypeslib.as_array(r.ec, shape=(N,)).copy() if r.ec else None
         self.lam = r.lam
@@ -522,22 +523,22 @@ class Server:
         self._N[rid.value] = N
         return rid.value
 
-    def _collect(self, fn, rid: int):
+    def _collect(self, fn, rid: int, graph: bool):
         r = Result()
         rc = fn(self._h, rid, C.byref(r))
         if rc == self.BUSY or rc == self.NOT_FOUND:
             return rc
         try:
-            return Served(r, self._N.pop(rid))
+            return Served(r, self._N.pop(rid), graph)
         finally:
             self._L.bna_result_free(C.byref(r))
 
-    def poll(self, rid: int):
+    def poll(self, rid: int, graph: bool = True):
         """A Served, or Server.BUSY / Server.NOT_FOUND."""
-        return self._collect(self._L.bna_poll, rid)
+        return self._collect(self._L.bna_poll, rid, graph)
 
-    def wait(self, rid: int):
-        return self._collect(self._L.bna_wait, rid)
+    def wait(self, rid: int, graph: bool = True):
+        return self._collect(self._L.bna_wait, rid, graph)
 
     def cancel(self, rid: int) -> int:
         return self._L.bna_cancel(self._h, rid)

@@ -76,11 +76,13 @@ static int dev_alloc(bna_ctx *ctx, void **p, size_t bytes)
     if (ctx->budget && ctx->held + bytes > ctx->budget)
         return fail(ctx, BNA_ERR_DEVICE_NOMEM, "HBM budget exceeded: need " + std::to_string(bytes) +
                                                    " more bytes, holding " + std::to_string(ctx->held));
-    hipError_t e = hipMalloc(p, bytes);
+    // stream-ordered: no device-wide synchronisation, so contexts on other
+    // streams keep running while this one allocates and frees
+    hipError_t e = hipMallocAsync(p, bytes, ctx->stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
-        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
     }
     ctx->held += bytes;
     return BNA_OK;
@@ -90,11 +92,11 @@ static int dev_alloc(bna_ctx *ctx, void **p, size_t bytes)
 static int out_alloc(bna_ctx *ctx, void **p, size_t bytes)
 {
     *p = nullptr;
-    hipError_t e = hipMalloc(p, bytes ? bytes : 256);
+    hipError_t e = hipMallocAsync(p, bytes ? bytes : 256, ctx->stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
-        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMalloc (output): ") + hipGetErrorString(e));
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMallocAsync (output): ") + hipGetErrorString(e));
     }
     return BNA_OK;
 }
@@ -102,7 +104,7 @@ static int out_alloc(bna_ctx *ctx, void **p, size_t bytes)
 static void dev_free(bna_ctx *ctx, void *p, size_t bytes)
 {
     if (!p) return;
-    (void)hipFree(p);
+    (void)hipFreeAsync(p, ctx->stream);
     if (bytes == 0) bytes = 256;
     ctx->held -= std::min(ctx->held, bytes);
 }
@@ -198,6 +200,13 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
             return fail(nullptr, BNA_ERR_HIP, m);
         }
     }
+    // allocations come from the device's default pool and stay there between
+    // builds instead of going back to the driver at every synchronisation
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         ctx->grid = (ncu + 7) / 8 * 8;
@@ -216,7 +225,8 @@ int bna_destroy(bna_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->ws)
-        if (kv.second.p) (void)hipFree(kv.second.p);
+        if (kv.second.p) (void)hipFreeAsync(kv.second.p, ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -822,14 +832,14 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
 }
 
 // Hands a finished job's arrays to out; a failed job's arrays are freed.
-static void job_result(Job &j, bna_csr *out)
+static void job_result(bna_ctx *ctx, Job &j, bna_csr *out)
 {
     bna_series *s = j.s;
     memset(out, 0, sizeof(*out));
     if (j.rc != BNA_OK) {
-        if (j.R32) (void)hipFree(j.R32);
-        if (j.Cd) (void)hipFree(j.Cd);
-        if (j.Vd) (void)hipFree(j.Vd);
+        if (j.R32) (void)hipFreeAsync(j.R32, ctx->stream);
+        if (j.Cd) (void)hipFreeAsync(j.Cd, ctx->stream);
+        if (j.Vd) (void)hipFreeAsync(j.Vd, ctx->stream);
         j.R32 = j.Cd = nullptr;
         j.Vd = nullptr;
         return;
@@ -893,7 +903,7 @@ int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *o, bna_csr
     HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     rc = build_batch(ctx, jobs, normed);
     if (rc == BNA_OK) rc = jobs[0].rc;
-    job_result(jobs[0], out);
+    job_result(ctx, jobs[0], out);
     if (rc) return rc;
     if (normed) {
         float ms = 0.f;
@@ -928,7 +938,7 @@ int bna_build_csr_batch(bna_ctx *ctx, bna_series *const *series, const bna_opts 
         if (rc && jobs[q].rc == BNA_OK) jobs[q].rc = rc;
         if (status) status[q] = jobs[q].rc;
         if (first == BNA_OK) first = jobs[q].rc;
-        job_result(jobs[q], &out[q]);
+        job_result(ctx, jobs[q], &out[q]);
     }
     if (normed && rc == BNA_OK) {
         float ms = 0.f;
@@ -953,9 +963,12 @@ void bna_csr_free(bna_csr *g)
 {
     if (!g) return;
     if (g->on_device) {
-        if (g->R) (void)hipFree(g->R);
-        if (g->C) (void)hipFree(g->C);
-        if (g->V) (void)hipFree(g->V);
+        // stream-ordered frees on the null stream: the library's streams are
+        // non-blocking, so this waits for nothing they run; the caller must not
+        // have work pending on these arrays
+        if (g->R) (void)hipFreeAsync(g->R, nullptr);
+        if (g->C) (void)hipFreeAsync(g->C, nullptr);
+        if (g->V) (void)hipFreeAsync(g->V, nullptr);
     } else {
         free(g->R);
         free(g->C);
