@@ -113,7 +113,7 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
     log(f"serve: generating {n_subj} subjects (sum N*L = {sum(N * L for N, L, _ in subs):.3g})")
     xs = [bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 5000 + q) for q, (N, L, _) in enumerate(subs)]
     lat, batch = [], []
-    with bna.Server(workers=workers, max_batch=8) as srv:
+    with bna.Server(workers=workers, max_batch=8, reserve_per_worker=int(os.environ.get("BNA_BENCH_RESERVE", "0"))) as srv:
         # warm the kernels once
         srv.wait(srv.submit(xs[0][:, :2000], density=1.0), graph=False)
         t0 = time.perf_counter()
@@ -131,6 +131,11 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
         st = srv.stats()
         failed = [r.error for r in res if r.status != 0]
         lat = np.array([r.latency_ms for r in res])
+        qms = np.array([r.queue_ms for r in res])
+        rms = np.array([r.run_ms for r in res])
+        t_arr = np.cumsum(gaps) * 1e3
+        phases = [r.phases_ms for r in res]
+        devt = [r.build for r in res]
         batch = [r.batch_size for r in res]
         log(f"serve: stream done in {wall:.1f} s, {st['rounds']} rounds, failed {len(failed)}")
         solo = []
@@ -141,7 +146,15 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
     ratio = lat / solo
     worst = np.argsort(ratio)[::-1][:6]
     log("serve: worst latency/solo: " + "; ".join(
-        f"N={subs[q][0]} L={subs[q][1]} {lat[q]:.0f}/{solo[q]:.0f} ms" for q in worst))
+        f"#{q} N={subs[q][0]} L={subs[q][1]} at {t_arr[q]:.0f} ms: {lat[q]:.0f} (queue {qms[q]:.0f} run {rms[q]:.0f})"
+        f"/{solo[q]:.0f} ms" for q in worst))
+    for q in worst[:3]:
+        log(f"serve:   #{q} phases {dict((k, round(v, 1)) for k, v in phases[q].items())} device "
+            f"{dict((k, round(v, 2)) for k, v in devt[q].items() if k.endswith('_ms'))}")
+    big = np.argsort([-s_[0] * s_[0] * s_[1] for s_ in subs])[:6]
+    log("serve: largest: " + "; ".join(
+        f"#{q} N={subs[q][0]} L={subs[q][1]} at {t_arr[q]:.0f} ms: {lat[q]:.0f} (queue {qms[q]:.0f} run {rms[q]:.0f})"
+        f"/{solo[q]:.0f} ms" for q in big))
     return {
         "subjects": n_subj,
         "arrival_rate_per_s": rate,

@@ -32,7 +32,8 @@ EXPORTS = [
     "bna_read_csr", "bna_write_nm", "bna_nifti_mask", "bna_nifti_bold", "bna_free",
     "bna_build_csr_batch", "bna_trim", "bna_server_config_default", "bna_server_create",
     "bna_server_destroy", "bna_submit", "bna_poll", "bna_wait", "bna_cancel", "bna_server_get_stats",
-    "bna_result_free", "bna_request_footprint", "bna_memory_info",
+    "bna_result_free", "bna_request_footprint", "bna_memory_info", "bna_reserve",
+    "bna_csr_release",
 ]
 
 
@@ -109,6 +110,7 @@ def lib() -> C.CDLL:
         "bna_build_csr_batch": (i32, [vp, C.POINTER(vp), C.POINTER(Opts), i32, C.POINTER(CSR),
                                       C.POINTER(C.c_int)]),
         "bna_trim": (i32, [vp]),
+        "bna_csr_release": (i32, [vp, C.POINTER(CSR)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -374,7 +376,7 @@ class Series:
             self.ctx._L.bna_csr_free(C.byref(h))
 
     def free_device(self, g: CSR):
-        self.ctx._L.bna_csr_free(C.byref(g))
+        self.ctx._L.bna_csr_release(self.ctx._h, C.byref(g))
 
     def build(self, **kw) -> Graph:
         g = self.build_device(**kw)
@@ -411,14 +413,14 @@ def build_batch(ctx: Context, series: list, opts: list) -> list:
             res.append(Graph(h))
         finally:
             ctx._L.bna_csr_free(C.byref(h))
-            ctx._L.bna_csr_free(C.byref(g))
+            ctx._L.bna_csr_release(ctx._h, C.byref(g))
     return res
 
 
 # ------------------------------------------------------------------ serving
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("workers", C.c_int),
-                ("max_batch", C.c_int)]
+                ("max_batch", C.c_int), ("reserve_per_worker", C.c_size_t)]
 
 
 class Request(C.Structure):
@@ -430,7 +432,9 @@ class Result(C.Structure):
     _fields_ = [("status", C.c_int), ("error", C.c_char * 256), ("csr", CSR),
                 ("degree", C.POINTER(C.c_float)), ("ec", C.POINTER(C.c_double)),
                 ("lam", C.c_double), ("ec_iters", C.c_int), ("batch_size", C.c_int),
-                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double)]
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
+                ("upload_ms", C.c_double), ("build_ms", C.c_double), ("maps_ms", C.c_double),
+                ("download_ms", C.c_double), ("build", Timings)]
 
 
 class ServerStats(C.Structure):
@@ -453,6 +457,9 @@ class Served:
         self.ec_iters = r.ec_iters
         self.batch_size = r.batch_size
         self.queue_ms, self.run_ms, self.latency_ms = r.queue_ms, r.run_ms, r.latency_ms
+        self.phases_ms = {"upload": r.upload_ms, "build": r.build_ms, "maps": r.maps_ms,
+                          "download": r.download_ms}
+        self.build = {k: getattr(r.build, k) for k, _ in Timings._fields_}
 
 
 _SERVER_SIGS = None
@@ -489,11 +496,13 @@ class Server:
 
     BUSY, CANCELLED, NOT_FOUND = -9, -10, -8
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2, max_batch: int = 16):
+    def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2, max_batch: int = 16,
+                 reserve_per_worker: int = 0):
         self._L = _server_lib()
         cfg = ServerConfig()
         self._L.bna_server_config_default(C.byref(cfg))
         cfg.device, cfg.hbm_budget, cfg.workers, cfg.max_batch = device, hbm_budget, workers, max_batch
+        cfg.reserve_per_worker = reserve_per_worker
         self._h = C.c_void_p()
         rc = self._L.bna_server_create(C.byref(cfg), C.byref(self._h))
         if rc:

@@ -38,6 +38,13 @@ struct bna_ctx {
     // tile table cache
     int tt_nb = -1;
     int64_t tt_count = 0;
+    // the context's own memory pool: allocations never wait for another
+    // context's stream to reach a free
+    hipMemPool_t pool = nullptr;
+    bool own_pool = false;
+    // pinned double buffer for host <-> device copies of pageable memory
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stev[2] = {};
 };
 
 struct bna_series {
@@ -78,11 +85,11 @@ static int dev_alloc(bna_ctx *ctx, void **p, size_t bytes)
                                                    " more bytes, holding " + std::to_string(ctx->held));
     // stream-ordered: no device-wide synchronisation, so contexts on other
     // streams keep running while this one allocates and frees
-    hipError_t e = hipMallocAsync(p, bytes, ctx->stream);
+    hipError_t e = hipMallocFromPoolAsync(p, bytes, ctx->pool, ctx->stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
-        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMallocFromPoolAsync: ") + hipGetErrorString(e));
     }
     ctx->held += bytes;
     return BNA_OK;
@@ -92,11 +99,11 @@ static int dev_alloc(bna_ctx *ctx, void **p, size_t bytes)
 static int out_alloc(bna_ctx *ctx, void **p, size_t bytes)
 {
     *p = nullptr;
-    hipError_t e = hipMallocAsync(p, bytes ? bytes : 256, ctx->stream);
+    hipError_t e = hipMallocFromPoolAsync(p, bytes ? bytes : 256, ctx->pool, ctx->stream);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *p = nullptr;
-        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMallocAsync (output): ") + hipGetErrorString(e));
+        return fail(ctx, BNA_ERR_DEVICE_NOMEM, std::string("hipMallocFromPoolAsync (output): ") + hipGetErrorString(e));
     }
     return BNA_OK;
 }
@@ -140,6 +147,70 @@ static int ws_get_t(bna_ctx *ctx, const char *name, size_t count, T **out)
 }
 
 static inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Copies between pageable host memory and the device go through the context's
+// own pinned double buffer in 32 MB pieces (a host memcpy of one piece overlaps
+// the DMA of the other), so they neither take the runtime's shared staging
+// path nor wait for copies that other contexts have in flight.
+static constexpr size_t STAGE_BYTES = (size_t)32 << 20;
+
+static int stage_ready(bna_ctx *ctx)
+{
+    if (ctx->stage[0]) return BNA_OK;
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(ctx, hipHostMalloc(&ctx->stage[b], STAGE_BYTES, hipHostMallocDefault));
+        HIPCHK(ctx, hipEventCreateWithFlags(&ctx->stev[b], hipEventDisableTiming));
+    }
+    return BNA_OK;
+}
+
+static int copy_h2d(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes < ((size_t)1 << 20)) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return BNA_OK;
+    }
+    int rc = stage_ready(ctx);
+    if (rc) return rc;
+    size_t k = 0;
+    for (size_t off = 0; off < bytes; off += STAGE_BYTES, k++) {
+        const int b = (int)(k & 1);
+        const size_t n = std::min(STAGE_BYTES, bytes - off);
+        if (k >= 2) HIPCHK(ctx, hipEventSynchronize(ctx->stev[b]));  // its previous DMA is done
+        memcpy(ctx->stage[b], (const char *)src + off, n);
+        HIPCHK(ctx, hipMemcpyAsync((char *)dst + off, ctx->stage[b], n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->stev[b], ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BNA_OK;
+}
+
+static int copy_d2h(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes < ((size_t)1 << 20)) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return BNA_OK;
+    }
+    int rc = stage_ready(ctx);
+    if (rc) return rc;
+    const size_t nch = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    auto issue = [&](size_t k) -> hipError_t {
+        const size_t off = k * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
+        hipError_t e = hipMemcpyAsync(ctx->stage[k & 1], (const char *)src + off, n, hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        return e == hipSuccess ? hipEventRecord(ctx->stev[k & 1], ctx->stream) : e;
+    };
+    HIPCHK(ctx, issue(0));
+    for (size_t k = 0; k < nch; k++) {
+        if (k + 1 < nch) HIPCHK(ctx, issue(k + 1));  // its buffer was emptied in the previous step
+        HIPCHK(ctx, hipEventSynchronize(ctx->stev[k & 1]));
+        const size_t off = k * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
+        memcpy((char *)dst + off, ctx->stage[k & 1], n);
+    }
+    return BNA_OK;
+}
 
 // ------------------------------------------------------------------ context
 extern "C" {
@@ -200,12 +271,30 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
             return fail(nullptr, BNA_ERR_HIP, m);
         }
     }
-    // allocations come from the device's default pool and stay there between
-    // builds instead of going back to the driver at every synchronisation
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+    // a pool of the context's own: memory stays there between builds instead of
+    // going back to the driver at every synchronisation, and a new allocation
+    // never picks up memory whose free is still queued behind another stream
+    hipMemPoolProps props;
+    memset(&props, 0, sizeof(props));
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    if (hipMemPoolCreate(&ctx->pool, &props) == hipSuccess) {
+        ctx->own_pool = true;
+    } else {
+        (void)hipGetLastError();
+        if ((e = hipDeviceGetDefaultMemPool(&ctx->pool, device)) != hipSuccess) {
+            std::string m = std::string("bna_create: no memory pool: ") + hipGetErrorString(e);
+            bna_destroy(ctx);
+            return fail(nullptr, BNA_ERR_HIP, m);
+        }
+    }
+    {
         uint64_t keep = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        int no = 0;
+        (void)hipMemPoolSetAttribute(ctx->pool, hipMemPoolAttrReleaseThreshold, &keep);
+        (void)hipMemPoolSetAttribute(ctx->pool, hipMemPoolReuseAllowInternalDependencies, &no);
     }
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
@@ -224,11 +313,19 @@ int bna_destroy(bna_ctx *ctx)
     if (!ctx) return BNA_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &kv : ctx->ws)
-        if (kv.second.p) (void)hipFreeAsync(kv.second.p, ctx->stream);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) {
+        for (auto &kv : ctx->ws)
+            if (kv.second.p) (void)hipFreeAsync(kv.second.p, ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    // outputs still held by callers keep a destroyed pool alive until they are freed
+    if (ctx->own_pool && ctx->pool) (void)hipMemPoolDestroy(ctx->pool);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (int b = 0; b < 2; b++) {
+        if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
+        if (ctx->stev[b]) (void)hipEventDestroy(ctx->stev[b]);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BNA_OK;
@@ -249,6 +346,18 @@ int bna_get_timings(bna_ctx *ctx, bna_timings *t)
 }
 
 int64_t bna_workspace_bytes(bna_ctx *ctx) { return ctx ? (int64_t)ctx->held : -1; }
+
+int bna_reserve(bna_ctx *ctx, size_t bytes)
+{
+    if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_reserve: ctx is NULL");
+    if (bytes == 0) return BNA_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    void *p = nullptr;
+    HIPCHK(ctx, hipMallocFromPoolAsync(&p, bytes, ctx->pool, ctx->stream));
+    HIPCHK(ctx, hipFreeAsync(p, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return BNA_OK;
+}
 
 int bna_memory_info(bna_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes)
 {
@@ -288,11 +397,9 @@ int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, in
         }
         s->own_x = true;
         s->bytes += xb;
-        hipError_t e = hipMemcpyAsync(s->x, x, xb, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) {
+        if ((rc = copy_h2d(ctx, s->x, x, xb))) {
             bna_series_destroy(ctx, s);
-            return fail(ctx, BNA_ERR_HIP, std::string("series upload: ") + hipGetErrorString(e));
+            return rc;
         }
     }
     *out = s;
@@ -954,8 +1061,27 @@ int bna_trim(bna_ctx *ctx)
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     for (auto &kv : ctx->ws) dev_free(ctx, kv.second.p, kv.second.bytes);
     ctx->ws.clear();
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemPoolTrimTo(ctx->pool, 0));
     ctx->tt_nb = -1;
     ctx->tt_count = 0;
+    return BNA_OK;
+}
+
+int bna_csr_release(bna_ctx *ctx, bna_csr *g)
+{
+    if (!ctx || !g) return fail(ctx, BNA_ERR_ARG, "bna_csr_release: NULL argument");
+    if (!g->on_device) {
+        bna_csr_free(g);
+        return BNA_OK;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (g->R) HIPCHK(ctx, hipFreeAsync(g->R, ctx->stream));
+    if (g->C) HIPCHK(ctx, hipFreeAsync(g->C, ctx->stream));
+    if (g->V) HIPCHK(ctx, hipFreeAsync(g->V, ctx->stream));
+    g->R = g->C = nullptr;
+    g->V = nullptr;
+    g->nnz = 0;
     return BNA_OK;
 }
 
@@ -1004,14 +1130,17 @@ int bna_csr_to_host(bna_ctx *ctx, const bna_csr *dev, bna_csr *out)
         return BNA_OK;
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpyAsync(out->R, dev->R, sizeof(int32_t) * ((size_t)dev->N + 1), hipMemcpyDeviceToHost, ctx->stream));
-    if (dev->nnz > 0) {
-        HIPCHK(ctx, hipMemcpyAsync(out->C, dev->C, sizeof(int32_t) * (size_t)dev->nnz, hipMemcpyDeviceToHost, ctx->stream));
-        if (dev->V)
-            HIPCHK(ctx, hipMemcpyAsync(out->V, dev->V, sizeof(float) * (size_t)dev->nnz, hipMemcpyDeviceToHost, ctx->stream));
+    int rc = copy_d2h(ctx, out->R, dev->R, sizeof(int32_t) * ((size_t)dev->N + 1));
+    if (!rc && dev->nnz > 0) rc = copy_d2h(ctx, out->C, dev->C, sizeof(int32_t) * (size_t)dev->nnz);
+    if (!rc && dev->nnz > 0 && dev->V) rc = copy_d2h(ctx, out->V, dev->V, sizeof(float) * (size_t)dev->nnz);
+    if (rc) {
+        free(out->R);
+        free(out->C);
+        free(out->V);
+        out->R = out->C = nullptr;
+        out->V = nullptr;
     }
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    return BNA_OK;
+    return rc;
 }
 
 int bna_build_csr(bna_ctx *ctx, const float *x, int N, int L, int layout, const bna_opts *o, bna_csr *out)
@@ -1025,7 +1154,7 @@ int bna_build_csr(bna_ctx *ctx, const float *x, int N, int L, int layout, const 
     rc = bna_build_csr_series(ctx, s, o, &dev);
     if (rc == BNA_OK) {
         rc = bna_csr_to_host(ctx, &dev, out);
-        bna_csr_free(&dev);
+        bna_csr_release(ctx, &dev);
     }
     bna_series_destroy(ctx, s);
     return rc;
@@ -1054,10 +1183,10 @@ static int device_view(bna_ctx *ctx, const bna_csr *g, const int32_t **R, const 
     if ((rc = ws_get_t(ctx, "g_R", (size_t)g->N + 1, &r))) return rc;
     if ((rc = ws_get_t(ctx, "g_C", nz, &c))) return rc;
     if (g->V && (rc = ws_get_t(ctx, "g_V", nz, &v))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(r, g->R, sizeof(int32_t) * ((size_t)g->N + 1), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = copy_h2d(ctx, r, g->R, sizeof(int32_t) * ((size_t)g->N + 1)))) return rc;
     if (g->nnz > 0) {
-        HIPCHK(ctx, hipMemcpyAsync(c, g->C, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice, ctx->stream));
-        if (g->V) HIPCHK(ctx, hipMemcpyAsync(v, g->V, sizeof(float) * (size_t)g->nnz, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, c, g->C, sizeof(int32_t) * (size_t)g->nnz))) return rc;
+        if (g->V && (rc = copy_h2d(ctx, v, g->V, sizeof(float) * (size_t)g->nnz))) return rc;
     }
     *R = r;
     *C = c;

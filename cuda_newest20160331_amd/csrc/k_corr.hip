@@ -32,6 +32,8 @@
 // r <= sure_thr are also queued as candidates for the FP64 recheck.
 #include "common.hpp"
 
+#include <atomic>
+
 namespace bna {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -445,18 +447,17 @@ static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, in
     constexpr size_t hst = (size_t)HIST_BINS * 4;
     constexpr size_t ring = (size_t)NSLOT * C::SLOT;
     constexpr size_t shm = ring > epi ? (ring > hst ? ring : hst) : (epi > hst ? epi : hst);
-    static int attr_dev = -1;
+    static std::atomic<int> attr_dev{-1};
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
-    if (attr_dev != dev) {
+    if (attr_dev.load() != dev) {
         e = hipFuncSetAttribute((const void *)corr_tile_kernel<PREC, HIST>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        attr_dev = dev;
+        attr_dev.store(dev);
     }
-    const int64_t per_xcd = (n_items + 7) / 8;
-    const int64_t grid = per_xcd * 8;
+    const int64_t grid = (n_items + 7) / 8 * 8;
     if (grid <= 0) return hipSuccess;
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((corr_tile_kernel<PREC, HIST>), dim3((unsigned)grid), dim3(CT_THREADS), shm, st, subj_dev,

@@ -129,6 +129,8 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         op[q] = rs[q]->opts;
         status[q] = bna_series_create(ctx, rs[q]->x.data(), rs[q]->N, rs[q]->L, rs[q]->layout, 0, &ser[q]);
     }
+    const Clock::time_point t_up = Clock::now();
+    bna_timings tm{};
     // the subjects that uploaded go through one batched build
     std::vector<int> ok;
     for (int q = 0; q < n; q++)
@@ -143,17 +145,23 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         std::vector<bna_csr> d2(ok.size());
         std::vector<int> st2(ok.size(), BNA_OK);
         const int rc = bna_build_csr_batch(ctx, s2.data(), o2.data(), (int)ok.size(), d2.data(), st2.data());
+        bna_get_timings(ctx, &tm);
         for (size_t t = 0; t < ok.size(); t++) {
             dev[ok[t]] = d2[t];
             status[ok[t]] = st2[t] != BNA_OK ? st2[t] : (rc != BNA_OK && !d2[t].R ? rc : BNA_OK);
         }
     }
+    const Clock::time_point t_built = Clock::now();
     for (int q = 0; q < n; q++) {
         Req &r = *rs[q];
         bna_result &res = r.res;
         memset(&res, 0, sizeof(res));
         res.status = status[q];
         res.batch_size = n;
+        res.build = tm;
+        res.upload_ms = ms_between(t0, t_up);
+        res.build_ms = ms_between(t_up, t_built);
+        const Clock::time_point tq = Clock::now();
         if (res.status == BNA_OK && r.want_degree) {
             res.degree = (float *)malloc(sizeof(float) * (size_t)r.N);
             res.status = res.degree ? bna_degree(ctx, &dev[q], res.degree) : BNA_ERR_NOMEM;
@@ -165,12 +173,15 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
                                               &res.lambda, &res.ec_iters)
                                 : BNA_ERR_NOMEM;
         }
+        const Clock::time_point tm_done = Clock::now();
+        res.maps_ms = ms_between(tq, tm_done);
         if (res.status == BNA_OK) res.status = bna_csr_to_host(ctx, &dev[q], &res.csr);
+        res.download_ms = ms_between(tm_done, Clock::now());
         if (res.status != BNA_OK) {
             strncpy(res.error, bna_last_error(ctx), sizeof(res.error) - 1);
             bna_csr_free(&res.csr);
         }
-        bna_csr_free(&dev[q]);
+        bna_csr_release(ctx, &dev[q]);
         if (ser[q]) bna_series_destroy(ctx, ser[q]);
         ser[q] = nullptr;
         const Clock::time_point t1 = Clock::now();
@@ -252,6 +263,8 @@ int bna_server_create(const bna_server_config *cfg, bna_server **out)
         int64_t fr = 0, tot = 0;
         if (bna_memory_info(srv->ctxs[0], &fr, &tot) == BNA_OK) srv->cfg.hbm_budget = (size_t)(0.8 * (double)fr);
     }
+    for (bna_ctx *c : srv->ctxs)
+        if (cfg->reserve_per_worker) (void)bna_reserve(c, cfg->reserve_per_worker);
     for (int w = 0; w < cfg->workers; w++) srv->workers.emplace_back(worker, srv, w);
     *out = srv;
     return BNA_OK;

@@ -119,6 +119,9 @@ typedef struct bna_ctx bna_ctx;
 int bna_create(bna_ctx **ctx, int device, size_t hbm_budget);
 int bna_destroy(bna_ctx *ctx);
 int bna_synchronize(bna_ctx *ctx);
+/* Frees a device CSR on the context's stream, ordered after the context's work
+ * (bna_csr_free orders device frees on the null stream instead). */
+int bna_csr_release(bna_ctx *ctx, bna_csr *g);
 const char *bna_status_string(int status);
 /* Last error message recorded on this context (thread-local when ctx is NULL). */
 const char *bna_last_error(bna_ctx *ctx);
@@ -223,6 +226,7 @@ typedef struct bna_server_config {
     size_t hbm_budget; /* bytes for admitted requests; 0 = 80 % of free device memory */
     int workers;       /* rounds in flight at once (contexts), 1..8 */
     int max_batch;     /* requests per round */
+    size_t reserve_per_worker; /* bytes each worker's pool takes at start (0: none) */
 } bna_server_config;
 
 typedef struct bna_request {
@@ -244,6 +248,8 @@ typedef struct bna_result {
     double queue_ms;  /* submit -> round start */
     double run_ms;    /* round start -> result ready */
     double latency_ms;/* submit -> result ready */
+    double upload_ms, build_ms, maps_ms, download_ms; /* phases of its round (build: the whole batch) */
+    bna_timings build; /* the batch's device timings */
 } bna_result;
 
 typedef struct bna_server_stats {
@@ -268,6 +274,9 @@ void bna_result_free(bna_result *r);
 /* Estimated device bytes of one request: series, operand panel, tables, edge
  * lists and CSR (edges: the density target, or 2 % of pairs for an r threshold). */
 int64_t bna_request_footprint(int N, int L, const bna_opts *o);
+/* Grows the context's memory pool by `bytes` now (allocated and returned to the
+ * pool), so later builds draw on it without asking the driver for memory. */
+int bna_reserve(bna_ctx *ctx, size_t bytes);
 /* Free and total device memory of the context's device. */
 int bna_memory_info(bna_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);
 
