@@ -113,9 +113,9 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
     log(f"serve: generating {n_subj} subjects (sum N*L = {sum(N * L for N, L, _ in subs):.3g})")
     xs = [bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 5000 + q) for q, (N, L, _) in enumerate(subs)]
     lat, batch = [], []
-    with bna.Server(workers=workers, max_batch=8, reserve_per_worker=int(os.environ.get("BNA_BENCH_RESERVE", "0"))) as srv:
+    with bna.Server(workers=workers, max_batch=8, reserve_per_worker=8 << 30) as srv:
         # warm the kernels once
-        srv.wait(srv.submit(xs[0][:, :2000], density=1.0), graph=False)
+        srv.wait(srv.submit(xs[0][:, :2000], density=1.0, device_result=True), graph=False)
         t0 = time.perf_counter()
         ids, t_next = [], t0
         for q, (x, (N, L, weighted)) in enumerate(zip(xs, subs)):
@@ -123,7 +123,7 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
             d = t_next - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            ids.append(srv.submit(x, density=1.0, weighted=weighted))
+            ids.append(srv.submit(x, density=1.0, weighted=weighted, device_result=True))
             if q % 16 == 15:
                 log(f"serve: submitted {q + 1}")
         res = [srv.wait(i, graph=False) for i in ids]
@@ -140,7 +140,7 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
         log(f"serve: stream done in {wall:.1f} s, {st['rounds']} rounds, failed {len(failed)}")
         solo = []
         for q, (x, (N, L, weighted)) in enumerate(zip(xs, subs)):
-            r = srv.wait(srv.submit(x, density=1.0, weighted=weighted), graph=False)
+            r = srv.wait(srv.submit(x, density=1.0, weighted=weighted, device_result=True), graph=False)
             solo.append(r.latency_ms)
         solo = np.array(solo)
     ratio = lat / solo
@@ -173,6 +173,7 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
         "failed": len(failed),
         "worst": [[subs[q][0], subs[q][1], round(float(lat[q]), 1), round(float(solo[q]), 1)] for q in worst],
         "config": "density 1 %, split-BF16, Poisson arrivals, N log-uniform [5000, 150000], "
+                  "CSR left in device memory (latency: submit -> device CSR ready), "
                   "L uniform [150, 1200], half weighted / half binary, "
                   f"{workers} worker contexts",
     }
@@ -191,7 +192,7 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-subjects", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2.0)
-    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

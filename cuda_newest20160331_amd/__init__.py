@@ -33,7 +33,7 @@ EXPORTS = [
     "bna_build_csr_batch", "bna_trim", "bna_server_config_default", "bna_server_create",
     "bna_server_destroy", "bna_submit", "bna_poll", "bna_wait", "bna_cancel", "bna_server_get_stats",
     "bna_result_free", "bna_request_footprint", "bna_memory_info", "bna_reserve",
-    "bna_csr_release",
+    "bna_csr_release", "bna_set_chunking",
 ]
 
 
@@ -420,12 +420,14 @@ def build_batch(ctx: Context, series: list, opts: list) -> list:
 # ------------------------------------------------------------------ serving
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("workers", C.c_int),
-                ("max_batch", C.c_int), ("reserve_per_worker", C.c_size_t)]
+                ("max_batch", C.c_int), ("reserve_per_worker", C.c_size_t),
+                ("big_round_ms", C.c_double), ("chunk_ms", C.c_double)]
 
 
 class Request(C.Structure):
     _fields_ = [("x", C.c_void_p), ("N", C.c_int), ("L", C.c_int), ("layout", C.c_int),
-                ("opts", Opts), ("want_degree", C.c_int), ("want_ec", C.c_int)]
+                ("opts", Opts), ("want_degree", C.c_int), ("want_ec", C.c_int),
+                ("device_result", C.c_int)]
 
 
 class Result(C.Structure):
@@ -450,7 +452,8 @@ class Served:
         self.status = r.status
         self.error = r.error.decode(errors="replace")
         self.nnz = r.csr.nnz if r.status == 0 else 0
-        self.graph = Graph(r.csr) if (r.status == 0 and graph) else None
+        self.on_device = bool(r.csr.on_device)
+        self.graph = Graph(r.csr) if (r.status == 0 and graph and not r.csr.on_device) else None
         self.degree = np.ctypeslib.as_array(r.degree, shape=(N,)).copy() if r.degree else None
         self.ec = np.ctypeslib.as_array(r.ec, shape=(N,)).copy() if r.ec else None
         self.lam = r.lam
@@ -497,12 +500,17 @@ class Server:
     BUSY, CANCELLED, NOT_FOUND = -9, -10, -8
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2, max_batch: int = 16,
-                 reserve_per_worker: int = 0):
+                 reserve_per_worker: int = 0, big_round_ms: float | None = None,
+                 chunk_ms: float | None = None):
         self._L = _server_lib()
         cfg = ServerConfig()
         self._L.bna_server_config_default(C.byref(cfg))
         cfg.device, cfg.hbm_budget, cfg.workers, cfg.max_batch = device, hbm_budget, workers, max_batch
         cfg.reserve_per_worker = reserve_per_worker
+        if big_round_ms is not None:
+            cfg.big_round_ms = big_round_ms
+        if chunk_ms is not None:
+            cfg.chunk_ms = chunk_ms
         self._h = C.c_void_p()
         rc = self._L.bna_server_create(C.byref(cfg), C.byref(self._h))
         if rc:
@@ -521,10 +529,10 @@ class Server:
         self.close()
 
     def submit(self, x: np.ndarray, layout: int = TIME_MAJOR, degree: bool = False, ec: bool = False,
-               **kw) -> int:
+               device_result: bool = False, **kw) -> int:
         x = np.ascontiguousarray(x, dtype=np.float32)
         L, N = x.shape if layout == TIME_MAJOR else x.shape[::-1]
-        r = Request(x.ctypes.data, N, L, layout, make_opts(**kw), int(degree), int(ec))
+        r = Request(x.ctypes.data, N, L, layout, make_opts(**kw), int(degree), int(ec), int(device_result))
         rid = C.c_int64()
         rc = self._L.bna_submit(self._h, C.byref(r), C.byref(rid))
         if rc:

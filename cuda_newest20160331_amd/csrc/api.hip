@@ -42,6 +42,12 @@ struct bna_ctx {
     // context's stream to reach a free
     hipMemPool_t pool = nullptr;
     bool own_pool = false;
+    // chunked launches (serving): long kernels are issued in pieces of about
+    // chunk_ms, at most two in flight, each preceded by gate() (which may block)
+    double chunk_ms = 0.0;
+    void (*gate)(void *) = nullptr;
+    void *gate_arg = nullptr;
+    hipEvent_t chev[2] = {};
     // pinned double buffer for host <-> device copies of pageable memory
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stev[2] = {};
@@ -212,6 +218,30 @@ static int copy_d2h(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
     return BNA_OK;
 }
 
+// Issues launch(first, count) over [0, n) in pieces of `per` units, keeping at
+// most two pieces in flight and calling the context's gate before each one.
+template <typename F>
+static int run_chunks(bna_ctx *ctx, int64_t n, int64_t per, F launch)
+{
+    if (per <= 0 || per >= n || ctx->chunk_ms <= 0.0) {
+        HIPCHK(ctx, launch((int64_t)0, n));
+        return BNA_OK;
+    }
+    int k = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += per, k++) {
+        if (ctx->gate) ctx->gate(ctx->gate_arg);
+        if (k >= 2) HIPCHK(ctx, hipEventSynchronize(ctx->chev[k & 1]));
+        HIPCHK(ctx, launch(c0, std::min(per, n - c0)));
+        HIPCHK(ctx, hipEventRecord(ctx->chev[k & 1], ctx->stream));
+    }
+    return BNA_OK;
+}
+
+// Throughput figures that size the pieces: tile kernel ~3e6 tile-points per
+// ms (tiles x L_pad), value refinement ~4e9 bytes per ms.
+static constexpr double TILE_POINTS_PER_MS = 3.0e6;
+static constexpr double REFINE_BYTES_PER_MS = 4.0e9;
+
 // ------------------------------------------------------------------ context
 extern "C" {
 
@@ -263,6 +293,13 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
         std::string m = std::string("bna_create: ") + hipGetErrorString(e);
         delete ctx;
         return fail(nullptr, BNA_ERR_HIP, m);
+    }
+    for (auto &ev : ctx->chev) {
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            std::string m = std::string("bna_create: ") + hipGetErrorString(e);
+            bna_destroy(ctx);
+            return fail(nullptr, BNA_ERR_HIP, m);
+        }
     }
     for (auto &ev : ctx->ev) {
         if ((e = hipEventCreate(&ev)) != hipSuccess) {
@@ -325,6 +362,7 @@ int bna_destroy(bna_ctx *ctx)
     for (int b = 0; b < 2; b++) {
         if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
         if (ctx->stev[b]) (void)hipEventDestroy(ctx->stev[b]);
+        if (ctx->chev[b]) (void)hipEventDestroy(ctx->chev[b]);
     }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -347,11 +385,23 @@ int bna_get_timings(bna_ctx *ctx, bna_timings *t)
 
 int64_t bna_workspace_bytes(bna_ctx *ctx) { return ctx ? (int64_t)ctx->held : -1; }
 
+int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *gate_arg)
+{
+    if (!ctx || chunk_ms < 0.0) return fail(ctx, BNA_ERR_ARG, "bna_set_chunking: bad argument");
+    ctx->chunk_ms = chunk_ms;
+    ctx->gate = gate;
+    ctx->gate_arg = gate_arg;
+    return BNA_OK;
+}
+
 int bna_reserve(bna_ctx *ctx, size_t bytes)
 {
     if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_reserve: ctx is NULL");
-    if (bytes == 0) return BNA_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc = stage_ready(ctx);
+    if (rc) return rc;
+    for (int b = 0; b < 2; b++) memset(ctx->stage[b], 0, STAGE_BYTES);  // first touch now, not in a request
+    if (bytes == 0) return BNA_OK;
     void *p = nullptr;
     HIPCHK(ctx, hipMallocFromPoolAsync(&p, bytes, ctx->pool, ctx->stream));
     HIPCHK(ctx, hipFreeAsync(p, ctx->stream));
@@ -747,6 +797,17 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
     return BNA_OK;
 }
 
+static int corr_chunks(bna_ctx *ctx, int prec, int hist, SubjDesc *dsd, TileItem *tiles, int64_t nt,
+                       const std::vector<Job *> &js)
+{
+    int Lmax = 32;
+    for (Job *j : js) Lmax = std::max(Lmax, j->s->L_pad);
+    const int64_t per = std::max<int64_t>(512, (int64_t)(ctx->chunk_ms * TILE_POINTS_PER_MS / Lmax) / 8 * 8);
+    return run_chunks(ctx, nt, per, [&](int64_t c0, int64_t n) {
+        return launch_corr(prec, hist, dsd, tiles + c0, n, ctx->stream);
+    });
+}
+
 static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_norm)
 {
     const int prec = jobs[0].o.precision;
@@ -781,7 +842,7 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         int64_t nt;
         if ((rc = upload_batch(ctx, dj, &dsd, &tiles, &nt))) return rc;
         HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        HIPCHK(ctx, launch_corr(prec, 1, dsd, tiles, nt, ctx->stream));
+        if ((rc = corr_chunks(ctx, prec, 1, dsd, tiles, nt, dj))) return rc;
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         std::vector<unsigned long long> hh(HIST_BINS * dj.size());
@@ -839,7 +900,7 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         int64_t nt;
         if ((rc = upload_batch(ctx, act, &dsd, &tiles, &nt))) return rc;
         HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        HIPCHK(ctx, launch_corr(prec, 0, dsd, tiles, nt, ctx->stream));
+        if ((rc = corr_chunks(ctx, prec, 0, dsd, tiles, nt, act))) return rc;
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         for (Job *j : act) {
@@ -917,9 +978,15 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
     for (size_t q = 0; q < jobs.size(); q++) {
         Job &j = jobs[q];
         bna_series *s = j.s;
-        if (j.rc == BNA_OK && j.Vd && j.o.refine && j.nnz > 0)
-            HIPCHK(ctx, launch_refine(j.R32, j.Cd, j.Vd, s->xv ? s->xv : s->x,
-                                      s->xv ? s->L_pad : s->L, s->mu, s->sd, s->N, s->L, ctx->stream));
+        if (j.rc == BNA_OK && j.Vd && j.o.refine && j.nnz > 0) {
+            const double bytes = (double)j.nnz / 2.0 * s->L * 4.0 + (double)s->N * s->L * 4.0;
+            const int64_t per = (int64_t)std::max(64.0, s->N * (ctx->chunk_ms * REFINE_BYTES_PER_MS / bytes));
+            if ((rc = run_chunks(ctx, s->N, per, [&](int64_t r0, int64_t n) {
+                     return launch_refine(j.R32, j.Cd, j.Vd, s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->mu,
+                                          s->sd, s->N, s->L, (int)r0, (int)(r0 + n), ctx->stream);
+                 })))
+                return rc;
+        }
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[7], ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

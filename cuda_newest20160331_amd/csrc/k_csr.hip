@@ -327,13 +327,13 @@ template <bool VEC>
 __global__ __launch_bounds__(256) void refine_kernel(const int32_t *__restrict__ R, const int32_t *__restrict__ C,
                                                      float *__restrict__ V, const float *__restrict__ xs,
                                                      int64_t ld, const double *__restrict__ mu,
-                                                     const double *__restrict__ sd, int N, int L)
+                                                     const double *__restrict__ sd, int N, int L, int row0, int row1)
 {
     const int lane = threadIdx.x & 63;
     const int g = lane / RG, gl = lane % RG, gbase = g * RG;
     const uint64_t gmask = 0xffffull << gbase;
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (i >= N) return;
+    const int64_t i = row0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (i >= row1) return;
     const int64_t r0 = R[i], r1 = R[i + 1];
     if (r0 == r1) return;
     // first entry with column > i
@@ -389,15 +389,16 @@ __global__ __launch_bounds__(256) void refine_kernel(const int32_t *__restrict__
 }
 
 hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
-                         const double *mu, const double *sd, int N, int L, hipStream_t st)
+                         const double *mu, const double *sd, int N, int L, int row0, int row1, hipStream_t st)
 {
-    const int64_t threads = (int64_t)N * 64;
+    if (row0 < 0 || row1 > N || row0 >= row1) return row0 == row1 ? hipSuccess : hipErrorInvalidValue;
+    const int64_t threads = (int64_t)(row1 - row0) * 64;
     const dim3 grid((unsigned)((threads + 255) / 256));
     // 16-byte loads need 16-byte aligned rows
     if (ld % 4 == 0 && ((uintptr_t)xs & 15) == 0)
-        hipLaunchKernelGGL(refine_kernel<true>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L);
+        hipLaunchKernelGGL(refine_kernel<true>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L, row0, row1);
     else
-        hipLaunchKernelGGL(refine_kernel<false>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L);
+        hipLaunchKernelGGL(refine_kernel<false>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L, row0, row1);
     return hipGetLastError();
 }
 

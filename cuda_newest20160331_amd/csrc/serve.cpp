@@ -38,7 +38,7 @@ struct Req {
     std::vector<float> x;
     int N = 0, L = 0, layout = 0;
     bna_opts opts{};
-    int want_degree = 0, want_ec = 0;
+    int want_degree = 0, want_ec = 0, device_result = 0;
     int64_t footprint = 0;
     Clock::time_point t_submit, t_start;
     bool done = false;
@@ -60,7 +60,28 @@ struct bna_server {
     std::vector<bna_ctx *> ctxs;
     bna_server_stats st{};
     int busy_workers = 0;
+    // Small rounds run ungated; while any is on the device, big rounds (issued
+    // in pieces) hold back their next piece, so a small subject never queues
+    // behind a whole long kernel of a big one.
+    std::mutex gate_mu;
+    std::condition_variable gate_cv;
+    int small_active = 0;
 };
+
+static void gate_fn(void *arg)
+{
+    bna_server *srv = (bna_server *)arg;
+    std::unique_lock<std::mutex> lk(srv->gate_mu);
+    srv->gate_cv.wait(lk, [&] { return srv->small_active == 0; });
+}
+
+// Tile-kernel time of a request, ms: tiles x L_pad at ~3e6 tile-points per ms,
+// two passes for a density threshold.
+static double est_ms(const Req &r)
+{
+    const double nb = (double)((r.N + 255) / 256), Lp = (double)((r.L + 31) / 32 * 32);
+    return nb * (nb + 1) / 2.0 * Lp / 3.0e6 * (r.opts.thr_type == BNA_THR_DENSITY ? 2.0 : 1.0);
+}
 
 extern "C" int64_t bna_request_footprint(int N, int L, const bna_opts *o)
 {
@@ -120,6 +141,15 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
 {
     const Clock::time_point t0 = Clock::now();
     const int n = (int)rs.size();
+    double work = 0.0;
+    for (auto &r : rs) work += est_ms(*r);
+    const bool big = srv->cfg.big_round_ms > 0.0 && work > srv->cfg.big_round_ms;
+    if (big) {
+        bna_set_chunking(ctx, srv->cfg.chunk_ms, gate_fn, srv);
+    } else {
+        std::lock_guard<std::mutex> lk(srv->gate_mu);
+        srv->small_active++;
+    }
     std::vector<bna_series *> ser(n, nullptr);
     std::vector<bna_opts> op(n);
     std::vector<bna_csr> dev(n);
@@ -152,6 +182,14 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         }
     }
     const Clock::time_point t_built = Clock::now();
+    auto small_done = [&] {
+        if (big) return;
+        {
+            std::lock_guard<std::mutex> lk(srv->gate_mu);
+            srv->small_active--;
+        }
+        srv->gate_cv.notify_all();
+    };
     for (int q = 0; q < n; q++) {
         Req &r = *rs[q];
         bna_result &res = r.res;
@@ -175,7 +213,13 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         }
         const Clock::time_point tm_done = Clock::now();
         res.maps_ms = ms_between(tq, tm_done);
-        if (res.status == BNA_OK) res.status = bna_csr_to_host(ctx, &dev[q], &res.csr);
+        if (q == n - 1) small_done();  // the rest is host copies
+        if (res.status == BNA_OK && r.device_result) {
+            res.csr = dev[q];  // handed over; freed by bna_result_free
+            memset(&dev[q], 0, sizeof(dev[q]));
+        } else if (res.status == BNA_OK) {
+            res.status = bna_csr_to_host(ctx, &dev[q], &res.csr);
+        }
         res.download_ms = ms_between(tm_done, Clock::now());
         if (res.status != BNA_OK) {
             strncpy(res.error, bna_last_error(ctx), sizeof(res.error) - 1);
@@ -196,6 +240,7 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         srv->cv_done.notify_all();
         srv->cv_work.notify_all();  // memory was released: more may be admitted
     }
+    if (big) bna_set_chunking(ctx, 0.0, nullptr, nullptr);
     // keep the workspace from growing without bound across rounds
     if (bna_workspace_bytes(ctx) > (int64_t)(srv->cfg.hbm_budget / 4)) bna_trim(ctx);
 }
@@ -239,6 +284,8 @@ void bna_server_config_default(bna_server_config *c)
     c->hbm_budget = 0;
     c->workers = 2;
     c->max_batch = 16;
+    c->big_round_ms = 15.0;
+    c->chunk_ms = 3.0;
 }
 
 int bna_server_create(const bna_server_config *cfg, bna_server **out)
@@ -263,8 +310,7 @@ int bna_server_create(const bna_server_config *cfg, bna_server **out)
         int64_t fr = 0, tot = 0;
         if (bna_memory_info(srv->ctxs[0], &fr, &tot) == BNA_OK) srv->cfg.hbm_budget = (size_t)(0.8 * (double)fr);
     }
-    for (bna_ctx *c : srv->ctxs)
-        if (cfg->reserve_per_worker) (void)bna_reserve(c, cfg->reserve_per_worker);
+    for (bna_ctx *c : srv->ctxs) (void)bna_reserve(c, cfg->reserve_per_worker);  // warm: pool and staging
     for (int w = 0; w < cfg->workers; w++) srv->workers.emplace_back(worker, srv, w);
     *out = srv;
     return BNA_OK;
@@ -308,6 +354,7 @@ int bna_submit(bna_server *srv, const bna_request *req, int64_t *id)
     r->opts = req->opts;
     r->want_degree = req->want_degree;
     r->want_ec = req->want_ec;
+    r->device_result = req->device_result;
     r->footprint = bna_request_footprint(req->N, req->L, &req->opts);
     r->t_submit = Clock::now();
     {

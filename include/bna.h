@@ -227,6 +227,9 @@ typedef struct bna_server_config {
     int workers;       /* rounds in flight at once (contexts), 1..8 */
     int max_batch;     /* requests per round */
     size_t reserve_per_worker; /* bytes each worker's pool takes at start (0: none) */
+    double big_round_ms; /* rounds whose estimated tile-kernel time exceeds this run in
+                            pieces that yield to small rounds (0: never) */
+    double chunk_ms;     /* piece length for big rounds */
 } bna_server_config;
 
 typedef struct bna_request {
@@ -234,12 +237,13 @@ typedef struct bna_request {
     int N, L, layout;
     bna_opts opts;
     int want_degree, want_ec; /* EC: tol 1e-6, 10000 iterations, the reference output of the graph's kind */
+    int device_result;        /* 1: the CSR stays in device memory (result.csr.on_device = 1) */
 } bna_request;
 
 typedef struct bna_result {
     int status;
     char error[256];
-    bna_csr csr;      /* host arrays */
+    bna_csr csr;      /* host arrays, or device arrays for device_result */
     float *degree;    /* N floats or NULL */
     double *ec;       /* N doubles or NULL */
     double lambda;
@@ -274,8 +278,14 @@ void bna_result_free(bna_result *r);
 /* Estimated device bytes of one request: series, operand panel, tables, edge
  * lists and CSR (edges: the density target, or 2 % of pairs for an r threshold). */
 int64_t bna_request_footprint(int N, int L, const bna_opts *o);
+/* Long kernels of later builds (tile passes, value refinement) are issued in
+ * pieces of about chunk_ms with at most two in flight, and gate(gate_arg) is
+ * called before each piece (it may block, e.g. while a small request runs on
+ * another context).  chunk_ms = 0 restores single launches. */
+int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *gate_arg);
 /* Grows the context's memory pool by `bytes` now (allocated and returned to the
- * pool), so later builds draw on it without asking the driver for memory. */
+ * pool), so later builds draw on it without asking the driver for memory, and
+ * sets up the pinned staging buffers; bytes = 0 does only the latter. */
 int bna_reserve(bna_ctx *ctx, size_t bytes);
 /* Free and total device memory of the context's device. */
 int bna_memory_info(bna_ctx *ctx, int64_t *free_bytes, int64_t *total_bytes);

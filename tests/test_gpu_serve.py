@@ -123,3 +123,12 @@ def test_cancel_and_poll():
             assert srv.wait(ids[0]).status == 0
         assert srv.poll(ids[0]) == bna.Server.NOT_FOUND
         assert srv.stats()["cancelled"] == 1
+
+
+def test_device_resident_result():
+    x = bna.synth_factor_model(3000, 200, 8, 0.7, 91)
+    with bna.Context(0) as ctx:
+        g = ctx.build_csr(x, density=1.0)
+    with bna.Server(workers=2) as srv:
+        r = srv.wait(srv.submit(x, density=1.0, device_result=True))
+    assert r.status == 0 and r.on_device and r.graph is None and r.nnz == g.nnz
