@@ -180,6 +180,53 @@ __device__ __forceinline__ void epi_mark(const f32x16_t &a, const EpiCtx &c, uin
     }
 }
 
+// Same marking for a tile that lies inside the matrix and off the diagonal,
+// where every element is a pair to test.  The sixteen comparisons become
+// sixteen wave masks first; a block none of whose elements is kept (most of
+// them at useful thresholds) costs those compares and one OR, and inside a
+// block only the elements some lane keeps take the recording branch.
+template <int RB, int CB>
+__device__ __forceinline__ void epi_mark_inner(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits,
+                                               uint32_t *colbits, WaveList &wl, KeptPair *ovf_pool,
+                                               unsigned long long *ovf_count, int64_t ovf_cap)
+{
+    uint64_t m[16];
+    uint64_t any = 0;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        m[e] = __ballot(a[e] > c.keep);
+        any |= m[e];
+    }
+    if (any == 0) return;
+    if (wl.ovf == nullptr && !wl.lost && wl.n + 1024 > LIST_LDS) {
+        unsigned long long base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(ovf_count, (unsigned long long)(WAVE_MAX - LIST_LDS));
+        base = __shfl(base, 0, 64);
+        if ((int64_t)(base + (WAVE_MAX - LIST_LDS)) <= ovf_cap)
+            wl.ovf = (glb_u64 *)(ovf_pool + base);
+        else
+            wl.lost = true;
+    }
+    const int lc = c.wc0 + CB * 32 + c.lrow;
+    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        if (m[e] == 0) continue;
+        const float v = a[e];
+        if (v > c.keep) {
+            const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
+            atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
+            atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
+            const int k = wl.n + __popcll(m[e] & lt);
+            const uint64_t rec = (uint64_t)((uint32_t)lr | ((uint32_t)lc << 16)) |
+                                 ((uint64_t)__builtin_bit_cast(uint32_t, v) << 32);
+            if (k < LIST_LDS) wl.lds[k] = rec;
+            else if (wl.ovf) wl.ovf[k - LIST_LDS] = rec;
+        }
+        wl.n += __popcll(m[e]);
+    }
+}
+
 #define BNA_FOR_BLOCKS(F, ...)                                                                     \
     do {                                                                                           \
         F<0, 0>(acc[0][0], __VA_ARGS__); F<0, 1>(acc[0][1], __VA_ARGS__);                          \
@@ -375,7 +422,10 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         __syncthreads();
 
         WaveList wl{(lds_u64 *)(lists + wave * LIST_LDS), nullptr, 0, false};
-        BNA_FOR_BLOCKS(epi_mark, ec, rowbits, colbits, wl, sd.ovf, sd.ovf_count, sd.ovf_cap);
+        if (ec.diag || ec.gi0 + TILE > sd.N || ec.gj0 + TILE > sd.N)
+            BNA_FOR_BLOCKS(epi_mark, ec, rowbits, colbits, wl, sd.ovf, sd.ovf_count, sd.ovf_cap);
+        else
+            BNA_FOR_BLOCKS(epi_mark_inner, ec, rowbits, colbits, wl, sd.ovf, sd.ovf_count, sd.ovf_cap);
         __syncthreads();
 
         int rc = 0, cc = 0;
