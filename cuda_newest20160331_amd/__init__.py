@@ -33,7 +33,7 @@ EXPORTS = [
     "bna_build_csr_batch", "bna_trim", "bna_server_config_default", "bna_server_create",
     "bna_server_destroy", "bna_submit", "bna_poll", "bna_wait", "bna_cancel", "bna_server_get_stats",
     "bna_result_free", "bna_request_footprint", "bna_memory_info", "bna_reserve",
-    "bna_csr_release", "bna_set_chunking",
+    "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size",
 ]
 
 

@@ -35,8 +35,11 @@ struct bna_ctx {
     hipEvent_t ev[8] = {};
     std::map<std::string, DevBuf> ws;
     int grid = 256;  // persistent correlation workgroups: CUs rounded up to a multiple of 8
-    // tile table cache
-    int tt_nb = -1;
+    int grid_limit = 0;  // > 0: launches use at most this many (a multiple of 8)
+    // tile table cache (single-subject builds): key of the cached table
+    int tt_nb = -1, tt_nk = -1;
+    const void *tt_panel = nullptr;
+    int64_t tt_kstride = 0;
     int64_t tt_count = 0;
     // the context's own memory pool: allocations never wait for another
     // context's stream to reach a free
@@ -394,6 +397,15 @@ int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *
     return BNA_OK;
 }
 
+int bna_set_grid_limit(bna_ctx *ctx, int max_workgroups)
+{
+    if (!ctx || max_workgroups < 0) return fail(ctx, BNA_ERR_ARG, "bna_set_grid_limit: bad argument");
+    ctx->grid_limit = max_workgroups == 0 ? 0 : std::max(8, max_workgroups / 8 * 8);
+    return BNA_OK;
+}
+
+int bna_grid_size(bna_ctx *ctx) { return ctx ? ctx->grid : -1; }
+
 int bna_reserve(bna_ctx *ctx, size_t bytes)
 {
     if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_reserve: ctx is NULL");
@@ -434,7 +446,7 @@ int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, in
     s->L = L;
     s->layout = layout;
     s->N_pad = round_up(N, TILE);
-    s->L_pad = round_up(L, KPAD);
+    s->L_pad = std::max(LPAD_MIN, round_up(L, KPAD));
     const size_t xb = sizeof(float) * (size_t)N * L;
     if (x_on_device) {
         s->x = (float *)x;
@@ -556,31 +568,37 @@ int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *
 // Upper-triangle tiles in bands of 8 tile-rows: within a band, tile-columns
 // ascend and the band's rows vary fastest, so a run of consecutive items shares
 // a few A panels (the band) and a few B panels (neighbouring columns).
-static void make_tile_table(int nb, int subj, std::vector<TileItem> &out)
+static void make_tile_table(int nb, int subj, const void *panel, int64_t kstride, int nk,
+                            std::vector<TileItem> &out)
 {
     const int G = 8;
     for (int I0 = 0; I0 < nb; I0 += G) {
         const int I1 = std::min(nb, I0 + G);
         for (int J = I0; J < nb; J++)
-            for (int I = I0; I < I1 && I <= J; I++) out.push_back(TileItem{subj, I, J, 0});
+            for (int I = I0; I < I1 && I <= J; I++) out.push_back(TileItem{panel, kstride, subj, I, J, nk});
     }
 }
 
-static int get_tile_table(bna_ctx *ctx, int nb, TileItem **dev, int64_t *count)
+static int get_tile_table(bna_ctx *ctx, int nb, const void *panel, int64_t kstride, int nk, TileItem **dev,
+                          int64_t *count)
 {
-    if (ctx->tt_nb == nb) {
+    if (ctx->tt_nb == nb && ctx->tt_panel == panel && ctx->tt_kstride == kstride && ctx->tt_nk == nk) {
         *dev = (TileItem *)ctx->ws["tiles"].p;
         *count = ctx->tt_count;
         return BNA_OK;
     }
+    ctx->tt_nb = -1;
     std::vector<TileItem> t;
-    make_tile_table(nb, 0, t);
+    make_tile_table(nb, 0, panel, kstride, nk, t);
     TileItem *d = nullptr;
     int rc = ws_get_t(ctx, "tiles", t.size(), &d);
     if (rc) return rc;
     HIPCHK(ctx, hipMemcpyAsync(d, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->tt_nb = nb;
+    ctx->tt_panel = panel;
+    ctx->tt_kstride = kstride;
+    ctx->tt_nk = nk;
     ctx->tt_count = (int64_t)t.size();
     *dev = d;
     *count = ctx->tt_count;
@@ -703,11 +721,11 @@ struct Job {
     double band = 0.0, band_used = 0.0;
     std::string pre;  // workspace name prefix
     SubjDesc h{};
-    unsigned long long *ctr = nullptr;  // [0] entries, [1] candidates, [2] max err, [3] flips, [4] pool
+    unsigned long long *ctr = nullptr;  // [0] entries, [1] candidates, [2] max err, [3] flips
     unsigned long long *hist = nullptr;
     SelState *sel = nullptr;
     double *r64c = nullptr;
-    int64_t ecap = 0, ccap = 0, ocap = 0;
+    int64_t ecap = 0, ccap = 0;
     int64_t *rowlen = nullptr, *R64 = nullptr, *bsum = nullptr, *off_rm = nullptr;
     int32_t *orig_rm = nullptr, *live_rm = nullptr;
     int32_t *R32 = nullptr, *Cd = nullptr;  // outputs, owned by the job until handed out
@@ -748,7 +766,6 @@ static int job_prepare(bna_ctx *ctx, Job &j)
     auto held = [&](const char *n, size_t unit) { return (int64_t)(ctx->ws[j.pre + n].bytes / unit); };
     j.ecap = std::max<int64_t>(1 << 20, held("entries", sizeof(Entry)));
     j.ccap = std::max<int64_t>(1 << 16, held("cands", sizeof(Candidate)));
-    j.ocap = std::max<int64_t>(1 << 16, held("kept_ovf", sizeof(KeptPair)));
     j.band = band_for(j.o.precision, s->L_pad);
     j.band_used = j.o.recheck ? j.band : 0.0;
     SubjDesc &h = j.h;
@@ -764,7 +781,6 @@ static int job_prepare(bna_ctx *ctx, Job &j)
     h.frag_off = frag_off;
     h.entry_count = j.ctr;
     h.cand_count = j.ctr + 1;
-    h.ovf_count = j.ctr + 4;
     h.hist = nullptr;
     if (j.k < 0) {
         if (j.o.recheck) {
@@ -788,9 +804,13 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
     int rc;
     if ((rc = ws_get_t(ctx, "batch_subjdesc", hd.size(), dsd))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(*dsd, hd.data(), hd.size() * sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
-    if (js.size() == 1) return get_tile_table(ctx, js[0]->s->N_pad / TILE, tiles, ntiles);  // cached
+    auto kstride = [](const Job *j) { return (int64_t)j->s->N_pad * panel_unit(j->o.precision); };
+    if (js.size() == 1)  // cached
+        return get_tile_table(ctx, js[0]->s->N_pad / TILE, js[0]->s->panel, kstride(js[0]), js[0]->s->L_pad / 16,
+                              tiles, ntiles);
     std::vector<TileItem> t;
-    for (size_t q = 0; q < js.size(); q++) make_tile_table(js[q]->s->N_pad / TILE, (int)q, t);
+    for (size_t q = 0; q < js.size(); q++)
+        make_tile_table(js[q]->s->N_pad / TILE, (int)q, js[q]->s->panel, kstride(js[q]), js[q]->s->L_pad / 16, t);
     if ((rc = ws_get_t(ctx, "batch_tiles", t.size(), tiles))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(*tiles, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
     *ntiles = (int64_t)t.size();
@@ -803,8 +823,14 @@ static int corr_chunks(bna_ctx *ctx, int prec, int hist, SubjDesc *dsd, TileItem
     int Lmax = 32;
     for (Job *j : js) Lmax = std::max(Lmax, j->s->L_pad);
     const int64_t per = std::max<int64_t>(512, (int64_t)(ctx->chunk_ms * TILE_POINTS_PER_MS / Lmax) / 8 * 8);
+    unsigned int *claim = nullptr;
+    uint64_t *spill = nullptr;
+    int rc = ws_get_t(ctx, "tile_claim", 8, &claim);
+    if (!rc && !hist) rc = ws_get_t(ctx, "tile_spill", (size_t)corr_spill_records(prec, ctx->grid), &spill);
+    if (rc) return rc;
     return run_chunks(ctx, nt, per, [&](int64_t c0, int64_t n) {
-        return launch_corr(prec, hist, dsd, tiles + c0, n, ctx->stream);
+        const int grid = ctx->grid_limit > 0 ? std::min(ctx->grid, ctx->grid_limit) : ctx->grid;
+        return launch_corr(prec, hist, dsd, tiles + c0, n, claim, spill, grid, ctx->stream);
     });
 }
 
@@ -881,14 +907,10 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         for (Job *j : act) {
             Entry *entries;
             Candidate *cands;
-            KeptPair *ovf;
             if ((rc = job_ws(ctx, *j, "entries", (size_t)j->ecap, &entries)) ||
                 (rc = job_ws(ctx, *j, "cands", (size_t)j->ccap, &cands)) ||
-                (rc = job_ws(ctx, *j, "kept_ovf", (size_t)j->ocap, &ovf)) ||
                 (j->k > 0 && (rc = job_ws(ctx, *j, "cand_r64", (size_t)j->ccap, &j->r64c))))
                 return rc;
-            j->h.ovf = ovf;
-            j->h.ovf_cap = j->ocap;
             j->h.entries = entries;
             j->h.entry_cap = j->ecap;
             j->h.cands = cands;
@@ -939,15 +961,13 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         ms_sel += ms;
         for (Job *j : act) {
             const unsigned long long *hc = j->hc;
-            const bool e_over = (int64_t)hc[0] > j->ecap, c_over = (int64_t)hc[1] > j->ccap,
-                       o_over = (int64_t)hc[4] > j->ocap;
-            if (!e_over && !c_over && !o_over) {
+            const bool e_over = (int64_t)hc[0] > j->ecap, c_over = (int64_t)hc[1] > j->ccap;
+            if (!e_over && !c_over) {
                 j->done = true;
                 continue;
             }
             if (e_over) j->ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
             if (c_over) j->ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
-            if (o_over) j->ocap = (int64_t)hc[4];
         }
     }
 

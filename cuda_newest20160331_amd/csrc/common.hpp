@@ -6,15 +6,11 @@
 
 namespace bna {
 
-// A kept pair of a correlation tile (tile row, tile column, r).
-struct KeptPair {
-    uint16_t lr, lc;
-    float r;
-};
-
-// Correlation tile edge (voxels) and the K padding of the operand panels.
+// Correlation tile edge (voxels), the K padding of the operand panels and the
+// shortest padded length (the tile kernel streams at least four 16-point stages).
 constexpr int TILE = 256;
-constexpr int KPAD = 32;
+constexpr int KPAD = 16;
+constexpr int LPAD_MIN = 64;
 
 // One entry of the per-tile compacted edge lists: the neighbour's voxel index
 // and its r.  col < 0 marks an entry the FP64 recheck rejected.
@@ -32,11 +28,15 @@ struct Candidate {
     int32_t pad;
 };
 
-// Tile work item: subject index and tile coordinates (I <= J).
+// Tile work item: the subject's operand panel and its stride between 16-point
+// blocks (N_pad * panel unit), the subject index, tile coordinates (I <= J) and
+// the number of 16-point stages (L_pad / 16, at least 4).
 struct TileItem {
+    const void *panel;
+    int64_t kstride;
     int32_t subj;
     int32_t I, J;
-    int32_t pad;
+    int32_t nk;
 };
 
 // Per-subject description used by the tile kernels (device memory).
@@ -56,9 +56,6 @@ struct SubjDesc {
     unsigned long long *cand_count;
     int64_t cand_cap;
     unsigned long long *hist; // histogram pass: HIST_BINS counters
-    struct KeptPair *ovf;     // kept-pair overflow pool for dense tiles
-    unsigned long long *ovf_count;
-    int64_t ovf_cap;          // entries in the pool
 };
 
 constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
@@ -100,10 +97,14 @@ hipError_t launch_znorm(const float *x, int layout, int N, int L, int N_pad, int
 hipError_t launch_zmat(const float *xs, int64_t ld, int N, int L, const double *mu, const double *sd, double *z64,
                        hipStream_t st);
 
-// Correlation + threshold + compaction over a tile table (kernel b + first half of c).
-// prec: 0 FP32, 1 BF16, 2 BF16X3.  hist_mode: 1 = histogram epilogue.
+// Correlation + threshold + compaction over a tile table (kernel b + first half of c),
+// as one persistent launch of at most grid_max workgroups.  prec: 0 FP32, 1 BF16,
+// 2 BF16X3.  hist_mode: 1 = histogram epilogue.  claim: 8 device counters (reset
+// by the launch).  spill: corr_spill_records(prec, grid_max) 8-byte records.
+// grid_max: a positive multiple of 8.
 hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const TileItem *items,
-                       int64_t n_items, hipStream_t st);
+                       int64_t n_items, unsigned int *claim, uint64_t *spill, int grid_max, hipStream_t st);
+int64_t corr_spill_records(int prec, int grid_max);
 
 // FP64 recheck of candidates.
 // Slots at or past entry_cap (overflowed lists) are skipped.

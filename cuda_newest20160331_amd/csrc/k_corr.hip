@@ -9,6 +9,16 @@
 // once in tile-column order (for row j's neighbours i < j), so the CSR rows can
 // be assembled later without a sort.
 //
+// Persistent launch: one workgroup per CU claims tiles from the tile table until
+// it is drained.  The table is split into eight contiguous ranges, one per XCD
+// (workgroup b runs on XCD b % 8), so tiles that share operand panels run on the
+// same L2; a workgroup whose range is drained takes tiles from the others.  Each
+// workgroup knows its next tile one tile ahead, and the operand stream runs on
+// across the tile boundary: the last three stages of a tile fetch the next
+// tile's first three stages, so those loads land while the epilogue runs.  A batch of
+// subjects (different N, L, thresholds) shares one launch: every tile item
+// carries its subject's operand panel and stage count.
+//
 // Work split: a 256 x 256 tile per workgroup of 4 waves; each wave owns a
 // 128 x 128 sub-tile held as 4 x 4 MFMA 32x32 accumulator blocks (256 AGPRs).
 //
@@ -16,22 +26,26 @@
 // voxel row r there is one UNIT-byte record at ((kb * N_pad) + r) * UNIT, so
 // one stage of a 256-row panel is a contiguous 256 * UNIT bytes.  Stages move
 // global -> LDS with global_load_lds_dwordx4 (no registers in between) into a
-// ring of 4 slots; the load for stage kt + 4 is issued while stage kt is
-// multiplied, so every load has three stages of MFMA work to arrive in.  The
-// MFMA fragments are double-buffered in registers: the LDS reads of stage
-// kt + 1 are issued before the MFMAs of stage kt.
+// ring of 4 slots; the load for stream position p + 3 is issued while position
+// p is multiplied, so every load has two to three stages of MFMA work to
+// arrive in.
 //
 // Precisions (template PREC), all accumulating in FP32:
 //   0 FP32    v_mfma_f32_32x32x2_f32,   UNIT 64: z[16] fp32
 //   1 BF16    v_mfma_f32_32x32x16_bf16, UNIT 32: hi[16]
 //   2 BF16X3  v_mfma_f32_32x32x16_bf16, UNIT 64: hi[16] lo[16]; hi*hi + hi*lo + lo*hi
 //
-// Epilogue: each kept pair (r > keep_thr) sets a bit in a tile-row and a
-// tile-column bitmap in LDS; per-row and per-column popcounts, two block scans
-// and one global atomic give every pair its slot in both lists.  Pairs with
-// r <= sure_thr are also queued as candidates for the FP64 recheck.
+// Epilogue (in an LDS region beside the ring): each kept pair (r > keep_thr)
+// sets a bit in a tile-row and a tile-column bitmap; per-row and per-column
+// popcounts, two block scans and one global atomic give every pair its slot in
+// both lists.  Kept pairs are recorded in a per-wave LDS list that continues, for
+// dense tiles, in a per-wave spill area in global memory.  Pairs with
+// r <= sure_thr are also queued as candidates for the FP64 recheck.  In
+// histogram mode the epilogue adds every valid pair's bin to an LDS histogram
+// that lives across tiles and is flushed when the subject changes.
 #include "common.hpp"
 
+#include <algorithm>
 #include <atomic>
 
 namespace bna {
@@ -40,14 +54,14 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) void gvoid;
 typedef __attribute__((address_space(3))) void lvoid;
 
 constexpr int CT_THREADS = 256;
-constexpr int NSLOT = 4;  // LDS ring of k16 stages
-// Epilogue LDS: bitmaps (16 KB), offsets (2 KB), scan partials, then the lists.
-constexpr int EPI_LIST_OFF = 20 * 1024;
+constexpr int NSLOT = 4;                   // LDS ring of k16 stages (power of two)
+constexpr int LDS_MAX = 160 * 1024;        // gfx950 LDS per workgroup
+constexpr int CTL_BYTES = 64;              // tile-claim broadcast words at the top of LDS
+constexpr int EPI_FIXED = 16384 + 2048 + 64;  // bitmaps, offsets, scan partials
 
 template <int PREC>
 struct CorrCfg {
@@ -56,7 +70,14 @@ struct CorrCfg {
     static constexpr int PANEL = TILE * UNIT;            // bytes of one side of a stage
     static constexpr int SLOT = 2 * PANEL;               // A + B
     static constexpr int DMA_PER_SIDE = PANEL / 1024 / 4;  // 1 KB wave instructions per wave per side
+    static constexpr int DPS = 2 * DMA_PER_SIDE;         // DMA instructions per wave per stage
+    static constexpr int RING = NSLOT * SLOT;            // epilogue region starts here
+    static constexpr int LIST_RAW = (LDS_MAX - CTL_BYTES - RING - EPI_FIXED) / (4 * 8);
+    static constexpr int LIST_LDS = (LIST_RAW > 2048 ? 2048 : LIST_RAW) & ~7;  // kept pairs per wave
 };
+static_assert(CorrCfg<2>::LIST_LDS >= 256, "epilogue lists do not fit beside the ring");
+static_assert(CorrCfg<0>::RING + EPI_FIXED + CTL_BYTES <= LDS_MAX, "LDS layout");
+static_assert(HIST_BINS * 4 <= EPI_FIXED + 4 * 8 * CorrCfg<2>::LIST_LDS, "histogram region");
 
 // XOR swizzle of the 16-byte chunk index within a row so that the 16 lanes of
 // each ds_read_b128 phase (16 consecutive rows, one chunk each) cover 16
@@ -69,13 +90,48 @@ __device__ __forceinline__ int swz(int row) {
 // s_waitcnt vmcnt <= n and lgkmcnt == 0 (gfx9 encoding: vmcnt bits 3:0 and
 // 15:14, expcnt 6:4 left unconstrained, lgkmcnt 11:8).  The lgkmcnt part makes
 // sure this wave's LDS reads of a slot have completed before the barrier that
-// lets the slot be refilled.
+// lets the slot be refilled.  The stage loop pairs it with a bare s_barrier: a
+// fence there would make the compiler wait for every stage load in flight.
 #define BNA_WAIT_VMCNT(n) \
     __builtin_amdgcn_s_waitcnt((((n) & 0xf) | (((n) >> 4) << 14)) | (0x7 << 4))
 
+// Workgroup barrier for the epilogue's LDS traffic: the wave's LDS operations
+// complete (lgkmcnt 0) and the compiler moves no memory access across it.
+// Outstanding global loads and stores, the next tile's operand stream among
+// them, are not waited for (an LDS fence would wait for that stream, whose LDS
+// writes are tracked by vmcnt; the stream's slots are disjoint from the
+// epilogue's LDS and are waited for before the next tile reads them).
+__device__ __forceinline__ void lds_sync() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), vmcnt and expcnt unconstrained
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Next tile for a workgroup homed on XCD `xcd`: its own range first, then the
+// others in turn.  Every index of every range is handed out exactly once.
+__device__ int64_t claim_tile(unsigned int *claim, int64_t n_items, int64_t per, int xcd)
+{
+    for (int r = 0; r < 8; r++) {
+        const int x = (xcd + r) & 7;
+        const int64_t lo = (int64_t)x * per;
+        const int64_t cnt = n_items - lo < per ? n_items - lo : per;
+        if (cnt <= 0) continue;
+        if ((int64_t)__hip_atomic_load(&claim[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= cnt) continue;
+        const unsigned int t = atomicAdd(&claim[x], 1u);
+        if ((int64_t)t < cnt) return lo + (int64_t)t;
+    }
+    return -1;
+}
+
 // Block-wide exclusive scan of one int per thread (256 threads).
-__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total, int lane, int w) {
     int inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -83,7 +139,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
         if (lane >= d) inc += t;
     }
     if (lane == 63) wsum[w] = inc;
-    __syncthreads();
+    lds_sync();
     int pre = 0;
     total = 0;
 #pragma unroll
@@ -101,34 +157,30 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total) {
 // accumulators stay in registers.  Element e of a block sits at tile row
 // wr0 + 32 RB + 8 (e / 4) + 4 half + e % 4 and tile column wc0 + 32 CB + lane % 32.
 struct EpiCtx {
-    int N, gi0, gj0, wr0, wc0, half, lrow;
+    int N, gi0, gj0, wr0, wc0, half, lrow, lane;
     bool diag;
     float keep;
 };
 
-// Per-wave kept-pair list: the first LIST_LDS entries in LDS, the rest (dense
-// tiles only) in a global overflow area reserved on first need.
-constexpr int LIST_LDS = 2048;
-constexpr int WAVE_MAX = 128 * 128;  // a wave's whole sub-tile
-
 // Records are stored packed in one 64-bit word (tile row | tile column << 16
-// in the low half, r bits in the high half), with explicit LDS and global
-// address spaces.
+// in the low half, r bits in the high half).  The first `cap` records of a wave
+// go to LDS, the rest (dense tiles only) to the wave's own spill area in global
+// memory, which holds the remainder of a whole 128 x 128 sub-tile.
 typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef __attribute__((address_space(1))) uint64_t glb_u64;
+constexpr int WAVE_PAIRS = 128 * 128;
 struct WaveList {
     lds_u64 *lds;
-    glb_u64 *ovf;    // null until reserved
-    int n;           // entries so far (wave-uniform)
-    bool lost;       // the overflow pool was exhausted: the host enlarges it and reruns
+    glb_u64 *spill;
+    int cap;  // LDS entries
+    int n;    // kept pairs of the wave so far (wave-uniform)
 };
 
 __device__ __forceinline__ bool epi_in(const EpiCtx &c, int lr, int lc) {
     return (c.gi0 + lr < c.N) && (c.gj0 + lc < c.N) && (!c.diag || lr < lc);
 }
 
-template <int RB, int CB>
-__device__ __forceinline__ void epi_hist(const f32x16_t &a, const EpiCtx &c, uint32_t *hist) {
+__device__ __forceinline__ void epi_hist(const f32x16_t &a, int RB, int CB, const EpiCtx &c, uint32_t *hist) {
     const int lc = c.wc0 + CB * 32 + c.lrow;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
@@ -142,122 +194,168 @@ __device__ __forceinline__ void epi_hist(const f32x16_t &a, const EpiCtx &c, uin
     }
 }
 
-// The single pass over the accumulators: marks the block's kept pairs in both
-// bitmaps and appends them to the wave's list at ballot-compacted positions.
-template <int RB, int CB>
-__device__ __forceinline__ void epi_mark(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits, uint32_t *colbits,
-                                         WaveList &wl, KeptPair *ovf_pool, unsigned long long *ovf_count,
-                                         int64_t ovf_cap)
+__device__ __forceinline__ void epi_record(WaveList &wl, uint64_t m, uint64_t lt, int lr, int lc, float v,
+                                           uint32_t *rowbits, uint32_t *colbits)
 {
-    if (wl.ovf == nullptr && !wl.lost && wl.n + 1024 > LIST_LDS) {
-        // this block could pass the LDS capacity: reserve the wave's worst case
-        unsigned long long base = 0;
-        if ((threadIdx.x & 63) == 0) base = atomicAdd(ovf_count, (unsigned long long)(WAVE_MAX - LIST_LDS));
-        base = __shfl(base, 0, 64);
-        if ((int64_t)(base + (WAVE_MAX - LIST_LDS)) <= ovf_cap)
-            wl.ovf = (glb_u64 *)(ovf_pool + base);
-        else
-            wl.lost = true;
-    }
+    atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
+    atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
+    const int k = wl.n + __popcll(m & lt);
+    const uint64_t rec = (uint64_t)((uint32_t)lr | ((uint32_t)lc << 16)) |
+                         ((uint64_t)__builtin_bit_cast(uint32_t, v) << 32);
+    if (k < wl.cap)
+        wl.lds[k] = rec;
+    else
+        wl.spill[k - wl.cap] = rec;
+}
+
+// The single pass over one accumulator block of a tile that lies inside the
+// matrix and off the diagonal, where every element is a pair to test: marks
+// the kept pairs in both bitmaps and appends them to the wave's list at
+// ballot-compacted positions.  The block is tested four elements (one row
+// quad) at a time: a quad none of whose elements exceeds the threshold costs
+// three maxima, one compare and one ballot (a NaN never wins the maximum and is
+// never kept); inside a quad only the elements some lane keeps take the
+// recording branch.
+template <int RB, int CB>
+__device__ __forceinline__ void epi_mark_inner(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits,
+                                               uint32_t *colbits, WaveList &wl)
+{
     const int lc = c.wc0 + CB * 32 + c.lrow;
-    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    const uint64_t lt = (1ull << c.lane) - 1ull;
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float mx = fmaxf(fmaxf(a[4 * g], a[4 * g + 1]), fmaxf(a[4 * g + 2], a[4 * g + 3]));
+        if (__ballot(mx > c.keep) == 0) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = 4 * g + q;
+            const bool kp = a[e] > c.keep;
+            const uint64_t m = __ballot(kp);
+            if (m == 0) continue;
+            if (kp) epi_record(wl, m, lt, c.wr0 + RB * 32 + g * 8 + c.half * 4 + q, lc, a[e], rowbits, colbits);
+            wl.n += __popcll(m);
+        }
+    }
+}
+
+// The same for a tile on the diagonal or at the matrix edge, where each element
+// is first checked to be a pair (row < column on the diagonal, both inside N).
+template <int RB, int CB>
+__device__ __forceinline__ void epi_mark_edge(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits,
+                                              uint32_t *colbits, WaveList &wl)
+{
+    const int lc = c.wc0 + CB * 32 + c.lrow;
+    const uint64_t lt = (1ull << c.lane) - 1ull;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
-        const float v = a[e];
-        const bool kp = v > c.keep && epi_in(c, lr, lc);
+        const bool kp = a[e] > c.keep && epi_in(c, lr, lc);
         const uint64_t m = __ballot(kp);
-        if (kp) {
-            atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
-            atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
-            const int k = wl.n + __popcll(m & lt);
-            const uint64_t rec = (uint64_t)((uint32_t)lr | ((uint32_t)lc << 16)) |
-                                 ((uint64_t)__builtin_bit_cast(uint32_t, v) << 32);
-            if (k < LIST_LDS) wl.lds[k] = rec;
-            else if (wl.ovf) wl.ovf[k - LIST_LDS] = rec;
-        }
+        if (kp) epi_record(wl, m, lt, lr, lc, a[e], rowbits, colbits);
         wl.n += __popcll(m);
     }
 }
 
-// Same marking for a tile that lies inside the matrix and off the diagonal,
-// where every element is a pair to test.  The sixteen comparisons become
-// sixteen wave masks first; a block none of whose elements is kept (most of
-// them at useful thresholds) costs those compares and one OR, and inside a
-// block only the elements some lane keeps take the recording branch.
-template <int RB, int CB>
-__device__ __forceinline__ void epi_mark_inner(const f32x16_t &a, const EpiCtx &c, uint32_t *rowbits,
-                                               uint32_t *colbits, WaveList &wl, KeptPair *ovf_pool,
-                                               unsigned long long *ovf_count, int64_t ovf_cap)
+// Slots of pair (lr, lc) in the tile's row-view and column-view lists.
+struct SlotCtx {
+    const uint32_t *rowbits, *colbits;
+    const int *rowoff, *coloff;
+    int64_t base, total;
+};
+__device__ __forceinline__ void pair_slots(const SlotCtx &s, int lr, int lc, int64_t &ps, int64_t &qs)
 {
-    uint64_t m[16];
-    uint64_t any = 0;
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-        m[e] = __ballot(a[e] > c.keep);
-        any |= m[e];
-    }
-    if (any == 0) return;
-    if (wl.ovf == nullptr && !wl.lost && wl.n + 1024 > LIST_LDS) {
-        unsigned long long base = 0;
-        if ((threadIdx.x & 63) == 0) base = atomicAdd(ovf_count, (unsigned long long)(WAVE_MAX - LIST_LDS));
-        base = __shfl(base, 0, 64);
-        if ((int64_t)(base + (WAVE_MAX - LIST_LDS)) <= ovf_cap)
-            wl.ovf = (glb_u64 *)(ovf_pool + base);
-        else
-            wl.lost = true;
-    }
-    const int lc = c.wc0 + CB * 32 + c.lrow;
-    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-        if (m[e] == 0) continue;
-        const float v = a[e];
-        if (v > c.keep) {
-            const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
-            atomicOr(&rowbits[lr * 8 + (lc >> 5)], 1u << (lc & 31));
-            atomicOr(&colbits[lc * 8 + (lr >> 5)], 1u << (lr & 31));
-            const int k = wl.n + __popcll(m[e] & lt);
-            const uint64_t rec = (uint64_t)((uint32_t)lr | ((uint32_t)lc << 16)) |
-                                 ((uint64_t)__builtin_bit_cast(uint32_t, v) << 32);
-            if (k < LIST_LDS) wl.lds[k] = rec;
-            else if (wl.ovf) wl.ovf[k - LIST_LDS] = rec;
+    int p = s.rowoff[lr], q = s.coloff[lc];
+    const int rwd = lc >> 5, cwd = lr >> 5;
+    for (int w = 0; w < rwd; w++) p += __popc(s.rowbits[lr * 8 + w]);
+    for (int w = 0; w < cwd; w++) q += __popc(s.colbits[lc * 8 + w]);
+    p += __popc(s.rowbits[lr * 8 + rwd] & ((1u << (lc & 31)) - 1u));
+    q += __popc(s.colbits[lc * 8 + cwd] & ((1u << (lr & 31)) - 1u));
+    ps = s.base + p;
+    qs = s.base + s.total + q;
+}
+
+// Writes one kept pair's two entries and, for band pairs, its candidate record
+// (one counter update per wave and call, slots by ballot rank).
+__device__ __forceinline__ void emit_pair(const SubjDesc &sd, const EpiCtx &c, const SlotCtx &s, bool fits, bool act,
+                                          int lr, int lc, float v)
+{
+    int64_t ps = 0, qs = 0;
+    if (act) {
+        pair_slots(s, lr, lc, ps, qs);
+        if (fits) {
+            sd.entries[ps] = Entry{c.gj0 + lc, v};
+            sd.entries[qs] = Entry{c.gi0 + lr, v};
         }
-        wl.n += __popcll(m[e]);
+    }
+    const bool cand = act && !(v > sd.sure_thr);
+    const uint64_t cm = __ballot(cand);
+    if (cm) {
+        const int lane = c.lane;
+        const int leader = __builtin_ctzll(cm);
+        unsigned long long cb = 0;
+        if (lane == leader) cb = atomicAdd(sd.cand_count, (unsigned long long)__popcll(cm));
+        cb = __shfl(cb, leader, 64);
+        if (cand) {
+            const unsigned long long ci = cb + (unsigned long long)__popcll(cm & ((1ull << lane) - 1ull));
+            if ((int64_t)ci < sd.cand_cap) sd.cands[ci] = Candidate{c.gi0 + lr, c.gj0 + lc, ps, qs, v, 0};
+        }
     }
 }
 
+// One accumulator block at a time: the block is copied to VGPRs behind an
+// opaque barrier, so the compiler cannot interleave (and keep live) the copies of
+// several blocks; without it the epilogue spills.
+#define BNA_BLOCK_T(F, RB, CB, ...)                                                                \
+    do {                                                                                           \
+        f32x16_t v_ = acc[RB][CB];                                                                 \
+        asm volatile("" : "+v"(v_));                                                               \
+        F<RB, CB>(v_, __VA_ARGS__);                                                                \
+    } while (0)
+#define BNA_FOR_BLOCKS_T(F, ...)                                                                   \
+    do {                                                                                           \
+        BNA_BLOCK_T(F, 0, 0, __VA_ARGS__); BNA_BLOCK_T(F, 0, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 0, 2, __VA_ARGS__); BNA_BLOCK_T(F, 0, 3, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 1, 0, __VA_ARGS__); BNA_BLOCK_T(F, 1, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 1, 2, __VA_ARGS__); BNA_BLOCK_T(F, 1, 3, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 2, 0, __VA_ARGS__); BNA_BLOCK_T(F, 2, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 2, 2, __VA_ARGS__); BNA_BLOCK_T(F, 2, 3, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 3, 0, __VA_ARGS__); BNA_BLOCK_T(F, 3, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 3, 2, __VA_ARGS__); BNA_BLOCK_T(F, 3, 3, __VA_ARGS__);                      \
+    } while (0)
 #define BNA_FOR_BLOCKS(F, ...)                                                                     \
     do {                                                                                           \
-        F<0, 0>(acc[0][0], __VA_ARGS__); F<0, 1>(acc[0][1], __VA_ARGS__);                          \
-        F<0, 2>(acc[0][2], __VA_ARGS__); F<0, 3>(acc[0][3], __VA_ARGS__);                          \
-        F<1, 0>(acc[1][0], __VA_ARGS__); F<1, 1>(acc[1][1], __VA_ARGS__);                          \
-        F<1, 2>(acc[1][2], __VA_ARGS__); F<1, 3>(acc[1][3], __VA_ARGS__);                          \
-        F<2, 0>(acc[2][0], __VA_ARGS__); F<2, 1>(acc[2][1], __VA_ARGS__);                          \
-        F<2, 2>(acc[2][2], __VA_ARGS__); F<2, 3>(acc[2][3], __VA_ARGS__);                          \
-        F<3, 0>(acc[3][0], __VA_ARGS__); F<3, 1>(acc[3][1], __VA_ARGS__);                          \
-        F<3, 2>(acc[3][2], __VA_ARGS__); F<3, 3>(acc[3][3], __VA_ARGS__);                          \
+        F(acc[0][0], 0, 0, __VA_ARGS__); F(acc[0][1], 0, 1, __VA_ARGS__);                          \
+        F(acc[0][2], 0, 2, __VA_ARGS__); F(acc[0][3], 0, 3, __VA_ARGS__);                          \
+        F(acc[1][0], 1, 0, __VA_ARGS__); F(acc[1][1], 1, 1, __VA_ARGS__);                          \
+        F(acc[1][2], 1, 2, __VA_ARGS__); F(acc[1][3], 1, 3, __VA_ARGS__);                          \
+        F(acc[2][0], 2, 0, __VA_ARGS__); F(acc[2][1], 2, 1, __VA_ARGS__);                          \
+        F(acc[2][2], 2, 2, __VA_ARGS__); F(acc[2][3], 2, 3, __VA_ARGS__);                          \
+        F(acc[3][0], 3, 0, __VA_ARGS__); F(acc[3][1], 3, 1, __VA_ARGS__);                          \
+        F(acc[3][2], 3, 2, __VA_ARGS__); F(acc[3][3], 3, 3, __VA_ARGS__);                          \
     } while (0)
+
+// Adds a workgroup's LDS histogram to the subject's global one and clears it.
+__device__ __forceinline__ void hist_flush(uint32_t *hist, unsigned long long *dst, int tid)
+{
+    for (int k = tid; k < HIST_BINS; k += CT_THREADS) {
+        const uint32_t h = hist[k];
+        if (h) {
+            atomicAdd(&dst[k], (unsigned long long)h);
+            hist[k] = 0;
+        }
+    }
+}
 
 template <int PREC, int HIST>
 __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc *__restrict__ subjs,
                                                                   const TileItem *__restrict__ items,
-                                                                  int64_t n_items)
+                                                                  int64_t n_items, unsigned int *__restrict__ claim,
+                                                                  uint64_t *__restrict__ spill)
 {
     using C = CorrCfg<PREC>;
     extern __shared__ __align__(1024) unsigned char smem[];
-
-    // XCD-aware order: workgroup b runs on XCD b % 8; give each XCD a contiguous
-    // run of the (band-ordered) tile table so tiles that share panels share an L2.
-    const int64_t per_xcd = (n_items + 7) / 8;
-    const int64_t b = blockIdx.x;
-    const int64_t logical = (b & 7) * per_xcd + (b >> 3);
-    if (logical >= n_items) return;
-    const TileItem it = items[logical];
-    const SubjDesc sd = subjs[it.subj];
-    const int I = it.I, J = it.J;
-    const int N_pad = sd.N_pad;
-    const int nk = sd.L_pad / 16;  // k16 stages
+    unsigned char *const epi = smem + C::RING;
+    int64_t *const ctl = (int64_t *)(smem + LDS_MAX - CTL_BYTES);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -266,61 +364,80 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     const int wc0 = (wave & 1) * 128;
     const int lrow = lane & 31;
     const int half = lane >> 5;
+    const int xcd = blockIdx.x & 7;
+    const int64_t per_xcd = (n_items + 7) / 8;
+
+    if (tid == 0) {
+        const int64_t t0 = claim_tile(claim, n_items, per_xcd, xcd);
+        ctl[0] = t0;
+        ctl[1] = t0 >= 0 ? claim_tile(claim, n_items, per_xcd, xcd) : -1;
+    }
+    if constexpr (HIST) {
+        for (int k = tid; k < HIST_BINS; k += CT_THREADS) ((uint32_t *)epi)[k] = 0;
+    }
+    __syncthreads();
+    int64_t cur = uni64(ctl[0]);
+    int64_t nxt = uni64(ctl[1]);
+    if (cur < 0) return;
+    TileItem ic = items[cur];
+    TileItem in = nxt >= 0 ? items[nxt] : ic;
 
     // DMA addressing.  A wave instruction fills one contiguous KB of LDS, lane l
     // the 16 bytes at l * 16: row l / CH of the KB's rows, physical chunk l % CH,
     // which holds logical chunk (l % CH) ^ swz(row).  Wave w fills rows
     // [w * 64, w * 64 + 64) of each side.
-    const unsigned char *gA = (const unsigned char *)sd.a0 + (int64_t)I * TILE * C::UNIT;
-    const unsigned char *gB = (const unsigned char *)sd.a0 + (int64_t)J * TILE * C::UNIT;
-    const int64_t kstride = (int64_t)N_pad * C::UNIT;  // bytes between k16 blocks
-    constexpr int ROWS_PER_KB = 1024 / C::UNIT;
-    uint32_t dsrc[C::DMA_PER_SIDE];  // byte offset within a stage panel, per instruction
-#pragma unroll
-    for (int q = 0; q < C::DMA_PER_SIDE; q++) {
-        const int row = wave * (TILE / 4) + q * ROWS_PER_KB + lane / C::CH;
-        const int pc = lane % C::CH;
-        dsrc[q] = (uint32_t)(row * C::UNIT + ((pc ^ swz<C::CH>(row)) << 4));
-    }
-#define BNA_DMA(STAGE_IDX)                                                                         \
+    // Instruction q covers rows w * 64 + q * (1024 / UNIT) + lane / CH; those row
+    // offsets are multiples of 16, so the swizzle does not depend on q and the
+    // byte offset is dsrc + q * 1024.
+    const uint32_t dsrc = (uint32_t)((wave * (TILE / 4) + lane / C::CH) * C::UNIT +
+                                     (((lane % C::CH) ^ swz<C::CH>(lane / C::CH)) << 4));
+    int g = 0;  // stream position of the current tile's stage 0; stage s lives in slot (g + s) % NSLOT
+
+    // Stream position s relative to the current tile: its own stage s, then the
+    // next tile's stages (s - nk < 3 < the next tile's nk) from xA/xB with
+    // stride xK, or, after the last tile, a repeat of the last stage (stride 0)
+    // so every wave issues the same count.
+#define BNA_DMA(S)                                                                                 \
     do {                                                                                           \
-        const int st_ = (STAGE_IDX) < nk ? (STAGE_IDX) : nk - 1;                                   \
-        unsigned char *slot_ = smem + ((STAGE_IDX) % NSLOT) * C::SLOT;                             \
-        const unsigned char *a_ = gA + st_ * kstride;                                              \
-        const unsigned char *b_ = gB + st_ * kstride;                                              \
+        const int s_ = (S);                                                                        \
+        const bool own_ = s_ < nk;                                                                 \
+        const int64_t o_ = own_ ? s_ * ic.kstride : (s_ - nk) * xK;                                \
+        const unsigned char *a_ = (own_ ? cA : xA) + o_;                                           \
+        const unsigned char *b_ = (own_ ? cB : xB) + o_;                                           \
+        unsigned char *slot_ = smem + ((g + s_) & (NSLOT - 1)) * C::SLOT;                          \
         _Pragma("unroll") for (int q = 0; q < C::DMA_PER_SIDE; q++) {                              \
             unsigned char *dA = slot_ + (wave * C::DMA_PER_SIDE + q) * 1024;                       \
-            __builtin_amdgcn_global_load_lds((gvoid *)(a_ + dsrc[q]), (lvoid *)dA, 16, 0, 0);      \
-            __builtin_amdgcn_global_load_lds((gvoid *)(b_ + dsrc[q]), (lvoid *)(dA + C::PANEL), 16, 0, 0); \
+            __builtin_amdgcn_global_load_lds((gvoid *)(a_ + dsrc + q * 1024), (lvoid *)dA, 16, 0, 0); \
+            __builtin_amdgcn_global_load_lds((gvoid *)(b_ + dsrc + q * 1024), (lvoid *)(dA + C::PANEL), 16, 0, 0); \
         }                                                                                          \
     } while (0)
-    constexpr int DMA_PER_STAGE = 2 * C::DMA_PER_SIDE;  // per thread (per wave instruction count)
 
-    f32x16_t acc[4][4];
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-        for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-            for (int e = 0; e < 16; e++) acc[rb][cb][e] = 0.0f;
-
-    // Fragment registers, two sets.  bf16: hi and lo 8-element vectors per
-    // block; fp32: two chunks (8 consecutive k) per block for lane half h.
-    u32x4 fa0[4][2], fb0[4][2], fa1[4][2], fb1[4][2];
+    // Fragment registers: bf16 hi and lo 8-element vectors per block; fp32 two
+    // chunks (8 consecutive k) per block for lane half h.  One set: the LDS
+    // reads of a stage are scheduled among its MFMAs by the compiler, and the
+    // registers a second set would take are what keeps the stage loop free of
+    // spills.
+    u32x4 fa[4][2], fb[4][2];
     // logical chunks a lane reads: bf16 (hi h, lo h) = (h, 2 + h); bf16 plain (h);
     // fp32 (2h, 2h + 1)
     const int c0 = (PREC == 0) ? 2 * half : half;
     const int c1 = (PREC == 0) ? 2 * half + 1 : 2 + half;
+    // Row wr0 + 32 blk + lrow of a slot; block offsets are multiples of 32 rows,
+    // so the swizzle is that of lrow and block blk sits 32 * blk * UNIT further.
+    const int sw = swz<C::CH>(lrow);
+    const uint32_t fA0 = (uint32_t)((wr0 + lrow) * C::UNIT + ((c0 ^ sw) << 4));
+    const uint32_t fA1 = (uint32_t)((wr0 + lrow) * C::UNIT + ((c1 ^ sw) << 4));
+    const uint32_t fB0 = (uint32_t)(C::PANEL + (wc0 + lrow) * C::UNIT + ((c0 ^ sw) << 4));
+    const uint32_t fB1 = (uint32_t)(C::PANEL + (wc0 + lrow) * C::UNIT + ((c1 ^ sw) << 4));
 #define BNA_LDFRAG(FA, FB, STAGE_IDX)                                                              \
     do {                                                                                           \
-        const unsigned char *slot_ = smem + ((STAGE_IDX) % NSLOT) * C::SLOT;                       \
+        const unsigned char *slot_ = smem + ((g + (STAGE_IDX)) & (NSLOT - 1)) * C::SLOT;           \
         _Pragma("unroll") for (int blk = 0; blk < 4; blk++) {                                      \
-            const int ra = wr0 + blk * 32 + lrow, rbw = wc0 + blk * 32 + lrow;                     \
-            FA[blk][0] = *(const u32x4 *)(slot_ + ra * C::UNIT + ((c0 ^ swz<C::CH>(ra)) << 4));    \
-            FB[blk][0] = *(const u32x4 *)(slot_ + C::PANEL + rbw * C::UNIT + ((c0 ^ swz<C::CH>(rbw)) << 4)); \
+            FA[blk][0] = *(const u32x4 *)(slot_ + fA0 + blk * 32 * C::UNIT);                       \
+            FB[blk][0] = *(const u32x4 *)(slot_ + fB0 + blk * 32 * C::UNIT);                       \
             if constexpr (PREC != 1) {                                                             \
-                FA[blk][1] = *(const u32x4 *)(slot_ + ra * C::UNIT + ((c1 ^ swz<C::CH>(ra)) << 4)); \
-                FB[blk][1] = *(const u32x4 *)(slot_ + C::PANEL + rbw * C::UNIT + ((c1 ^ swz<C::CH>(rbw)) << 4)); \
+                FA[blk][1] = *(const u32x4 *)(slot_ + fA1 + blk * 32 * C::UNIT);                   \
+                FB[blk][1] = *(const u32x4 *)(slot_ + fB1 + blk * 32 * C::UNIT);                   \
             }                                                                                      \
         }                                                                                          \
     } while (0)
@@ -355,178 +472,216 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         }                                                                                          \
     } while (0)
 
-    // Prologue: stages 0..3 in flight; wait for stage 0, read its fragments.
-    BNA_DMA(0);
-    BNA_DMA(1);
-    BNA_DMA(2);
-    BNA_DMA(3);
-    BNA_WAIT_VMCNT(3 * DMA_PER_STAGE);
-    __builtin_amdgcn_s_barrier();
-    BNA_LDFRAG(fa0, fb0, 0);
-
-    // Iteration kt: fragments of stage kt are in set (kt & 1).  Wait for stage
-    // kt + 1 (two later stages may stay in flight), barrier (every wave has its
-    // stage-kt fragments, so slot kt % 4 is free), refill that slot with stage
-    // kt + 4, read stage kt + 1's fragments, then multiply stage kt.  L_pad is a
-    // multiple of 32, so nk is even and the loop runs whole pairs of stages.
-    for (int kt = 0; kt < nk; kt += 2) {
-        BNA_WAIT_VMCNT(2 * DMA_PER_STAGE);
-        __builtin_amdgcn_s_barrier();
-        BNA_DMA(kt + 4);
-        BNA_LDFRAG(fa1, fb1, kt + 1);
-        BNA_MFMA(fa0, fb0);
-        BNA_WAIT_VMCNT(2 * DMA_PER_STAGE);
-        __builtin_amdgcn_s_barrier();
-        BNA_DMA(kt + 5);
-        BNA_LDFRAG(fa0, fb0, kt + 2);
-        BNA_MFMA(fa1, fb1);
+    // Prologue of the first tile: stream positions 0..2.
+    {
+        const int nk = ic.nk;
+        const unsigned char *cA = (const unsigned char *)ic.panel + (int64_t)ic.I * C::PANEL;
+        const unsigned char *cB = (const unsigned char *)ic.panel + (int64_t)ic.J * C::PANEL;
+        const unsigned char *xA = cA, *xB = cB;
+        const int64_t xK = 0;
+        BNA_DMA(0);
+        BNA_DMA(1);
+        BNA_DMA(2);
     }
-    // drain the ring before the epilogue reuses the LDS
-    BNA_WAIT_VMCNT(0);
-    __syncthreads();
+
+    for (;;) {
+        const int nk = ic.nk;
+        const unsigned char *cA = (const unsigned char *)ic.panel + (int64_t)ic.I * C::PANEL;
+        const unsigned char *cB = (const unsigned char *)ic.panel + (int64_t)ic.J * C::PANEL;
+        const unsigned char *xA = nxt >= 0 ? (const unsigned char *)in.panel + (int64_t)in.I * C::PANEL
+                                           : cA + (int64_t)(nk - 1) * ic.kstride;
+        const unsigned char *xB = nxt >= 0 ? (const unsigned char *)in.panel + (int64_t)in.J * C::PANEL
+                                           : cB + (int64_t)(nk - 1) * ic.kstride;
+        const int64_t xK = nxt >= 0 ? in.kstride : 0;
+        const SubjDesc sd = subjs[ic.subj];  // read now, used by the epilogue
+
+        f32x16_t acc[4][4];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[rb][cb][e] = 0.0f;
+
+        // Everything in flight is waited for first: the previous tile's epilogue
+        // stores may complete out of order with the stage loads, so only an empty
+        // counter proves that the stages issued before them have landed.  (Those
+        // stages, this tile's first three, had the epilogue's time to arrive.)
+        BNA_WAIT_VMCNT(0);
+
+        // Stage kt: wait for it (positions kt + 1 and kt + 2 may stay in flight),
+        // barrier (every wave is done with stage kt - 1, so its slot is free),
+        // refill that slot with position kt + 3, read stage kt's fragments and
+        // multiply them.
+        for (int kt = 0; kt < nk; kt++) {
+            BNA_WAIT_VMCNT(2 * C::DPS);
+            __builtin_amdgcn_s_barrier();
+            BNA_DMA(kt + 3);
+            BNA_LDFRAG(fa, fb, kt);
+            BNA_MFMA(fa, fb);
+        }
+
+        // ------------------------------------------------------------ epilogue
+        // The epilogue's lane coordinates are re-derived from an opaque copy of
+        // the thread index so that the compiler does not hoist its ~thousand
+        // per-element addresses out of the tile loop (they would not fit in
+        // registers across it).
+        int etid = tid;
+        asm volatile("" : "+v"(etid));
+        const int elane = etid & 63, ewave = etid >> 6;
+        // the tile after next is claimed now; the answer is needed at the boundary
+        int64_t nn_claim = -1;
+        if (etid == 0 && nxt >= 0) nn_claim = claim_tile(claim, n_items, per_xcd, xcd);
+        EpiCtx ec;
+        ec.N = sd.N;
+        ec.gi0 = ic.I * TILE;
+        ec.gj0 = ic.J * TILE;
+        ec.wr0 = (ewave >> 1) * 128;
+        ec.wc0 = (ewave & 1) * 128;
+        ec.half = elane >> 5;
+        ec.lrow = elane & 31;
+        ec.lane = elane;
+        ec.diag = (ic.I == ic.J);
+        ec.keep = sd.keep_thr;
+
+        if constexpr (HIST) {
+            BNA_FOR_BLOCKS(epi_hist, ec, (uint32_t *)epi);
+        } else {
+            uint32_t *rowbits = (uint32_t *)epi;         // [256][8] tile-row bitmaps
+            uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
+            int *rowoff = (int *)(colbits + TILE * 8);   // [256] first slot of each tile row
+            int *coloff = rowoff + TILE;                 // [256] first slot of each tile column
+            int *wsum = coloff + TILE;                   // [4]
+            int *wsum2 = wsum + 4;                       // [4]
+            unsigned long long *sbase = (unsigned long long *)(wsum2 + 4);
+            lds_u64 *lists = (lds_u64 *)(epi + EPI_FIXED);  // [4 waves][LIST_LDS]
+            for (int k = etid; k < TILE * 8 * 2; k += CT_THREADS) rowbits[k] = 0;
+            lds_sync();
+
+            WaveList wl{lists + ewave * C::LIST_LDS,
+                        (glb_u64 *)(spill + ((int64_t)blockIdx.x * 4 + ewave) * (WAVE_PAIRS - C::LIST_LDS)),
+                        C::LIST_LDS, 0};
+            const bool full = !(ec.diag || ec.gi0 + TILE > sd.N || ec.gj0 + TILE > sd.N);
+            if (full)
+                BNA_FOR_BLOCKS_T(epi_mark_inner, ec, rowbits, colbits, wl);
+            else
+                BNA_FOR_BLOCKS_T(epi_mark_edge, ec, rowbits, colbits, wl);
+            lds_sync();
+
+            int rc = 0, cc = 0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                rc += __popc(rowbits[etid * 8 + w]);
+                cc += __popc(colbits[etid * 8 + w]);
+            }
+            int total = 0, total2 = 0;
+            const int ro = block_excl_scan(rc, wsum, total, elane, ewave);
+            const int co = block_excl_scan(cc, wsum2, total2, elane, ewave);
+            rowoff[etid] = ro;
+            coloff[etid] = co;
+            if (etid == 0) *sbase = total ? atomicAdd(sd.entry_count, (unsigned long long)(2 * total)) : 0ull;
+            lds_sync();
+            const int64_t base = (int64_t)*sbase;
+            const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
+
+            // fragment tables, [k][row] so that a tile's writes are contiguous
+            const int N = sd.N;
+            if (ec.gi0 + etid < N) {
+                const int64_t f = (int64_t)(ic.J + 1) * N + ec.gi0 + etid;
+                sd.frag_orig[f] = rc;
+                sd.frag_live[f] = rc;
+                sd.frag_off[f] = base + ro;
+            }
+            if (ec.gj0 + etid < N) {
+                const int64_t f = (int64_t)ic.I * N + ec.gj0 + etid;
+                sd.frag_orig[f] = cc;
+                sd.frag_live[f] = cc;
+                sd.frag_off[f] = base + total + co;
+            }
+            const SlotCtx sc{rowbits, colbits, rowoff, coloff, base, total};
+            if (wl.n > wl.cap) __builtin_amdgcn_s_waitcnt(0);  // this wave's spilled records are stored
+            for (int k0 = 0; k0 < wl.n; k0 += 64) {
+                const int k = k0 + elane;
+                const bool act = k < wl.n;
+                uint64_t raw = 0;
+                if (act) raw = k < wl.cap ? (uint64_t)wl.lds[k] : __builtin_nontemporal_load(wl.spill + (k - wl.cap));
+                const uint32_t lo32 = (uint32_t)raw;
+                emit_pair(sd, ec, sc, fits, act, (int)(lo32 & 0xffff), (int)(lo32 >> 16),
+                          __builtin_bit_cast(float, (uint32_t)(raw >> 32)));
+            }
+        }
+
+        // ------------------------------------------------------------ boundary
+        // Publish the claimed tile after next.
+        if (etid == 0) ctl[2] = nn_claim;
+        __syncthreads();
+        const int64_t nn = uni64(ctl[2]);
+        if constexpr (HIST) {
+            if (nxt < 0 || in.subj != ic.subj) {
+                hist_flush((uint32_t *)epi, sd.hist, etid);
+                __syncthreads();
+            }
+        }
+        if (nxt < 0) break;
+        cur = nxt;
+        ic = items[cur];
+        nxt = nn;
+        in = nxt >= 0 ? items[nxt] : ic;
+        g += nk;
+    }
 #undef BNA_DMA
 #undef BNA_LDFRAG
 #undef BNA_MFMA
+}
 
-    // ------------------------------------------------------------ epilogue
-    EpiCtx ec;
-    ec.N = sd.N;
-    ec.gi0 = I * TILE;
-    ec.gj0 = J * TILE;
-    ec.wr0 = wr0;
-    ec.wc0 = wc0;
-    ec.half = half;
-    ec.lrow = lrow;
-    ec.diag = (I == J);
-    ec.keep = sd.keep_thr;
-
-    if constexpr (HIST) {
-        uint32_t *hist = (uint32_t *)smem;
-        for (int k = tid; k < HIST_BINS; k += CT_THREADS) hist[k] = 0;
-        __syncthreads();
-        BNA_FOR_BLOCKS(epi_hist, ec, hist);
-        __syncthreads();
-        for (int k = tid; k < HIST_BINS; k += CT_THREADS)
-            if (hist[k]) atomicAdd(&sd.hist[k], (unsigned long long)hist[k]);
-        return;
-    } else {
-        uint32_t *rowbits = (uint32_t *)smem;        // [256][8] tile-row bitmaps
-        uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
-        int *rowoff = (int *)(colbits + TILE * 8);   // [256] first slot of each tile row
-        int *coloff = rowoff + TILE;                 // [256] first slot of each tile column
-        int *wsum = coloff + TILE;                   // [4]
-        int *wsum2 = wsum + 4;                       // [4]
-        unsigned long long *sbase = (unsigned long long *)(wsum2 + 4);
-        KeptPair *lists = (KeptPair *)(smem + EPI_LIST_OFF);  // [4 waves][LIST_LDS]
-        for (int k = tid; k < TILE * 8 * 2; k += CT_THREADS) rowbits[k] = 0;
-        __syncthreads();
-
-        WaveList wl{(lds_u64 *)(lists + wave * LIST_LDS), nullptr, 0, false};
-        if (ec.diag || ec.gi0 + TILE > sd.N || ec.gj0 + TILE > sd.N)
-            BNA_FOR_BLOCKS(epi_mark, ec, rowbits, colbits, wl, sd.ovf, sd.ovf_count, sd.ovf_cap);
-        else
-            BNA_FOR_BLOCKS(epi_mark_inner, ec, rowbits, colbits, wl, sd.ovf, sd.ovf_count, sd.ovf_cap);
-        __syncthreads();
-
-        int rc = 0, cc = 0;
-#pragma unroll
-        for (int w = 0; w < 8; w++) {
-            rc += __popc(rowbits[tid * 8 + w]);
-            cc += __popc(colbits[tid * 8 + w]);
-        }
-        int total = 0, total2 = 0;
-        const int ro = block_excl_scan(rc, wsum, total);
-        const int co = block_excl_scan(cc, wsum2, total2);
-        rowoff[tid] = ro;
-        coloff[tid] = co;
-        if (tid == 0) *sbase = atomicAdd(sd.entry_count, (unsigned long long)(2 * total));
-        __syncthreads();
-        const int64_t base = (int64_t)*sbase;
-        const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
-
-        // fragment tables, [k][row] so that a tile's writes are contiguous
-        const int N = sd.N;
-        if (ec.gi0 + tid < N) {
-            const int64_t f = (int64_t)(J + 1) * N + ec.gi0 + tid;
-            sd.frag_orig[f] = rc;
-            sd.frag_live[f] = rc;
-            sd.frag_off[f] = base + ro;
-        }
-        if (ec.gj0 + tid < N) {
-            const int64_t f = (int64_t)I * N + ec.gj0 + tid;
-            sd.frag_orig[f] = cc;
-            sd.frag_live[f] = cc;
-            sd.frag_off[f] = base + total + co;
-        }
-        if (wl.n == 0 || wl.lost) return;
-        if (wl.n > LIST_LDS) __builtin_amdgcn_s_waitcnt(0);  // overflow entries stored by this wave
-        const float sure = sd.sure_thr;
-        for (int k = lane; k < wl.n; k += 64) {
-            uint64_t raw;
-            if (k < LIST_LDS)
-                raw = wl.lds[k];
-            else
-                raw = __builtin_nontemporal_load(wl.ovf + (k - LIST_LDS));
-            const uint32_t lo32 = (uint32_t)raw;
-            const int lr = lo32 & 0xffff, lc = lo32 >> 16;
-            const float v = __builtin_bit_cast(float, (uint32_t)(raw >> 32));
-            int p = rowoff[lr], q = coloff[lc];
-            const int rwd = lc >> 5, cwd = lr >> 5;
-            for (int w = 0; w < rwd; w++) p += __popc(rowbits[lr * 8 + w]);
-            for (int w = 0; w < cwd; w++) q += __popc(colbits[lc * 8 + w]);
-            p += __popc(rowbits[lr * 8 + rwd] & ((1u << (lc & 31)) - 1u));
-            q += __popc(colbits[lc * 8 + cwd] & ((1u << (lr & 31)) - 1u));
-            const int64_t ps = base + p, qs = base + total + q;
-            if (fits) {
-                sd.entries[ps] = Entry{ec.gj0 + lc, v};
-                sd.entries[qs] = Entry{ec.gi0 + lr, v};
-            }
-            if (!(v > sure)) {
-                const unsigned long long ci = atomicAdd(sd.cand_count, 1ull);
-                if ((int64_t)ci < sd.cand_cap) sd.cands[ci] = Candidate{ec.gi0 + lr, ec.gj0 + lc, ps, qs, v, 0};
-            }
-        }
-    }
+// Clears the per-range claim counters of one launch.
+__global__ void claim_reset_kernel(unsigned int *claim)
+{
+    if (threadIdx.x < 8) claim[threadIdx.x] = 0u;
 }
 
 template <int PREC, int HIST>
-static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, int64_t n_items, hipStream_t st)
+static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, int64_t n_items, unsigned int *claim,
+                             uint64_t *spill, int grid_max, hipStream_t st)
 {
-    using C = CorrCfg<PREC>;
-    constexpr size_t epi = (size_t)EPI_LIST_OFF + 4 * LIST_LDS * sizeof(KeptPair);
-    constexpr size_t hst = (size_t)HIST_BINS * 4;
-    constexpr size_t ring = (size_t)NSLOT * C::SLOT;
-    constexpr size_t shm = ring > epi ? (ring > hst ? ring : hst) : (epi > hst ? epi : hst);
     static std::atomic<int> attr_dev{-1};
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (attr_dev.load() != dev) {
         e = hipFuncSetAttribute((const void *)corr_tile_kernel<PREC, HIST>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
         if (e != hipSuccess) return e;
         attr_dev.store(dev);
     }
-    const int64_t grid = (n_items + 7) / 8 * 8;
-    if (grid <= 0) return hipSuccess;
-    if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((corr_tile_kernel<PREC, HIST>), dim3((unsigned)grid), dim3(CT_THREADS), shm, st, subj_dev,
-                       items, n_items);
+    if (n_items <= 0) return hipSuccess;
+    if (grid_max < 8 || grid_max % 8) return hipErrorInvalidValue;
+    const int64_t grid = std::min<int64_t>(grid_max, (n_items + 7) / 8 * 8);
+    hipLaunchKernelGGL(claim_reset_kernel, dim3(1), dim3(64), 0, st, claim);
+    hipLaunchKernelGGL((corr_tile_kernel<PREC, HIST>), dim3((unsigned)grid), dim3(CT_THREADS), LDS_MAX, st,
+                       subj_dev, items, n_items, claim, spill);
     return hipGetLastError();
 }
 
 hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const TileItem *items,
-                       int64_t n_items, hipStream_t st)
+                       int64_t n_items, unsigned int *claim, uint64_t *spill, int grid_max, hipStream_t st)
 {
     switch (prec * 2 + (hist_mode ? 1 : 0)) {
-    case 0: return launch_one<0, 0>(subj_dev, items, n_items, st);
-    case 1: return launch_one<0, 1>(subj_dev, items, n_items, st);
-    case 2: return launch_one<1, 0>(subj_dev, items, n_items, st);
-    case 3: return launch_one<1, 1>(subj_dev, items, n_items, st);
-    case 4: return launch_one<2, 0>(subj_dev, items, n_items, st);
-    case 5: return launch_one<2, 1>(subj_dev, items, n_items, st);
+    case 0: return launch_one<0, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 1: return launch_one<0, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 2: return launch_one<1, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 3: return launch_one<1, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 4: return launch_one<2, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 5: return launch_one<2, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// Records of the per-wave spill area for a launch of at most grid_max workgroups.
+int64_t corr_spill_records(int prec, int grid_max)
+{
+    const int cap = prec == 0 ? CorrCfg<0>::LIST_LDS : prec == 1 ? CorrCfg<1>::LIST_LDS : CorrCfg<2>::LIST_LDS;
+    return (int64_t)grid_max * 4 * (WAVE_PAIRS - cap);
 }
 
 }  // namespace bna

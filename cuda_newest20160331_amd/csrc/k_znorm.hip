@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void zmat_kernel(const float *__restrict__ xs,
 hipError_t launch_znorm(const float *x, int layout, int N, int L, int N_pad, int L_pad, double *mu,
                         double *sd, float *xv, uint8_t *valid, int prec, unsigned char *panel, hipStream_t st)
 {
-    if (N_pad % ZV != 0 || L_pad % 32 != 0 || prec < 0 || prec > 2 || L < 1) return hipErrorInvalidValue;
+    if (N_pad % ZV != 0 || L_pad % KPAD != 0 || L_pad < L || prec < 0 || prec > 2 || L < 1) return hipErrorInvalidValue;
     if (layout == 0 && !xv) return hipErrorInvalidValue;
     hipLaunchKernelGGL(znorm_kernel, dim3(N_pad / ZV), dim3(ZT), 0, st, x, layout, N, L, N_pad, L_pad, mu, sd,
                        layout == 0 ? xv : nullptr, valid, prec, panel);

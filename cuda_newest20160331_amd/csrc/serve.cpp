@@ -66,6 +66,7 @@ struct bna_server {
     std::mutex gate_mu;
     std::condition_variable gate_cv;
     int small_active = 0;
+    int grid_cap = 0;  // workgroup cap of big rounds' tile launches: the CUs minus one XCD's share
 };
 
 static void gate_fn(void *arg)
@@ -79,14 +80,14 @@ static void gate_fn(void *arg)
 // two passes for a density threshold.
 static double est_ms(const Req &r)
 {
-    const double nb = (double)((r.N + 255) / 256), Lp = (double)((r.L + 31) / 32 * 32);
+    const double nb = (double)((r.N + 255) / 256), Lp = (double)std::max(64, (r.L + 15) / 16 * 16);
     return nb * (nb + 1) / 2.0 * Lp / 3.0e6 * (r.opts.thr_type == BNA_THR_DENSITY ? 2.0 : 1.0);
 }
 
 extern "C" int64_t bna_request_footprint(int N, int L, const bna_opts *o)
 {
     if (N < 2 || L < 2 || !o) return -1;
-    const int64_t Np = (N + 255) / 256 * 256, Lp = (L + 31) / 32 * 32, nb1 = Np / 256 + 1;
+    const int64_t Np = (N + 255) / 256 * 256, Lp = std::max<int64_t>(64, (L + 15) / 16 * 16), nb1 = Np / 256 + 1;
     const int unit = o->precision == BNA_BF16 ? 32 : 64;
     const double M1 = (double)N * (N - 1) / 2.0;
     // edges: the density target, or 2 % of the pairs for an r threshold
@@ -146,6 +147,8 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
     const bool big = srv->cfg.big_round_ms > 0.0 && work > srv->cfg.big_round_ms;
     if (big) {
         bna_set_chunking(ctx, srv->cfg.chunk_ms, gate_fn, srv);
+        // leave one XCD's worth of CUs to the small rounds of other workers
+        if (srv->grid_cap > 0) bna_set_grid_limit(ctx, srv->grid_cap);
     } else {
         std::lock_guard<std::mutex> lk(srv->gate_mu);
         srv->small_active++;
@@ -240,7 +243,10 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         srv->cv_done.notify_all();
         srv->cv_work.notify_all();  // memory was released: more may be admitted
     }
-    if (big) bna_set_chunking(ctx, 0.0, nullptr, nullptr);
+    if (big) {
+        bna_set_chunking(ctx, 0.0, nullptr, nullptr);
+        bna_set_grid_limit(ctx, 0);
+    }
     // keep the workspace from growing without bound across rounds
     if (bna_workspace_bytes(ctx) > (int64_t)(srv->cfg.hbm_budget / 4)) bna_trim(ctx);
 }
@@ -309,6 +315,10 @@ int bna_server_create(const bna_server_config *cfg, bna_server **out)
     if (srv->cfg.hbm_budget == 0) {
         int64_t fr = 0, tot = 0;
         if (bna_memory_info(srv->ctxs[0], &fr, &tot) == BNA_OK) srv->cfg.hbm_budget = (size_t)(0.8 * (double)fr);
+    }
+    {
+        const int g = bna_grid_size(srv->ctxs[0]);
+        if (g >= 16) srv->grid_cap = g - g / 8;
     }
     for (bna_ctx *c : srv->ctxs) (void)bna_reserve(c, cfg->reserve_per_worker);  // warm: pool and staging
     for (int w = 0; w < cfg->workers; w++) srv->workers.emplace_back(worker, srv, w);

@@ -283,6 +283,15 @@ int64_t bna_request_footprint(int N, int L, const bna_opts *o);
  * called before each piece (it may block, e.g. while a small request runs on
  * another context).  chunk_ms = 0 restores single launches. */
 int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *gate_arg);
+/* Caps the workgroups of later correlation launches (the persistent tile kernel
+ * runs one per CU) at max_workgroups, rounded down to a multiple of 8; 0 lifts
+ * the cap.  A server gives long builds fewer CUs than the device has, so short
+ * requests on other contexts find CUs free.  No counterpart in the reference,
+ * which runs one subject at a time. */
+int bna_set_grid_limit(bna_ctx *ctx, int max_workgroups);
+/* Workgroups of an uncapped correlation launch: the device's CUs rounded up to a
+ * multiple of 8 (-1 for a NULL context).  No counterpart in the reference. */
+int bna_grid_size(bna_ctx *ctx);
 /* Grows the context's memory pool by `bytes` now (allocated and returned to the
  * pool), so later builds draw on it without asking the driver for memory, and
  * sets up the pinned staging buffers; bytes = 0 does only the latter. */
