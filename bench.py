@@ -97,7 +97,8 @@ def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float):
     }
 
 
-def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
+def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms: float | None = None,
+                 reserve_eighths: int | None = None) -> dict:
     """The serving form: n_subj subjects with Poisson arrivals (mean `rate` per
     second), N log-uniform in [5000, 150000], L uniform in [150, 1200], 1 %
     density, a mix of weighted and binary requests, split-BF16.  Latency is
@@ -113,7 +114,8 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
     log(f"serve: generating {n_subj} subjects (sum N*L = {sum(N * L for N, L, _ in subs):.3g})")
     xs = [bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 5000 + q) for q, (N, L, _) in enumerate(subs)]
     lat, batch = [], []
-    with bna.Server(workers=workers, max_batch=8, reserve_per_worker=8 << 30) as srv:
+    with bna.Server(workers=workers, max_batch=8, reserve_per_worker=8 << 30, chunk_ms=chunk_ms,
+                    reserve_eighths=reserve_eighths) as srv:
         # warm the kernels once
         srv.wait(srv.submit(xs[0][:, :2000], density=1.0, device_result=True), graph=False)
         t0 = time.perf_counter()
@@ -179,6 +181,94 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4) -> dict:
     }
 
 
+def density_and_maps(ctx, steps: int) -> dict:
+    """The 1 %-density form: one subject, N = 45,000 x L = 240 (K = 20, phi = 0.6),
+    split-BF16, weighted CSR with exactly k = 10,124,775 edges; then degree and
+    eigenvector centrality on the device-resident CSR.  Roofline bytes: degree
+    4(N+1) + 4 nnz + 4N; one SpMV iteration 4(N+1) + 8 nnz + 8N (x) + 8N (y)."""
+    N, L = 45000, 240
+    x = bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 2000)
+    M1 = N * (N - 1) // 2
+    out = {}
+    with ctx.series(x) as s:
+        for _ in range(2):
+            s.normalize(bna.BF16X3)
+            s.free_device(s.build_device(density=1.0, precision="bf16x3"))
+        ctx._L.bna_synchronize(ctx._h)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.normalize(bna.BF16X3)
+            g = s.build_device(density=1.0, precision="bf16x3")
+            if _ < steps - 1:
+                s.free_device(g)
+        ctx._L.bna_synchronize(ctx._h)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        st = ctx.timings()
+        nnz = g.nnz
+        ctx.degree_device(g)
+        deg_ms = ctx.timings()["degree_ms"]
+        _, lam, it = ctx.eigcent_device(g)
+        t = ctx.timings()
+        ec_ms, iters = t["ec_ms"], t["ec_iters"]
+        s.free_device(g)
+    deg_bytes = 4 * (N + 1) + 4 * nnz + 4 * N
+    spmv_bytes = 4 * (N + 1) + 8 * nnz + 16 * N
+    spmv_gbs = spmv_bytes * iters / (ec_ms / 1e3) / 1e9
+    deg_gbs = deg_bytes / (deg_ms / 1e3) / 1e9
+    out["build"] = {"workload": f"one subject, N={N} x L={L}, 1 % density (k = {nnz // 2} edges), split-BF16, "
+                                "weighted CSR", "ms_per_step": ms, "voxel_pairs_per_s": M1 / (ms / 1e3),
+                    "edges_per_s": (nnz // 2) / (ms / 1e3), "corr_ms": st["corr_ms"], "select_ms": st["select_ms"],
+                    "recheck_ms": st["recheck_ms"], "refine_ms": st["refine_ms"]}
+    out["degree"] = {"ms": deg_ms, "GB_per_s": deg_gbs, "roofline_frac": deg_gbs / 8000.0, "bytes": deg_bytes}
+    out["eigcent"] = {"ms": ec_ms, "iterations": iters, "lambda": lam, "spmv_GB_per_s": spmv_gbs,
+                      "roofline_frac": spmv_gbs / 8000.0, "bytes_per_iteration": spmv_bytes,
+                      "note": "whole power iteration on the device (SpMV, FP64 norms, stopping test); "
+                              "GB/s = algorithmic bytes x iterations / time"}
+    return out
+
+
+def batch_form(n_subj: int, seed: int) -> dict:
+    """The batched form: n_subj subjects with N uniform in [8000, 60000] and L in
+    {150, 240, 480, 1200}, 1 % density each, split-BF16, built by one
+    bna_build_csr_batch (one histogram launch and one emission launch of the
+    persistent tile kernel over every subject's tiles)."""
+    rng = np.random.default_rng(seed)
+    Ns = rng.integers(8000, 60001, size=n_subj)
+    Ls = rng.choice([150, 240, 480, 1200], size=n_subj)
+    log(f"batch: generating {n_subj} subjects (sum N*L = {float(np.sum(Ns * Ls)):.3g})")
+    xs = [bna.synth_factor_model(int(N), int(L), 20, 0.6, seed + q) for q, (N, L) in enumerate(zip(Ns, Ls))]
+    pairs = int(sum(int(N) * (int(N) - 1) // 2 for N in Ns))
+    with bna.Context(0) as ctx:
+        ser = [ctx.series(x) for x in xs]
+        del xs
+        opts = [bna.make_opts(density=1.0, precision="bf16x3") for _ in ser]
+
+        def run():
+            for s_ in ser:
+                s_.normalize(bna.BF16X3)
+            gs, st = bna.build_batch_device(ctx, ser, opts)
+            ctx._L.bna_synchronize(ctx._h)
+            return gs, st
+
+        gs, st = run()  # warm: kernels, workspaces
+        for g in gs:
+            ctx._L.bna_csr_release(ctx._h, bna.C.byref(g))
+        t0 = time.perf_counter()
+        gs, st = run()
+        dt = time.perf_counter() - t0
+        tm = ctx.timings()
+        edges = sum(g.nnz // 2 for g in gs)
+        for g in gs:
+            ctx._L.bna_csr_release(ctx._h, bna.C.byref(g))
+        for s_ in ser:
+            s_.close()
+    return {"subjects": n_subj, "failed": int(sum(1 for v in st if v)), "ms": dt * 1e3,
+            "voxel_pairs_per_s": pairs / dt, "edges": edges, "edges_per_s": edges / dt,
+            "corr_ms": tm["corr_ms"], "corr_launches": tm["corr_launches"],
+            "N_range": [int(Ns.min()), int(Ns.max())], "L_set": sorted(set(int(v) for v in Ls)),
+            "config": "density 1 % each, split-BF16, weighted CSR left in device memory, series resident"}
+
+
 def measured_traffic(N: int, L: int):
     """HBM bytes per launch of the split-BF16 tile kernel at N=45000, L=1200, from
     the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a
@@ -222,6 +312,11 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-chunk-ms", type=float, default=None)
+    ap.add_argument("--serve-reserve-eighths", type=int, default=None)
+    ap.add_argument("--no-maps", action="store_true")
+    ap.add_argument("--batch-subjects", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     a = ap.parse_args()
     if a.calibrate_copy:
@@ -239,13 +334,27 @@ def main() -> int:
     flops = float(N) * (N - 1) * L  # 2L per pair
 
     res = {}
-    with bna.Context(0) as ctx, ctx.series(x) as s:
-        for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"])):
-            log(f"bench: {prec} warmup={a.warmup} steps={a.steps}")
-            ms, corr_ms, st, st_ms = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
-            res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st, "stages_ms": st_ms}
-            log(f"bench: {prec} {ms:.3f} ms/step, corr kernel {corr_ms:.3f} ms, nnz={st[0]} "
-                f"cand={st[1]} flips={st[2]} band={st[3]:.3g} max_err={st[4]:.3g} stages={st_ms}")
+    maps = None
+    tuned = {}
+    with bna.Context(0) as ctx:
+        # tile traversal (band height) per precision and length bucket
+        for prec, Lt in (("bf16x3", L), ("fp32", L), ("bf16x3", 240)):
+            if prec == "fp32" and a.no_fp32:
+                continue
+            gb, gms = ctx.autotune(prec, Lt, N=N)
+            tuned[f"{prec}_L{Lt}"] = {"band_rows": gb, f"tile_ms_at_N{N}": round(gms, 4)}
+        log(f"bench: autotuned {tuned}")
+        with ctx.series(x) as s:
+            for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"])):
+                log(f"bench: {prec} warmup={a.warmup} steps={a.steps}")
+                ms, corr_ms, st, st_ms = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
+                res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st, "stages_ms": st_ms}
+                log(f"bench: {prec} {ms:.3f} ms/step, corr kernel {corr_ms:.3f} ms, nnz={st[0]} "
+                    f"cand={st[1]} flips={st[2]} band={st[3]:.3g} max_err={st[4]:.3g} stages={st_ms}")
+        if not a.no_maps:
+            log("bench: 1 % density form and graph maps")
+            maps = density_and_maps(ctx, 5)
+            log(f"bench: maps {maps}")
 
     b = res["bf16x3"]
     value = M1 / (b["ms"] / 1e3)
@@ -268,6 +377,7 @@ def main() -> int:
                    "N": N, "L": L, "thr": thr, "precision": "bf16x3", "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK["bf16x3"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["bf16x3"], "traffic": measured_traffic(N, L)},
+        "autotune": tuned,
         "edges": b["stats"][0] // 2,
         "edges_per_s": (b["stats"][0] // 2) / (b["ms"] / 1e3),
         "band_pairs_rechecked": b["stats"][1],
@@ -283,9 +393,18 @@ def main() -> int:
         line["fp32_roofline_frac"] = flops / (f["corr_ms"] / 1e3) / 1e12 / PEAK["fp32"]
         line["same_csr_as_fp32"] = f["stats"][0] == b["stats"][0]
         line["fp32_stages_ms"] = f["stages_ms"]
+    if maps is not None:
+        line["density_form"] = maps["build"]
+        line["degree"] = maps["degree"]
+        line["eigcent"] = maps["eigcent"]
+    if not a.no_batch:
+        log("bench: batched form")
+        line["batch"] = batch_form(a.batch_subjects, 0xB17A0000 + 4000)
+        log(f"bench: batch {line['batch']}")
     if not a.no_serve:
         log("bench: serving stream")
-        sv = serve_stream(a.serve_subjects, a.serve_rate, 0xB17A0000 + 5000, a.serve_workers)
+        sv = serve_stream(a.serve_subjects, a.serve_rate, 0xB17A0000 + 5000, a.serve_workers,
+                          a.serve_chunk_ms, a.serve_reserve_eighths)
         line["serving"] = sv
         line["p99_subject_latency_ms"] = sv["p99_ms"]
     if not a.no_cpu:

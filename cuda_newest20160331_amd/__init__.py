@@ -33,7 +33,8 @@ EXPORTS = [
     "bna_build_csr_batch", "bna_trim", "bna_server_config_default", "bna_server_create",
     "bna_server_destroy", "bna_submit", "bna_poll", "bna_wait", "bna_cancel", "bna_server_get_stats",
     "bna_result_free", "bna_request_footprint", "bna_memory_info", "bna_reserve",
-    "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size",
+    "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
+    "bna_autotune_load", "bna_autotune_get", "bna_autotune_set",
 ]
 
 
@@ -60,6 +61,7 @@ class Timings(C.Structure):
         ("csr_ms", C.c_double), ("refine_ms", C.c_double), ("select_ms", C.c_double),
         ("total_ms", C.c_double),
         ("corr_launches", C.c_int64),
+        ("degree_ms", C.c_double), ("ec_ms", C.c_double), ("ec_iters", C.c_int64),
     ]
 
 
@@ -111,6 +113,10 @@ def lib() -> C.CDLL:
                                       C.POINTER(C.c_int)]),
         "bna_trim": (i32, [vp]),
         "bna_csr_release": (i32, [vp, C.POINTER(CSR)]),
+        "bna_autotune": (i32, [vp, i32, i32, i32, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+        "bna_autotune_load": (i32, [vp, C.c_char_p]),
+        "bna_autotune_get": (i32, [vp, i32, i32, C.POINTER(C.c_int)]),
+        "bna_autotune_set": (i32, [vp, i32, i32, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -292,6 +298,44 @@ class Context:
         self._check(self._L.bna_degree(self._h, C.byref(h[0]), _f32p(deg)), "bna_degree")
         return deg
 
+    def autotune(self, precision: str, L: int, N: int = 16384, cache: str | None = None):
+        """Tunes the tile traversal band height for (precision, L's bucket);
+        returns (band height, best tile-kernel ms)."""
+        g = C.c_int()
+        ms = C.c_double()
+        self._check(self._L.bna_autotune(self._h, PRECISIONS[precision], L, N,
+                                         cache.encode() if cache else None, C.byref(g), C.byref(ms)),
+                    "bna_autotune")
+        return g.value, ms.value
+
+    def autotune_load(self, path: str):
+        self._check(self._L.bna_autotune_load(self._h, path.encode()), "bna_autotune_load")
+
+    def band_rows(self, precision: str, L: int) -> int:
+        g = C.c_int()
+        self._check(self._L.bna_autotune_get(self._h, PRECISIONS[precision], L, C.byref(g)), "bna_autotune_get")
+        return g.value
+
+    def set_band_rows(self, precision: str, L: int, rows: int):
+        self._check(self._L.bna_autotune_set(self._h, PRECISIONS[precision], L, rows), "bna_autotune_set")
+
+    def degree_device(self, g: CSR) -> np.ndarray:
+        """Degree of a device-resident CSR (bna_csr from Series.build_device)."""
+        deg = np.empty(g.N, np.float32)
+        self._check(self._L.bna_degree(self._h, C.byref(g), _f32p(deg)), "bna_degree")
+        return deg
+
+    def eigcent_device(self, g: CSR, tol: float = 1e-6, max_iter: int = 10000,
+                       output: int = EC_WEIGHTED_REF):
+        """Eigenvector centrality of a device-resident CSR."""
+        ec = np.empty(g.N, np.float64)
+        lam = C.c_double()
+        it = C.c_int()
+        self._check(self._L.bna_eigcent(self._h, C.byref(g), tol, max_iter, output,
+                                        ec.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam),
+                                        C.byref(it)), "bna_eigcent")
+        return ec, lam.value, it.value
+
     def eigcent(self, g: Graph, tol: float = 1e-6, max_iter: int = 10000,
                 output: int = EC_WEIGHTED_REF, binary: bool = False):
         h = _host_csr(g, drop_values=binary)
@@ -421,7 +465,7 @@ def build_batch(ctx: Context, series: list, opts: list) -> list:
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_size_t), ("workers", C.c_int),
                 ("max_batch", C.c_int), ("reserve_per_worker", C.c_size_t),
-                ("big_round_ms", C.c_double), ("chunk_ms", C.c_double)]
+                ("big_round_ms", C.c_double), ("chunk_ms", C.c_double), ("reserve_eighths", C.c_int)]
 
 
 class Request(C.Structure):
@@ -501,7 +545,7 @@ class Server:
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, workers: int = 2, max_batch: int = 16,
                  reserve_per_worker: int = 0, big_round_ms: float | None = None,
-                 chunk_ms: float | None = None):
+                 chunk_ms: float | None = None, reserve_eighths: int | None = None):
         self._L = _server_lib()
         cfg = ServerConfig()
         self._L.bna_server_config_default(C.byref(cfg))
@@ -511,6 +555,8 @@ class Server:
             cfg.big_round_ms = big_round_ms
         if chunk_ms is not None:
             cfg.chunk_ms = chunk_ms
+        if reserve_eighths is not None:
+            cfg.reserve_eighths = reserve_eighths
         self._h = C.c_void_p()
         rc = self._L.bna_server_create(C.byref(cfg), C.byref(self._h))
         if rc:

@@ -32,12 +32,15 @@ struct bna_ctx {
     size_t held = 0;  // workspace + series + csr bytes allocated through the context
     std::string err;
     bna_timings tm{};
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};  // [0..7] builds, [8..9] graph maps
     std::map<std::string, DevBuf> ws;
     int grid = 256;  // persistent correlation workgroups: CUs rounded up to a multiple of 8
     int grid_limit = 0;  // > 0: launches use at most this many (a multiple of 8)
+    bool graphs = true;  // replay captured HIP graphs where a path has them (BNA_GRAPHS=0: off)
+    // tile traversal band height per (precision, L bucket), from the autotuner
+    int band[3][4] = {{8, 8, 8, 8}, {8, 8, 8, 8}, {8, 8, 8, 8}};
     // tile table cache (single-subject builds): key of the cached table
-    int tt_nb = -1, tt_nk = -1;
+    int tt_nb = -1, tt_nk = -1, tt_band = -1;
     const void *tt_panel = nullptr;
     int64_t tt_kstride = 0;
     int64_t tt_count = 0;
@@ -187,6 +190,8 @@ static int copy_h2d(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
         const int b = (int)(k & 1);
         const size_t n = std::min(STAGE_BYTES, bytes - off);
         if (k >= 2) HIPCHK(ctx, hipEventSynchronize(ctx->stev[b]));  // its previous DMA is done
+        // a chunked (long) build lets the copies of short ones go first
+        if (ctx->chunk_ms > 0.0 && ctx->gate) ctx->gate(ctx->gate_arg);
         memcpy(ctx->stage[b], (const char *)src + off, n);
         HIPCHK(ctx, hipMemcpyAsync((char *)dst + off, ctx->stage[b], n, hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(ctx, hipEventRecord(ctx->stev[b], ctx->stream));
@@ -339,6 +344,10 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         ctx->grid = (ncu + 7) / 8 * 8;
+    {
+        const char *g = std::getenv("BNA_GRAPHS");
+        ctx->graphs = !(g && g[0] == '0');
+    }
     if (hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) hbm_budget = (size_t)(0.9 * (double)fr);
@@ -565,13 +574,17 @@ int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *
 }  // extern "C"
 
 // ------------------------------------------------------------ tile table
+// Series-length buckets of the tuned tile traversal: up to 192, 320 and 640
+// padded points, and longer.
+static int l_bucket(int L_pad) { return L_pad <= 192 ? 0 : L_pad <= 320 ? 1 : L_pad <= 640 ? 2 : 3; }
+static constexpr int BAND_CANDIDATES[4] = {4, 8, 16, 32};
+
 // Upper-triangle tiles in bands of 8 tile-rows: within a band, tile-columns
 // ascend and the band's rows vary fastest, so a run of consecutive items shares
 // a few A panels (the band) and a few B panels (neighbouring columns).
-static void make_tile_table(int nb, int subj, const void *panel, int64_t kstride, int nk,
+static void make_tile_table(int nb, int subj, const void *panel, int64_t kstride, int nk, int G,
                             std::vector<TileItem> &out)
 {
-    const int G = 8;
     for (int I0 = 0; I0 < nb; I0 += G) {
         const int I1 = std::min(nb, I0 + G);
         for (int J = I0; J < nb; J++)
@@ -579,17 +592,18 @@ static void make_tile_table(int nb, int subj, const void *panel, int64_t kstride
     }
 }
 
-static int get_tile_table(bna_ctx *ctx, int nb, const void *panel, int64_t kstride, int nk, TileItem **dev,
+static int get_tile_table(bna_ctx *ctx, int nb, const void *panel, int64_t kstride, int nk, int G, TileItem **dev,
                           int64_t *count)
 {
-    if (ctx->tt_nb == nb && ctx->tt_panel == panel && ctx->tt_kstride == kstride && ctx->tt_nk == nk) {
+    if (ctx->tt_nb == nb && ctx->tt_panel == panel && ctx->tt_kstride == kstride && ctx->tt_nk == nk &&
+        ctx->tt_band == G) {
         *dev = (TileItem *)ctx->ws["tiles"].p;
         *count = ctx->tt_count;
         return BNA_OK;
     }
     ctx->tt_nb = -1;
     std::vector<TileItem> t;
-    make_tile_table(nb, 0, panel, kstride, nk, t);
+    make_tile_table(nb, 0, panel, kstride, nk, G, t);
     TileItem *d = nullptr;
     int rc = ws_get_t(ctx, "tiles", t.size(), &d);
     if (rc) return rc;
@@ -599,6 +613,7 @@ static int get_tile_table(bna_ctx *ctx, int nb, const void *panel, int64_t kstri
     ctx->tt_panel = panel;
     ctx->tt_kstride = kstride;
     ctx->tt_nk = nk;
+    ctx->tt_band = G;
     ctx->tt_count = (int64_t)t.size();
     *dev = d;
     *count = ctx->tt_count;
@@ -805,12 +820,14 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
     if ((rc = ws_get_t(ctx, "batch_subjdesc", hd.size(), dsd))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(*dsd, hd.data(), hd.size() * sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
     auto kstride = [](const Job *j) { return (int64_t)j->s->N_pad * panel_unit(j->o.precision); };
+    auto band = [&](const Job *j) { return ctx->band[j->o.precision][l_bucket(j->s->L_pad)]; };
     if (js.size() == 1)  // cached
         return get_tile_table(ctx, js[0]->s->N_pad / TILE, js[0]->s->panel, kstride(js[0]), js[0]->s->L_pad / 16,
-                              tiles, ntiles);
+                              band(js[0]), tiles, ntiles);
     std::vector<TileItem> t;
     for (size_t q = 0; q < js.size(); q++)
-        make_tile_table(js[q]->s->N_pad / TILE, (int)q, js[q]->s->panel, kstride(js[q]), js[q]->s->L_pad / 16, t);
+        make_tile_table(js[q]->s->N_pad / TILE, (int)q, js[q]->s->panel, kstride(js[q]), js[q]->s->L_pad / 16,
+                        band(js[q]), t);
     if ((rc = ws_get_t(ctx, "batch_tiles", t.size(), tiles))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(*tiles, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
     *ntiles = (int64_t)t.size();
@@ -1292,9 +1309,14 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg)
     if (rc) return rc;
     float *d;
     if ((rc = ws_get_t(ctx, "deg", (size_t)g->N, &d))) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
     HIPCHK(ctx, launch_degree(R, V, g->N, d, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(deg, d, sizeof(float) * (size_t)g->N, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.degree_ms = ms;
     return BNA_OK;
 }
 
@@ -1318,10 +1340,133 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     if ((rc = ws_get_t(ctx, "ec_part", (size_t)3 * ec_nparts(N), &part))) return rc;
     if ((rc = ws_get(ctx, "ec_state", ec_state_bytes(), &stt))) return rc;
     int it = 0;
-    HIPCHK(ctx, run_eigcent(R, C, V, N, tol, max_iter, output, xa, xb, part, stt, outd, lambda, &it, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    HIPCHK(ctx, run_eigcent(R, C, V, N, tol, max_iter, output, xa, xb, part, stt, outd, lambda, &it, ctx->graphs,
+                            ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ec, outd, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.ec_ms = ms;
+    ctx->tm.ec_iters = it;
     if (iters) *iters = it;
+    return BNA_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ autotuner
+// The tile kernel claims tiles in table order, each XCD from its own contiguous
+// range, so the band height G of the table (tiles of G consecutive tile-rows
+// for each tile-column in turn) fixes which operand panels the workgroups of an
+// XCD share in L2.  The best G depends on the series length (the panel bytes per
+// tile) and the precision (bytes per point).  bna_autotune times the emission
+// pass for every candidate on a synthetic subject of the bucket's length and
+// keeps the fastest; results persist in a small text file, one line per
+// (precision, bucket): "<precision> <bucket> <G> <ms>".
+extern "C" {
+
+int bna_autotune_get(bna_ctx *ctx, int precision, int L, int *band_rows)
+{
+    if (!ctx || !band_rows || precision < BNA_FP32 || precision > BNA_BF16X3 || L < 2)
+        return fail(ctx, BNA_ERR_ARG, "bna_autotune_get: bad argument");
+    *band_rows = ctx->band[precision][l_bucket(std::max(LPAD_MIN, round_up(L, KPAD)))];
+    return BNA_OK;
+}
+
+int bna_autotune_set(bna_ctx *ctx, int precision, int L, int band_rows)
+{
+    if (!ctx || precision < BNA_FP32 || precision > BNA_BF16X3 || L < 2 || band_rows < 1 || band_rows > 1024)
+        return fail(ctx, BNA_ERR_ARG, "bna_autotune_set: bad argument");
+    ctx->band[precision][l_bucket(std::max(LPAD_MIN, round_up(L, KPAD)))] = band_rows;
+    ctx->tt_nb = -1;
+    return BNA_OK;
+}
+
+int bna_autotune_load(bna_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return fail(ctx, BNA_ERR_ARG, "bna_autotune_load: NULL argument");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(ctx, BNA_ERR_IO, std::string("bna_autotune_load: cannot open ") + path);
+    int p, b, g, n = 0;
+    double ms;
+    while (std::fscanf(f, "%d %d %d %lf", &p, &b, &g, &ms) == 4) {
+        if (p < BNA_FP32 || p > BNA_BF16X3 || b < 0 || b > 3 || g < 1 || g > 1024) {
+            std::fclose(f);
+            return fail(ctx, BNA_ERR_IO, std::string("bna_autotune_load: bad line in ") + path);
+        }
+        ctx->band[p][b] = g;
+        n++;
+    }
+    std::fclose(f);
+    ctx->tt_nb = -1;
+    return n > 0 ? BNA_OK : fail(ctx, BNA_ERR_IO, std::string("bna_autotune_load: no entries in ") + path);
+}
+
+int bna_autotune(bna_ctx *ctx, int precision, int L, int N, const char *cache_path, int *band_rows, double *best_ms)
+{
+    if (!ctx || precision < BNA_FP32 || precision > BNA_BF16X3 || L < 2 || N < TILE)
+        return fail(ctx, BNA_ERR_ARG, "bna_autotune: bad argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int Lp = std::max(LPAD_MIN, round_up(L, KPAD));
+    const int b = l_bucket(Lp);
+    std::vector<float> x((size_t)N * L);
+    int rc = bna_synth_factor_model(x.data(), N, L, 20, 0.6, 0xB17A7000ull + (uint64_t)b, BNA_TIME_MAJOR);
+    if (rc) return fail(ctx, rc, "bna_autotune: synthetic series");
+    bna_series *s = nullptr;
+    if ((rc = bna_series_create(ctx, x.data(), N, L, BNA_TIME_MAJOR, 0, &s))) return rc;
+    bna_opts o;
+    bna_opts_default(&o);
+    o.thr_type = BNA_THR_R;
+    o.thr = 0.4f;
+    o.precision = precision;
+    o.weighted = 0;
+    const int saved = ctx->band[precision][b];
+    int best = saved;
+    double best_t = 1e30;
+    for (int c = 0; c < 4 && rc == BNA_OK; c++) {
+        ctx->band[precision][b] = BAND_CANDIDATES[c];
+        double t = 1e30;
+        for (int rep = 0; rep < 3 && rc == BNA_OK; rep++) {  // first run warms the table and workspaces
+            bna_csr g;
+            rc = bna_build_csr_series(ctx, s, &o, &g);
+            if (rc == BNA_OK) {
+                if (rep > 0) t = std::min(t, ctx->tm.corr_ms);
+                bna_csr_release(ctx, &g);
+            }
+        }
+        if (rc == BNA_OK && t < best_t) {
+            best_t = t;
+            best = BAND_CANDIDATES[c];
+        }
+    }
+    bna_series_destroy(ctx, s);
+    ctx->band[precision][b] = rc == BNA_OK ? best : saved;
+    ctx->tt_nb = -1;
+    if (rc) return rc;
+    if (band_rows) *band_rows = best;
+    if (best_ms) *best_ms = best_t;
+    if (cache_path) {
+        // keep the file's other entries, replace this one
+        std::vector<std::string> keep;
+        if (FILE *f = std::fopen(cache_path, "r")) {
+            int p, bb, g;
+            double ms;
+            while (std::fscanf(f, "%d %d %d %lf", &p, &bb, &g, &ms) == 4)
+                if (!(p == precision && bb == b)) {
+                    char line[96];
+                    std::snprintf(line, sizeof line, "%d %d %d %.6f", p, bb, g, ms);
+                    keep.push_back(line);
+                }
+            std::fclose(f);
+        }
+        FILE *f = std::fopen(cache_path, "w");
+        if (!f) return fail(ctx, BNA_ERR_IO, std::string("bna_autotune: cannot write ") + cache_path);
+        for (const std::string &l : keep) std::fprintf(f, "%s\n", l.c_str());
+        std::fprintf(f, "%d %d %d %.6f\n", precision, b, best, best_t);
+        std::fclose(f);
+    }
     return BNA_OK;
 }
 

@@ -2,18 +2,22 @@
 // over a device CSR.
 //
 // Degree follows W/Degree/Degree/Degree.cpp:83-87: a float running sum of V in
-// CSR order, one lane per row so the order (and every rounding) is the
-// reference's; binary graphs take the row length (B/Degree/Degree/Degree.cpp:84-91).
+// CSR order, so the order (and every rounding) is the reference's.  One wave per
+// row: the lanes load 64 consecutive values (coalesced) and every lane adds them
+// in order, taking each from its lane with readlane; binary graphs take the row
+// length (B/Degree/Degree/Degree.cpp:84-91).
 //
 // Eigenvector centrality follows Lead_Vector_GPU (W/CUEC/weighted-net.cu:55-221):
 //   x0 <- x; n = ||x||; x <- x * (1/n); y = A x; x <- y;
 //   err1 = ||x - x0||; err2 = ||x + x0||; loop while err1 > tol && err2 > tol && it < max.
 // One iteration is two launches: spmv_ec_kernel computes y = A (x * inv_n) with
-// one wave per row (coalesced C and V reads, FP64 accumulation) and writes
-// per-block partial sums of (y-x)^2, (y+x)^2 and y^2; ec_ctrl_kernel reduces the
-// partials in a fixed order, applies the stopping rule, and stores the next
-// 1/n.  Once the rule fires, every later launch returns at once, so the host can
-// queue iterations in batches and look at the flag once per batch.
+// one wave per row (coalesced C and V reads, FP64 accumulation) over a fixed
+// grid of EC_SPMV_BLOCKS workgroups that stride over the rows, and writes one
+// partial sum of (y-x)^2, (y+x)^2 and y^2 per workgroup; ec_ctrl_kernel reduces
+// the partials in a fixed order, applies the stopping rule, and stores the next
+// 1/n.  Once the rule fires, every later launch returns at once, so the host
+// queues iterations in batches of 16 and looks at the flag once per batch.  A
+// batch is captured once into a HIP graph and replayed (32 launches become one).
 #include "common.hpp"
 
 namespace bna {
@@ -21,21 +25,28 @@ namespace bna {
 __global__ __launch_bounds__(256) void degree_kernel(const int32_t *__restrict__ R, const float *__restrict__ V, int N,
                                                      float *__restrict__ deg)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     if (i >= N) return;
     const int b = R[i], e = R[i + 1];
     if (!V) {
-        deg[i] = (float)(e - b);
+        if (lane == 0) deg[i] = (float)(e - b);
         return;
     }
     float s = 0.0f;
-    for (int t = b; t < e; t++) s += V[t];
-    deg[i] = s;
+    for (int base = b; base < e; base += 64) {
+        const float v = base + lane < e ? V[base + lane] : 0.0f;
+        const int n = e - base < 64 ? e - base : 64;
+        const int vb = __builtin_bit_cast(int, v);
+        for (int k = 0; k < n; k++) s += __builtin_bit_cast(float, __builtin_amdgcn_readlane(vb, k));
+    }
+    if (lane == 0) deg[i] = s;
 }
 
 hipError_t launch_degree(const int32_t *R, const float *V, int N, float *deg, hipStream_t st)
 {
-    hipLaunchKernelGGL(degree_kernel, dim3((N + 255) / 256), dim3(256), 0, st, R, V, N, deg);
+    const int64_t threads = (int64_t)N * 64;
+    hipLaunchKernelGGL(degree_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, R, V, N, deg);
     return hipGetLastError();
 }
 
@@ -52,7 +63,8 @@ struct EcState {
     int32_t pad;
 };
 
-constexpr int EC_ROWS_PER_BLOCK = 4;  // one wave per row
+constexpr int EC_ROWS_PER_BLOCK = 4;  // waves per workgroup, one row at a time each
+constexpr int EC_SPMV_BLOCKS = 16384;  // fixed grid, so the partial sums keep one order
 
 __global__ __launch_bounds__(256) void ec_init_kernel(double *x, int N)
 {
@@ -89,9 +101,8 @@ __global__ __launch_bounds__(256) void spmv_ec_kernel(const int32_t *__restrict_
     __shared__ double sh[3][EC_ROWS_PER_BLOCK];
     const double inv = st->inv_n;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row = blockIdx.x * EC_ROWS_PER_BLOCK + w;
     double a = 0.0, b = 0.0, c = 0.0;
-    if (row < N) {
+    for (int row = blockIdx.x * EC_ROWS_PER_BLOCK + w; row < N; row += gridDim.x * EC_ROWS_PER_BLOCK) {
         const int rb = R[row], re = R[row + 1];
         double s = 0.0;
         if (V) {
@@ -105,9 +116,9 @@ __global__ __launch_bounds__(256) void spmv_ec_kernel(const int32_t *__restrict_
             y[row] = s;
             const double xo = x[row];
             const double dm = s - xo, dp = s + xo;
-            a = dm * dm;
-            b = dp * dp;
-            c = s * s;
+            a += dm * dm;
+            b += dp * dp;
+            c += s * s;
         }
     }
     if (lane == 0) {
@@ -190,7 +201,7 @@ namespace bna {
 // Returns the iteration count through *iters and the eigenvalue through *lambda.
 hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, double tol, int max_iter,
                        int output, double *xa, double *xb, double *part, void *state_dev, double *out,
-                       double *lambda, int *iters, hipStream_t st)
+                       double *lambda, int *iters, bool use_graph, hipStream_t st)
 {
     EcState *sd = (EcState *)state_dev;
     EcState h{};
@@ -204,24 +215,49 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
     hipLaunchKernelGGL(ec_ctrl_kernel, dim3(1), dim3(1024), 0, st, part, nb_norm, sd, 1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
-    const int nb_spmv = (N + EC_ROWS_PER_BLOCK - 1) / EC_ROWS_PER_BLOCK;
+    const int rows_blocks = (N + EC_ROWS_PER_BLOCK - 1) / EC_ROWS_PER_BLOCK;
+    const int nb_spmv = rows_blocks < EC_SPMV_BLOCKS ? rows_blocks : EC_SPMV_BLOCKS;
     double *buf[2] = {xa, xb};
-    int launched = 0;
-    const int batch = 16;
-    while (launched < max_iter) {
-        const int upto = (launched + batch < max_iter) ? launched + batch : max_iter;
-        for (int k = launched; k < upto; k++) {
+    const int batch = 16;  // even, so every batch starts from buf[0]
+    auto issue = [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++) {
             hipLaunchKernelGGL(spmv_ec_kernel, dim3(nb_spmv), dim3(256), 0, st, R, C, V, N, buf[k & 1],
                                buf[(k + 1) & 1], part, sd);
             hipLaunchKernelGGL(ec_ctrl_kernel, dim3(1), dim3(1024), 0, st, part, nb_spmv, sd, 0);
         }
-        if ((e = hipGetLastError()) != hipSuccess) return e;
+    };
+    // Whole batches replay one captured graph; the last, partial batch (and
+    // every batch when graphs are off) is launched directly.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (use_graph && max_iter >= batch) {
+        if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+        issue(0, batch);
+        e = hipStreamEndCapture(st, &graph);
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return e;
+        }
+    }
+    int launched = 0;
+    while (launched < max_iter) {
+        const int upto = (launched + batch < max_iter) ? launched + batch : max_iter;
+        if (exec && upto - launched == batch)
+            e = hipGraphLaunch(exec, st);
+        else {
+            issue(launched, upto);
+            e = hipGetLastError();
+        }
+        if (e != hipSuccess) break;
         launched = upto;
-        e = hipMemcpyAsync(&h, sd, sizeof(EcState), hipMemcpyDeviceToHost, st);
-        if (e != hipSuccess) return e;
-        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(&h, sd, sizeof(EcState), hipMemcpyDeviceToHost, st)) != hipSuccess) break;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
         if (h.done) break;
     }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return e;
     const int K = h.iter;  // iterations actually executed
     if (K == 0) {
         // no iteration: the reference returns x as initialised and n = 0
@@ -241,7 +277,8 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
 
 size_t ec_state_bytes() { return sizeof(EcState); }
 int ec_nparts(int N) {
-    const int a = (N + EC_ROWS_PER_BLOCK - 1) / EC_ROWS_PER_BLOCK;
+    int a = (N + EC_ROWS_PER_BLOCK - 1) / EC_ROWS_PER_BLOCK;
+    a = a < EC_SPMV_BLOCKS ? a : EC_SPMV_BLOCKS;
     const int b = (N + 255) / 256;
     return a > b ? a : b;
 }

@@ -292,13 +292,16 @@ void bna_server_config_default(bna_server_config *c)
     c->max_batch = 16;
     c->big_round_ms = 15.0;
     c->chunk_ms = 3.0;
+    c->reserve_eighths = 1;
 }
 
 int bna_server_create(const bna_server_config *cfg, bna_server **out)
 {
     if (!cfg || !out) return BNA_ERR_ARG;
     *out = nullptr;
-    if (cfg->workers < 1 || cfg->workers > 8 || cfg->max_batch < 1) return BNA_ERR_ARG;
+    if (cfg->workers < 1 || cfg->workers > 8 || cfg->max_batch < 1 || cfg->reserve_eighths < 0 ||
+        cfg->reserve_eighths > 7)
+        return BNA_ERR_ARG;
     bna_server *srv = new (std::nothrow) bna_server();
     if (!srv) return BNA_ERR_NOMEM;
     srv->cfg = *cfg;
@@ -318,7 +321,7 @@ int bna_server_create(const bna_server_config *cfg, bna_server **out)
     }
     {
         const int g = bna_grid_size(srv->ctxs[0]);
-        if (g >= 16) srv->grid_cap = g - g / 8;
+        if (g >= 16 && cfg->reserve_eighths > 0) srv->grid_cap = g - g / 8 * cfg->reserve_eighths;
     }
     for (bna_ctx *c : srv->ctxs) (void)bna_reserve(c, cfg->reserve_per_worker);  // warm: pool and staging
     for (int w = 0; w < cfg->workers; w++) srv->workers.emplace_back(worker, srv, w);

@@ -110,6 +110,9 @@ typedef struct bna_timings {
     double select_ms;  /* density selection */
     double total_ms;   /* first to last event of the build */
     int64_t corr_launches;
+    double degree_ms;  /* last bna_degree: the degree kernel */
+    double ec_ms;      /* last bna_eigcent: the whole power iteration on the device */
+    int64_t ec_iters;  /* last bna_eigcent: iterations */
 } bna_timings;
 
 /* ----------------------------------------------------------------- context */
@@ -230,6 +233,8 @@ typedef struct bna_server_config {
     double big_round_ms; /* rounds whose estimated tile-kernel time exceeds this run in
                             pieces that yield to small rounds (0: never) */
     double chunk_ms;     /* piece length for big rounds */
+    int reserve_eighths; /* while a big round runs, its tile launches leave this many
+                            eighths of the CUs (one XCD each on MI355X) to other rounds */
 } bna_server_config;
 
 typedef struct bna_request {
@@ -289,6 +294,21 @@ int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *
  * requests on other contexts find CUs free.  No counterpart in the reference,
  * which runs one subject at a time. */
 int bna_set_grid_limit(bna_ctx *ctx, int max_workgroups);
+/* ---------------------------------------------------------------- autotuner
+ * No counterpart in the reference (its cuBLAS tiles are fixed at B = 2048).
+ * The tile kernel walks its upper-triangle tile table in bands of G tile-rows
+ * per tile-column, each XCD over its own contiguous range; G (the traversal
+ * shape) decides which operand panels share an XCD's L2.  It is tuned per
+ * precision and per series-length bucket (padded L <= 192, <= 320, <= 640,
+ * longer): bna_autotune builds a synthetic N-voxel subject of length L with
+ * G in {4, 8, 16, 32}, keeps the fastest tile-kernel time, and rewrites its line
+ * of cache_path (NULL: no file).  The file holds "<precision> <bucket> <G> <ms>"
+ * lines; bna_autotune_load applies one to a context.  Default G = 8. */
+int bna_autotune(bna_ctx *ctx, int precision, int L, int N, const char *cache_path, int *band_rows,
+                 double *best_ms);
+int bna_autotune_load(bna_ctx *ctx, const char *path);
+int bna_autotune_get(bna_ctx *ctx, int precision, int L, int *band_rows);
+int bna_autotune_set(bna_ctx *ctx, int precision, int L, int band_rows);
 /* Workgroups of an uncapped correlation launch: the device's CUs rounded up to a
  * multiple of 8 (-1 for a NULL context).  No counterpart in the reference. */
 int bna_grid_size(bna_ctx *ctx);

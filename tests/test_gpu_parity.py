@@ -199,6 +199,24 @@ def test_degree_and_ec_parity(ctx):
         assert abs(it - oit) <= 1
 
 
+def test_ec_graph_replay_matches_launches(ctx, monkeypatch):
+    """The power iteration replays captured 16-iteration graphs; with graphs
+    off (BNA_GRAPHS=0 at context creation) every kernel is launched directly.
+    Both must give the same bits, call after call."""
+    say("EC graph replay vs direct launches")
+    x, z64, valid = series_and_oracle(3000, 200, 31)
+    g = ctx.build_csr(x, thr=0.45)
+    runs = [ctx.eigcent(g, tol=1e-12) for _ in range(2)]
+    monkeypatch.setenv("BNA_GRAPHS", "0")
+    with bna.Context(0) as c2:
+        direct = c2.eigcent(g, tol=1e-12)
+    assert direct[2] > 32  # several whole batches were replayed
+    for ec, lam, it in runs:
+        assert it == direct[2]
+        assert lam == direct[1]
+        assert np.array_equal(ec, direct[0])
+
+
 def test_ec_known_answer_4x4(ctx):
     # B/src/cuda_EC/cuda_EC/ecGPU.cu:217-233
     g = bna.Graph.__new__(bna.Graph)
