@@ -49,7 +49,7 @@ struct bna_ctx {
     hipMemPool_t pool = nullptr;
     bool own_pool = false;
     // chunked launches (serving): long kernels are issued in pieces of about
-    // chunk_ms, at most two in flight, each preceded by gate() (which may block)
+    // chunk_ms, each preceded by gate() (which may block)
     double chunk_ms = 0.0;
     void (*gate)(void *) = nullptr;
     void *gate_arg = nullptr;
@@ -226,8 +226,10 @@ static int copy_d2h(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
     return BNA_OK;
 }
 
-// Issues launch(first, count) over [0, n) in pieces of `per` units, keeping at
-// most two pieces in flight and calling the context's gate before each one.
+// Issues launch(first, count) over [0, n) in pieces of `per` units.  With a
+// gate, a piece is issued only once the previous one has finished and the gate
+// has let it through, so work that the gate waits for shares the device with
+// at most the remainder of one piece; without one, two pieces stay in flight.
 template <typename F>
 static int run_chunks(bna_ctx *ctx, int64_t n, int64_t per, F launch)
 {
@@ -237,8 +239,9 @@ static int run_chunks(bna_ctx *ctx, int64_t n, int64_t per, F launch)
     }
     int k = 0;
     for (int64_t c0 = 0; c0 < n; c0 += per, k++) {
+        if (ctx->gate && k >= 1) HIPCHK(ctx, hipEventSynchronize(ctx->chev[(k - 1) & 1]));
+        if (!ctx->gate && k >= 2) HIPCHK(ctx, hipEventSynchronize(ctx->chev[k & 1]));
         if (ctx->gate) ctx->gate(ctx->gate_arg);
-        if (k >= 2) HIPCHK(ctx, hipEventSynchronize(ctx->chev[k & 1]));
         HIPCHK(ctx, launch(c0, std::min(per, n - c0)));
         HIPCHK(ctx, hipEventRecord(ctx->chev[k & 1], ctx->stream));
     }

@@ -161,6 +161,9 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         rs[q]->t_start = t0;
         op[q] = rs[q]->opts;
         status[q] = bna_series_create(ctx, rs[q]->x.data(), rs[q]->N, rs[q]->L, rs[q]->layout, 0, &ser[q]);
+        // the host copy is on the device now; release it here, outside the server lock
+        // (returning hundreds of MB to the OS takes milliseconds)
+        std::vector<float>().swap(rs[q]->x);
     }
     const Clock::time_point t_up = Clock::now();
     bna_timings tm{};
@@ -223,7 +226,8 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         } else if (res.status == BNA_OK) {
             res.status = bna_csr_to_host(ctx, &dev[q], &res.csr);
         }
-        res.download_ms = ms_between(tm_done, Clock::now());
+        const Clock::time_point t_dl = Clock::now();
+        res.download_ms = ms_between(tm_done, t_dl);
         if (res.status != BNA_OK) {
             strncpy(res.error, bna_last_error(ctx), sizeof(res.error) - 1);
             bna_csr_free(&res.csr);
@@ -232,6 +236,7 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
         if (ser[q]) bna_series_destroy(ctx, ser[q]);
         ser[q] = nullptr;
         const Clock::time_point t1 = Clock::now();
+        res.release_ms = ms_between(t_dl, t1);
         res.queue_ms = ms_between(r.t_submit, r.t_start);
         res.run_ms = ms_between(r.t_start, t1);
         res.latency_ms = ms_between(r.t_submit, t1);
@@ -280,6 +285,45 @@ static void worker(bna_server *srv, int w)
     }
 }
 
+// One small build per precision, binary and weighted, then degree and
+// centrality, on a worker's context before it takes requests: the first
+// request a context serves then pays for no first-use set-up (kernel lookup,
+// workspace growth, the centrality graph) that its solo time does not include.
+static int warm_ctx(bna_ctx *ctx)
+{
+    const int N = 1000, L = 96;
+    std::vector<float> x((size_t)N * L);
+    (void)bna_synth_factor_model(x.data(), N, L, 8, 0.6, 0x5EA7ULL, 0);
+    bna_series *s = nullptr;
+    int rc = bna_series_create(ctx, x.data(), N, L, 0, 0, &s);
+    if (rc) return rc;
+    std::vector<float> deg(N);
+    std::vector<double> ec(N);
+    for (int prec = BNA_FP32; prec <= BNA_BF16X3 && rc == BNA_OK; prec++)
+        for (int weighted = 0; weighted <= 1 && rc == BNA_OK; weighted++) {
+            bna_opts o;
+            bna_opts_default(&o);
+            o.thr_type = BNA_THR_DENSITY;
+            o.thr = 1.0;
+            o.weighted = weighted;
+            o.precision = prec;
+            bna_csr g{};
+            int st = BNA_OK;
+            rc = bna_build_csr_batch(ctx, &s, &o, 1, &g, &st);
+            if (rc == BNA_OK && prec == BNA_BF16X3) {
+                double lam = 0.0;
+                int it = 0;
+                rc = bna_degree(ctx, &g, deg.data());
+                if (rc == BNA_OK)
+                    rc = bna_eigcent(ctx, &g, 1e-6, 10000, weighted ? BNA_EC_WEIGHTED_REF : BNA_EC_BINARY_REF,
+                                     ec.data(), &lam, &it);
+            }
+            (void)bna_csr_release(ctx, &g);
+        }
+    (void)bna_series_destroy(ctx, s);
+    return rc;
+}
+
 extern "C" {
 
 void bna_server_config_default(bna_server_config *c)
@@ -323,7 +367,15 @@ int bna_server_create(const bna_server_config *cfg, bna_server **out)
         const int g = bna_grid_size(srv->ctxs[0]);
         if (g >= 16 && cfg->reserve_eighths > 0) srv->grid_cap = g - g / 8 * cfg->reserve_eighths;
     }
-    for (bna_ctx *c : srv->ctxs) (void)bna_reserve(c, cfg->reserve_per_worker);  // warm: pool and staging
+    for (bna_ctx *c : srv->ctxs) {
+        (void)bna_reserve(c, cfg->reserve_per_worker);  // pool and staging
+        const int rc = warm_ctx(c);
+        if (rc) {
+            for (bna_ctx *x : srv->ctxs) bna_destroy(x);
+            delete srv;
+            return rc;
+        }
+    }
     for (int w = 0; w < cfg->workers; w++) srv->workers.emplace_back(worker, srv, w);
     *out = srv;
     return BNA_OK;
