@@ -34,7 +34,9 @@ EXPORTS = [
     "bna_server_destroy", "bna_submit", "bna_poll", "bna_wait", "bna_cancel", "bna_server_get_stats",
     "bna_result_free", "bna_request_footprint", "bna_memory_info", "bna_reserve",
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
-    "bna_autotune_load", "bna_autotune_get", "bna_autotune_set",
+    "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
+    "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
+    "bna_group_build_csr",
 ]
 
 
@@ -117,6 +119,13 @@ def lib() -> C.CDLL:
         "bna_autotune_load": (i32, [vp, C.c_char_p]),
         "bna_autotune_get": (i32, [vp, i32, i32, C.POINTER(C.c_int)]),
         "bna_autotune_set": (i32, [vp, i32, i32, i32]),
+        "bna_group_create": (i32, [vp, i32, i32, C.POINTER(vp)]),
+        "bna_group_destroy": (i32, [vp, vp]),
+        "bna_group_add": (i32, [vp, vp, vp, i32]),
+        "bna_group_count": (i32, [vp]),
+        "bna_group_finish": (i32, [vp, vp]),
+        "bna_group_download": (i32, [vp, vp, fp]),
+        "bna_group_build_csr": (i32, [vp, vp, C.POINTER(Opts), C.POINTER(CSR)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -292,6 +301,9 @@ class Context:
     def series(self, x: np.ndarray, layout: int = TIME_MAJOR) -> "Series":
         return Series(self, x, layout)
 
+    def group(self, N: int, fisher: bool = False) -> "Group":
+        return Group(self, N, fisher)
+
     def degree(self, g: Graph) -> np.ndarray:
         h = _host_csr(g)
         deg = np.empty(g.N, np.float32)
@@ -428,6 +440,63 @@ class Series:
             return self.to_host(g)
         finally:
             self.free_device(g)
+
+
+class Group:
+    """Group averaging of correlation matrices (bna_group): add subjects, finish,
+    then threshold the averages into a graph."""
+
+    def __init__(self, ctx: Context, N: int, fisher: bool = False):
+        self.ctx = ctx
+        self.N = N
+        self.M1 = N * (N - 1) // 2
+        self._h = C.c_void_p()
+        ctx._check(ctx._L.bna_group_create(ctx._h, N, 1 if fisher else 0, C.byref(self._h)),
+                   "bna_group_create")
+
+    def close(self):
+        if self._h:
+            self.ctx._L.bna_group_destroy(self.ctx._h, self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, s: "Series", precision: str = "bf16x3"):
+        self.ctx._check(self.ctx._L.bna_group_add(self.ctx._h, self._h, s._h, PRECISIONS[precision]),
+                        "bna_group_add")
+
+    @property
+    def count(self) -> int:
+        return self.ctx._L.bna_group_count(self._h)
+
+    def finish(self):
+        self.ctx._check(self.ctx._L.bna_group_finish(self.ctx._h, self._h), "bna_group_finish")
+
+    def packed(self) -> np.ndarray:
+        out = np.empty(self.M1, np.float32)
+        self.ctx._check(self.ctx._L.bna_group_download(self.ctx._h, self._h, _f32p(out)),
+                        "bna_group_download")
+        return out
+
+    def build(self, thr: float = 0.0, *, density: float | None = None, weighted: bool = True) -> Graph:
+        o = make_opts(thr, density=density, weighted=weighted)
+        g = CSR()
+        self.ctx._check(self.ctx._L.bna_group_build_csr(self.ctx._h, self._h, C.byref(o), C.byref(g)),
+                        "bna_group_build_csr")
+        h = CSR()
+        try:
+            self.ctx._check(self.ctx._L.bna_csr_to_host(self.ctx._h, C.byref(g), C.byref(h)),
+                            "bna_csr_to_host")
+        finally:
+            self.ctx._L.bna_csr_release(self.ctx._h, C.byref(g))
+        try:
+            return Graph(h)
+        finally:
+            self.ctx._L.bna_csr_free(C.byref(h))
 
 
 def build_batch_device(ctx: Context, series: list, opts: list) -> tuple[list, list]:

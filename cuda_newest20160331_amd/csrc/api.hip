@@ -1474,3 +1474,258 @@ int bna_autotune(bna_ctx *ctx, int precision, int L, int N, const char *cache_pa
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ group averaging
+// The reference's avg = y/b path (CorMat.cpp:418-670): per-pair sums of r or
+// Fisher z over the subjects of a directory, averaged, then thresholded like a
+// single subject's matrix.  The sums live in device memory for the whole group.
+struct bna_group {
+    int N = 0, fisher = 0, count = 0;
+    bool finished = false;
+    int64_t M1 = 0;
+    float *acc = nullptr;  // packed upper triangle, row-major, i < j
+};
+
+static inline int64_t packed_pos(int64_t N, int64_t i, int64_t j) { return i * N - (i + 1) * i / 2 + j - i - 1; }
+
+static float key_to_float(uint32_t k)
+{
+    const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
+// k-th largest of the averages by a 4-pass radix select over order-preserving
+// keys.  above: values strictly greater; ties: values equal to it.
+static int group_select(bna_ctx *ctx, bna_group *g, int64_t k, float *r_star, int64_t *above, int64_t *ties)
+{
+    unsigned long long *hist = nullptr;
+    int rc = ws_get_t(ctx, "group_hist", 256, &hist);
+    if (rc) return rc;
+    uint32_t prefix = 0, mask = 0;
+    int64_t need = k, gt = 0, last = 0;
+    std::vector<unsigned long long> h(256);
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        HIPCHK(ctx, hipMemsetAsync(hist, 0, 256 * sizeof(unsigned long long), ctx->stream));
+        HIPCHK(ctx, launch_group_digits(g->acc, g->M1, prefix, mask, shift, hist, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(h.data(), hist, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        int d = 255;
+        for (; d >= 0; d--) {
+            if ((int64_t)h[d] >= need) break;
+            need -= (int64_t)h[d];
+            gt += (int64_t)h[d];
+        }
+        if (d < 0) return fail(ctx, BNA_ERR_DENSITY, "group density: fewer pairs than the edge count");
+        prefix |= (uint32_t)d << shift;
+        mask |= 0xffu << shift;
+        last = (int64_t)h[d];
+    }
+    *r_star = key_to_float(prefix);
+    *above = gt;
+    *ties = last;
+    return BNA_OK;
+}
+
+extern "C" {
+
+int bna_group_create(bna_ctx *ctx, int N, int fisher, bna_group **out)
+{
+    if (!ctx || !out) return fail(ctx, BNA_ERR_ARG, "bna_group_create: NULL argument");
+    *out = nullptr;
+    if (N < 2) return fail(ctx, BNA_ERR_ARG, "bna_group_create: N < 2");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    bna_group *g = new (std::nothrow) bna_group();
+    if (!g) return fail(ctx, BNA_ERR_NOMEM, "bna_group_create: host allocation");
+    g->N = N;
+    g->fisher = fisher ? 1 : 0;
+    g->M1 = (int64_t)N * (N - 1) / 2;
+    int rc = dev_alloc(ctx, (void **)&g->acc, (size_t)g->M1 * 4);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(g->acc, 0, (size_t)g->M1 * 4, ctx->stream);
+    if (e != hipSuccess) {
+        dev_free(ctx, g->acc, (size_t)g->M1 * 4);
+        delete g;
+        return fail(ctx, BNA_ERR_HIP, std::string("bna_group_create: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return BNA_OK;
+}
+
+int bna_group_destroy(bna_ctx *ctx, bna_group *g)
+{
+    if (!g) return BNA_OK;
+    if (!ctx) return fail(nullptr, BNA_ERR_ARG, "bna_group_destroy: ctx is NULL");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, g->acc, (size_t)g->M1 * 4);
+    delete g;
+    return BNA_OK;
+}
+
+int bna_group_add(bna_ctx *ctx, bna_group *g, bna_series *s, int precision)
+{
+    if (!ctx || !g || !s) return fail(ctx, BNA_ERR_ARG, "bna_group_add: NULL argument");
+    if (g->finished) return fail(ctx, BNA_ERR_ARG, "bna_group_add: the group is already averaged");
+    if (s->N != g->N)
+        return fail(ctx, BNA_ERR_ARG, "bna_group_add: subject has " + std::to_string(s->N) + " voxels, group " +
+                                          std::to_string(g->N));
+    if (precision < BNA_FP32 || precision > BNA_BF16X3) return fail(ctx, BNA_ERR_ARG, "bna_group_add: bad precision");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    memset(&ctx->tm, 0, sizeof(ctx->tm));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    int rc;
+    if (s->prec_ready != precision && (rc = series_normalize(ctx, s, precision))) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    SubjDesc h{};
+    h.a0 = s->panel;
+    h.N = s->N;
+    h.N_pad = s->N_pad;
+    h.L_pad = s->L_pad;
+    h.nb = s->N_pad / TILE;
+    h.dense = g->acc;
+    h.fisher = g->fisher;
+    SubjDesc *dsd = nullptr;
+    if ((rc = ws_get_t(ctx, "group_subjdesc", 1, &dsd))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(dsd, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    TileItem *tiles = nullptr;
+    int64_t nt = 0;
+    const int64_t kstride = (int64_t)s->N_pad * panel_unit(precision);
+    if ((rc = get_tile_table(ctx, h.nb, s->panel, kstride, s->L_pad / 16, ctx->band[precision][l_bucket(s->L_pad)],
+                             &tiles, &nt)))
+        return rc;
+    unsigned int *claim = nullptr;
+    if ((rc = ws_get_t(ctx, "tile_claim", 8, &claim))) return rc;
+    const int grid = ctx->grid_limit > 0 ? std::min(ctx->grid, ctx->grid_limit) : ctx->grid;
+    HIPCHK(ctx, launch_corr(precision, 2, dsd, tiles, nt, claim, nullptr, grid, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->tm.znorm_ms = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->tm.corr_ms = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]) == hipSuccess) ctx->tm.total_ms = ms;
+    ctx->tm.corr_launches = 1;
+    g->count++;
+    return BNA_OK;
+}
+
+int bna_group_finish(bna_ctx *ctx, bna_group *g)
+{
+    if (!ctx || !g) return fail(ctx, BNA_ERR_ARG, "bna_group_finish: NULL argument");
+    if (g->finished) return BNA_OK;
+    if (g->count < 1) return fail(ctx, BNA_ERR_ARG, "bna_group_finish: no subject was added");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_group_finish(g->acc, g->M1, g->count, g->fisher, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    g->finished = true;
+    return BNA_OK;
+}
+
+int bna_group_count(const bna_group *g) { return g ? g->count : 0; }
+
+int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed)
+{
+    if (!ctx || !g || !packed) return fail(ctx, BNA_ERR_ARG, "bna_group_download: NULL argument");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return copy_d2h(ctx, packed, g->acc, (size_t)g->M1 * 4);
+}
+
+int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *out)
+{
+    if (!ctx || !g || !o || !out) return fail(ctx, BNA_ERR_ARG, "bna_group_build_csr: NULL argument");
+    memset(out, 0, sizeof(*out));
+    if (!g->finished) return fail(ctx, BNA_ERR_ARG, "bna_group_build_csr: call bna_group_finish first");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int N = g->N;
+    const bool dens = o->thr_type == BNA_THR_DENSITY;
+    if (!dens && o->thr_type != BNA_THR_R) return fail(ctx, BNA_ERR_ARG, "bna_group_build_csr: bad threshold type");
+    float thr = (float)o->thr;
+    int64_t p_cut = 0;
+    int rc;
+    if (dens) {
+        const int64_t k = bna_density_to_k(N, (float)o->thr);
+        if (k < 1) return fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
+        if (k > g->M1) return fail(ctx, BNA_ERR_DENSITY, "density: more edges than pairs");
+        int64_t above = 0, ties = 0;
+        if ((rc = group_select(ctx, g, k, &thr, &above, &ties))) return rc;
+        const int64_t m = k - above;  // ties kept, the first m in row-major order (CorMat.cpp:784-812)
+        p_cut = g->M1;
+        if (m < ties) {
+            int64_t *tr = nullptr;
+            if ((rc = ws_get_t(ctx, "group_ties", (size_t)N, &tr))) return rc;
+            HIPCHK(ctx, launch_group_ties(g->acc, N, thr, tr, ctx->stream));
+            std::vector<int64_t> rows(N);
+            HIPCHK(ctx, hipMemcpyAsync(rows.data(), tr, sizeof(int64_t) * N, hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            int64_t before = 0;
+            int i = 0;
+            for (; i < N && before + rows[i] < m; i++) before += rows[i];
+            if (i >= N - 1) return fail(ctx, BNA_ERR_DENSITY, "group density: tie counts are inconsistent");
+            std::vector<float> row((size_t)(N - 1 - i));
+            HIPCHK(ctx, hipMemcpyAsync(row.data(), g->acc + packed_pos(N, i, i + 1), sizeof(float) * row.size(),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            int64_t want = m - before, c = 0;
+            size_t q = 0;
+            for (; q < row.size(); q++)
+                if (row[q] == thr && ++c == want) break;
+            if (q == row.size()) return fail(ctx, BNA_ERR_DENSITY, "group density: tie row is inconsistent");
+            p_cut = packed_pos(N, i, i + 1 + (int64_t)q);
+        }
+    }
+    uint8_t *keep = nullptr;
+    unsigned int *flag = nullptr;
+    int64_t *cnt = nullptr;
+    if ((rc = ws_get_t(ctx, "group_keep", (size_t)N * N, &keep))) return rc;
+    if ((rc = ws_get_t(ctx, "group_flag", 1, &flag))) return rc;
+    if ((rc = ws_get_t(ctx, "group_cnt", (size_t)N, &cnt))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(flag, 0, sizeof(unsigned int), ctx->stream));
+    HIPCHK(ctx, launch_group_keep(g->acc, N, thr, dens ? 1 : 0, p_cut, keep, flag, ctx->stream));
+    HIPCHK(ctx, launch_group_rowcount(keep, N, cnt, ctx->stream));
+    std::vector<int64_t> c(N);
+    unsigned int bad = 0;
+    HIPCHK(ctx, hipMemcpyAsync(c.data(), cnt, sizeof(int64_t) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (bad) return fail(ctx, BNA_ERR_FORMAT, "FormFullGraph: illegal correlation coefficient (an average above 1)");
+    std::vector<int32_t> R((size_t)N + 1);
+    int64_t nnz = 0;
+    R[0] = 0;
+    for (int i = 0; i < N; i++) {
+        nnz += c[i];
+        if (nnz > INT32_MAX) return fail(ctx, BNA_ERR_OVERFLOW, "group graph: more than 2^31 - 1 entries");
+        R[i + 1] = (int32_t)nnz;
+    }
+    if (dens && nnz != 2 * bna_density_to_k(N, (float)o->thr))
+        return fail(ctx, BNA_ERR_DENSITY, "group density: selected edge count differs from k");
+    bna_csr r{};
+    r.N = N;
+    r.nnz = nnz;
+    r.on_device = 1;
+    if ((rc = out_alloc(ctx, (void **)&r.R, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
+    if ((rc = out_alloc(ctx, (void **)&r.C, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)))) {
+        bna_csr_release(ctx, &r);
+        return rc;
+    }
+    if (o->weighted && (rc = out_alloc(ctx, (void **)&r.V, sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)))) {
+        bna_csr_release(ctx, &r);
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(r.R, R.data(), sizeof(int32_t) * R.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_group_fill(keep, g->acc, N, r.R, r.C, r.V, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        bna_csr_release(ctx, &r);
+        return fail(ctx, BNA_ERR_HIP, std::string("bna_group_build_csr: ") + hipGetErrorString(e));
+    }
+    r.thr_used = thr;
+    r.density_pct = 100.0 * (double)nnz / (double)g->M1 / 2.0;
+    *out = r;
+    return BNA_OK;
+}
+
+}  // extern "C"

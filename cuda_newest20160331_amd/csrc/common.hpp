@@ -56,6 +56,8 @@ struct SubjDesc {
     unsigned long long *cand_count;
     int64_t cand_cap;
     unsigned long long *hist; // histogram pass: HIST_BINS counters
+    float *dense;        // accumulation pass: packed upper triangle, M1 floats
+    int32_t fisher;      // accumulation pass: add Fisher z instead of r
 };
 
 constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
@@ -99,12 +101,34 @@ hipError_t launch_zmat(const float *xs, int64_t ld, int N, int L, const double *
 
 // Correlation + threshold + compaction over a tile table (kernel b + first half of c),
 // as one persistent launch of at most grid_max workgroups.  prec: 0 FP32, 1 BF16,
-// 2 BF16X3.  hist_mode: 1 = histogram epilogue.  claim: 8 device counters (reset
+// 2 BF16X3.  hist_mode: 1 = histogram epilogue, 2 = add every pair's r (or
+// Fisher z) to the subject's packed dense sums.  claim: 8 device counters (reset
 // by the launch).  spill: corr_spill_records(prec, grid_max) 8-byte records.
 // grid_max: a positive multiple of 8.
 hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const TileItem *items,
                        int64_t n_items, unsigned int *claim, uint64_t *spill, int grid_max, hipStream_t st);
 int64_t corr_spill_records(int prec, int grid_max);
+
+// Group averaging over a packed upper triangle of M1 floats (k_group.hip).
+// Divide every sum by the subject count S in float and, for Fisher sums, map
+// back with the inverse transform.
+hipError_t launch_group_finish(float *acc, int64_t M1, int S, int fisher, hipStream_t st);
+// 256-bin histogram of one radix digit of the order-preserving keys of the
+// values whose higher digits equal `prefix` under `mask` (added to hist).
+hipError_t launch_group_digits(const float *acc, int64_t M1, uint32_t prefix, uint32_t mask, int shift,
+                               unsigned long long *hist, hipStream_t st);
+// Per packed row i: how many of its values equal `val` (ties of a density threshold).
+hipError_t launch_group_ties(const float *acc, int N, float val, int64_t *tie_rows, hipStream_t st);
+// Full N x N keep matrix (bytes, both triangles, zero diagonal): density mode
+// (dens = 1) keeps v > thr and the ties v == thr at packed index <= p_cut;
+// r mode keeps v > thr.  flag[0] gets 1 if any value exceeds 1.
+hipError_t launch_group_keep(const float *acc, int N, float thr, int dens, int64_t p_cut, uint8_t *keep,
+                             unsigned int *flag, hipStream_t st);
+// Row lengths of the keep matrix, then the CSR rows (columns ascending, values
+// from the packed averages when V is not null).
+hipError_t launch_group_rowcount(const uint8_t *keep, int N, int64_t *cnt, hipStream_t st);
+hipError_t launch_group_fill(const uint8_t *keep, const float *acc, int N, const int32_t *R, int32_t *C, float *V,
+                             hipStream_t st);
 
 // FP64 recheck of candidates.
 // Slots at or past entry_cap (overflowed lists) are skipped.

@@ -180,6 +180,35 @@ __device__ __forceinline__ bool epi_in(const EpiCtx &c, int lr, int lc) {
     return (c.gi0 + lr < c.N) && (c.gj0 + lc < c.N) && (!c.diag || lr < lc);
 }
 
+// Fisher z of one r as the group average takes it (CorMat.cpp:36-42): r == 1 is
+// moved to the float below 1 - 1e-6, the quotient and its log are float, the
+// half is taken in double and the result stored as float.
+__device__ __forceinline__ float fisher_z(float r) {
+    if (r == 1.0f) r = (float)((double)r - 1e-6);
+    return (float)(0.5 * (double)logf((1.0f + r) / (1.0f - r)));
+}
+
+// Accumulation pass: every pair of the block whose r lies in [-1, 1] (a NaN
+// from an invalid voxel does not) adds r, or its Fisher z, to its slot of the
+// packed upper triangle, row-major with i < j (CorMat.cpp:851-866).  Lanes 0-31
+// and 32-63 touch two runs of 32 neighbouring slots.  Each pair belongs to one
+// tile, so the read-modify-write needs no atomic.
+template <int RB, int CB>
+__device__ __forceinline__ void epi_accum(const f32x16_t &a, const EpiCtx &c, float *dense, int fisher) {
+    const int lc = c.wc0 + CB * 32 + c.lrow;
+    const int64_t j = c.gj0 + lc;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
+        const float v = a[e];
+        if (epi_in(c, lr, lc) && v >= -1.0f && v <= 1.0f) {
+            const int64_t i = c.gi0 + lr;
+            float *slot = dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
+            *slot += fisher ? fisher_z(v) : v;
+        }
+    }
+}
+
 __device__ __forceinline__ void epi_hist(const f32x16_t &a, int RB, int CB, const EpiCtx &c, uint32_t *hist) {
     const int lc = c.wc0 + CB * 32 + c.lrow;
 #pragma unroll
@@ -372,7 +401,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         ctl[0] = t0;
         ctl[1] = t0 >= 0 ? claim_tile(claim, n_items, per_xcd, xcd) : -1;
     }
-    if constexpr (HIST) {
+    if constexpr (HIST == 1) {
         for (int k = tid; k < HIST_BINS; k += CT_THREADS) ((uint32_t *)epi)[k] = 0;
     }
     __syncthreads();
@@ -544,8 +573,10 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         ec.diag = (ic.I == ic.J);
         ec.keep = sd.keep_thr;
 
-        if constexpr (HIST) {
+        if constexpr (HIST == 1) {
             BNA_FOR_BLOCKS(epi_hist, ec, (uint32_t *)epi);
+        } else if constexpr (HIST == 2) {
+            BNA_FOR_BLOCKS_T(epi_accum, ec, sd.dense, sd.fisher);
         } else {
             uint32_t *rowbits = (uint32_t *)epi;         // [256][8] tile-row bitmaps
             uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
@@ -616,7 +647,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         if (etid == 0) ctl[2] = nn_claim;
         __syncthreads();
         const int64_t nn = uni64(ctl[2]);
-        if constexpr (HIST) {
+        if constexpr (HIST == 1) {
             if (nxt < 0 || in.subj != ic.subj) {
                 hist_flush((uint32_t *)epi, sd.hist, etid);
                 __syncthreads();
@@ -666,13 +697,17 @@ static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, in
 hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const TileItem *items,
                        int64_t n_items, unsigned int *claim, uint64_t *spill, int grid_max, hipStream_t st)
 {
-    switch (prec * 2 + (hist_mode ? 1 : 0)) {
+    if (hist_mode < 0 || hist_mode > 2) return hipErrorInvalidValue;
+    switch (prec * 3 + hist_mode) {
     case 0: return launch_one<0, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
     case 1: return launch_one<0, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 2: return launch_one<1, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 3: return launch_one<1, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 4: return launch_one<2, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 5: return launch_one<2, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 2: return launch_one<0, 2>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 3: return launch_one<1, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 4: return launch_one<1, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 5: return launch_one<1, 2>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 6: return launch_one<2, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 7: return launch_one<2, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 8: return launch_one<2, 2>(subj_dev, items, n_items, claim, spill, grid_max, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -1,0 +1,217 @@
+// k_group.hip -- group averaging of correlation matrices and the dense
+// threshold that turns the average into a CSR graph.
+//
+// The reference's group mode (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:418-670)
+// adds every subject's r (or its Fisher z, :36-42) into a packed upper
+// triangle of M1 floats (post_block, :824-887), divides by the subject count and
+// maps Fisher sums back (:548-558), then thresholds the averages (FormFullGraph
+// :683-703 or the exact-k FormFullGraph_s :743-822) and lists each row's
+// neighbours in column order (FormCSRGraph :889-918).  Here the sums stay in
+// HBM (the tile kernel's accumulation pass adds each subject), and the
+// threshold writes a full N x N keep matrix of bytes (2 GB at N = 45000), so
+// that the symmetric CSR is read row by row with coalesced loads instead of
+// gathering column entries from the packed triangle.
+#include "common.hpp"
+
+namespace bna {
+
+__device__ __forceinline__ int64_t packed_at(int64_t N, int64_t i, int64_t j)
+{
+    return i * N - (i + 1) * i / 2 + j - i - 1;
+}
+
+// Order-preserving 32-bit key of a float (larger value, larger key); -0 is
+// taken as +0 so that the two compare equal, as floats do.
+__device__ __forceinline__ uint32_t fkey(float v)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, v == 0.0f ? 0.0f : v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// CorMat.cpp:548-558: Cormat[i] /= FileNumber in float, then for Fisher sums
+// inv_fishertrans (:44-50): r = exp(2 z), (r - 1) / (r + 1), all in float.
+__global__ __launch_bounds__(256) void group_finish_kernel(float *__restrict__ acc, int64_t M1, float S, int fisher)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < M1; t += stride) {
+        float v = acc[t] / S;
+        if (fisher) {
+            const float e = expf(2.0f * v);
+            v = (e - 1.0f) / (e + 1.0f);
+        }
+        acc[t] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void group_digits_kernel(const float *__restrict__ acc, int64_t M1, uint32_t prefix,
+                                                           uint32_t mask, int shift,
+                                                           unsigned long long *__restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < M1; t += stride) {
+        const uint32_t k = fkey(acc[t]);
+        if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// One wave per packed row.
+__global__ __launch_bounds__(256) void group_ties_kernel(const float *__restrict__ acc, int N, float val,
+                                                         int64_t *__restrict__ tie_rows)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N) return;
+    int64_t c = 0;
+    if (i + 1 < N) {
+        const float *row = acc + packed_at(N, i, i + 1);  // columns i + 1 .. N - 1
+        for (int64_t j = lane; j < N - 1 - i; j += 64) c += row[j] == val;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0) tie_rows[i] = c;
+}
+
+// One 64 x 64 block (bi <= bj) of the upper triangle per workgroup: decide
+// each pair, write the block to its place in the keep matrix and its
+// transpose through LDS, so both writes are row runs.
+__global__ __launch_bounds__(256) void group_keep_kernel(const float *__restrict__ acc, int N, float thr, int dens,
+                                                         int64_t p_cut, uint8_t *__restrict__ keep,
+                                                         unsigned int *__restrict__ flag)
+{
+    const int bj = blockIdx.x, bi = blockIdx.y;
+    if (bi > bj) return;
+    __shared__ uint8_t tile[64][65];
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)bi * 64, j0 = (int64_t)bj * 64;
+    {
+        const int r = t >> 2, cs = (t & 3) * 16;
+        const int64_t i = i0 + r;
+        bool bad = false;
+#pragma unroll 4
+        for (int c = 0; c < 16; c++) {
+            const int64_t j = j0 + cs + c;
+            uint8_t k = 0;
+            if (i < N && j < N && i < j) {
+                const int64_t p = packed_at(N, i, j);
+                const float v = acc[p];
+                k = dens ? (uint8_t)(v > thr || (v == thr && p <= p_cut)) : (uint8_t)(v > thr);
+                bad |= v > 1.0f;
+            }
+            tile[r][cs + c] = k;
+            if (i < N && j < N && i <= j) keep[i * N + j] = k;  // the diagonal gets 0
+        }
+        if (bad && !dens) atomicOr(flag, 1u);
+    }
+    __syncthreads();
+    {
+        const int jj = t >> 2, is = (t & 3) * 16;
+        const int64_t j = j0 + jj;
+#pragma unroll 4
+        for (int c = 0; c < 16; c++) {
+            const int64_t i = i0 + is + c;
+            if (i < N && j < N && i < j) keep[j * N + i] = tile[is + c][jj];
+        }
+    }
+}
+
+// One wave per row of the keep matrix.
+__global__ __launch_bounds__(256) void group_rowcount_kernel(const uint8_t *__restrict__ keep, int N,
+                                                             int64_t *__restrict__ cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N) return;
+    const uint8_t *row = keep + i * N;
+    int64_t c = 0;
+    for (int64_t j = lane; j < N; j += 64) c += row[j];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0) cnt[i] = c;
+}
+
+__global__ __launch_bounds__(256) void group_fill_kernel(const uint8_t *__restrict__ keep,
+                                                         const float *__restrict__ acc, int N,
+                                                         const int32_t *__restrict__ R, int32_t *__restrict__ C,
+                                                         float *__restrict__ V)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N) return;
+    const uint8_t *row = keep + i * N;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int64_t base = R[i];
+    for (int64_t j0 = 0; j0 < N; j0 += 64) {
+        const int64_t j = j0 + lane;
+        const bool k = j < N && row[j] != 0;
+        const uint64_t m = __ballot(k);
+        if (k) {
+            const int64_t pos = base + __popcll(m & lt);
+            C[pos] = (int32_t)j;
+            if (V) V[pos] = acc[i < j ? packed_at(N, i, j) : packed_at(N, j, i)];
+        }
+        base += __popcll(m);
+    }
+}
+
+static unsigned stream_blocks(int64_t n)
+{
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+hipError_t launch_group_finish(float *acc, int64_t M1, int S, int fisher, hipStream_t st)
+{
+    if (M1 <= 0) return hipSuccess;
+    if (S < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_finish_kernel, dim3(stream_blocks(M1)), dim3(256), 0, st, acc, M1, (float)S, fisher);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_digits(const float *acc, int64_t M1, uint32_t prefix, uint32_t mask, int shift,
+                               unsigned long long *hist, hipStream_t st)
+{
+    if (M1 <= 0) return hipSuccess;
+    if (shift < 0 || shift > 24) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_digits_kernel, dim3(stream_blocks(M1)), dim3(256), 0, st, acc, M1, prefix, mask, shift,
+                       hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_ties(const float *acc, int N, float val, int64_t *tie_rows, hipStream_t st)
+{
+    if (N < 1) return hipSuccess;
+    hipLaunchKernelGGL(group_ties_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, acc, N, val, tie_rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_keep(const float *acc, int N, float thr, int dens, int64_t p_cut, uint8_t *keep,
+                             unsigned int *flag, hipStream_t st)
+{
+    if (N < 1) return hipSuccess;
+    const unsigned nbk = (unsigned)((N + 63) / 64);
+    if (nbk > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_keep_kernel, dim3(nbk, nbk), dim3(256), 0, st, acc, N, thr, dens, p_cut, keep, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_rowcount(const uint8_t *keep, int N, int64_t *cnt, hipStream_t st)
+{
+    if (N < 1) return hipSuccess;
+    hipLaunchKernelGGL(group_rowcount_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, N, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_fill(const uint8_t *keep, const float *acc, int N, const int32_t *R, int32_t *C, float *V,
+                             hipStream_t st)
+{
+    if (N < 1) return hipSuccess;
+    hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, acc, N, R, C, V);
+    return hipGetLastError();
+}
+
+}  // namespace bna

@@ -167,6 +167,36 @@ int bna_csr_to_host(bna_ctx *ctx, const bna_csr *dev, bna_csr *out);
  * voxel-major), mean and norm per voxel (each may be NULL). */
 int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *mu, double *sd);
 
+/* ------------------------------------------------------- group averaging */
+/* The reference's group mode (avg = y or b, CorMat.cpp:418-670): every
+ * subject's r, or its Fisher z (fishertrans, CorMat.cpp:36-42), is added to a
+ * per-pair sum if it lies in [-1, 1] (post_block, :824-887; a NaN from an
+ * invalid voxel adds nothing); the sums are divided by the subject count and
+ * Fisher sums mapped back (inv_fishertrans, :44-50, :548-558); the averages are
+ * thresholded like one subject's matrix (FormFullGraph strict r > thr, :683-703,
+ * or the exact-k FormFullGraph_s with its tie rule, :743-822) and listed as CSR
+ * rows in column order with V = the average (FormCSRGraph, :889-918).  The sums
+ * are a packed upper triangle of M1 = N(N-1)/2 floats, row-major with i < j
+ * (4 GB at N = 45000), held in device memory. */
+typedef struct bna_group bna_group;
+int bna_group_create(bna_ctx *ctx, int N, int fisher, bna_group **out);
+int bna_group_destroy(bna_ctx *ctx, bna_group *g);
+/* Adds one subject (the correlation pass of CorMat_gpu, CorMat.cpp:489, and
+ * post_block, :501-504).  The series is normalised for `precision` if needed;
+ * every subject must have the group's N, L may differ. */
+int bna_group_add(bna_ctx *ctx, bna_group *g, bna_series *s, int precision);
+/* Subjects added so far. */
+int bna_group_count(const bna_group *g);
+/* Averages the sums (CorMat.cpp:548-558); no subject can be added afterwards. */
+int bna_group_finish(bna_ctx *ctx, bna_group *g);
+/* The packed averages (or sums, before bna_group_finish): M1 floats, the
+ * payload of the .cormat file (CorMat.cpp:570-579). */
+int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed);
+/* Thresholds the averages: opts->thr_type, thr and weighted are used.  out gets
+ * a device CSR; thr_used is the r threshold or, for a density, the k-th
+ * largest average. */
+int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *opts, bna_csr *out);
+
 /* ------------------------------------------------------------ graph maps */
 /* Degree.  Weighted: float sum of V along each row in CSR order
  * (W/Degree/Degree/Degree.cpp:83-87); binary or V == NULL: row length as float

@@ -35,6 +35,14 @@
  *                       order); binary B/Degree/Degree/Degree.cpp:84-91.
  *   orc_eigcent         W/CUEC/weighted-net.cu:129-190 (weighted, output A*y) and
  *                       B/CUEC/CUEC/CUEC_unweighted.cu:156-195 (binary, output y).
+ *   orc_fisher,         fishertrans / inv_fishertrans (CorMat.cpp:36-50) in the
+ *   orc_inv_fisher      reference's float arithmetic (float quotient, logf/expf).
+ *   orc_group_add       post_block (CorMat.cpp:824-887): each pair's r, as float,
+ *                       is added (or its Fisher z) when it lies in [-1, 1].
+ *   orc_group_finish    CorMat.cpp:548-558: divide by the subject count in float,
+ *                       then inv_fishertrans for Fisher sums.
+ *   orc_threshold_packed FormFullGraph over packed values (strict v > thr) and
+ *                       FormCSRGraph, for the averaged matrix.
  *
  * Pairs involving a voxel whose z has a non-finite entry (a constant series gives
  * 0/0) are never edges and take no part in density selection.
@@ -441,6 +449,58 @@ orc_graph *orc_build_density(const double *z64, const unsigned char *valid, int 
     orc_all_pairs(z64, valid, N, L, nthreads, packed);
     orc_graph *g = orc_select_packed(packed, N, k, literal);
     free(packed);
+    return g;
+}
+
+/* fishertrans (CorMat.cpp:36-42): r == 1 is moved below 1 by 1e-6 (in double,
+ * stored as float), the quotient is float, log of a float is float
+ * (std::log(float)), the half is applied in double and the result stored as
+ * float. */
+float orc_fisher(float r)
+{
+    if (r == 1.0f) r = (float)((double)r - 1e-6);
+    return (float)(0.5 * (double)logf((1.0f + r) / (1.0f - r)));
+}
+
+/* inv_fishertrans (CorMat.cpp:44-50): exp of a float is float, the rest float. */
+float orc_inv_fisher(float z)
+{
+    float e = expf(2.0f * z);
+    return (e - 1.0f) / (e + 1.0f);
+}
+
+/* post_block (CorMat.cpp:824-887) for one subject whose pair values are
+ * packed_r (double, NaN for invalid pairs), taken as float the way the
+ * reference's float matrix holds them. */
+void orc_group_add(float *acc, const double *packed_r, int64_t M1, int fisher)
+{
+    for (int64_t t = 0; t < M1; t++) {
+        float r = (float)packed_r[t];
+        if (r >= -1.0f && r <= 1.0f) acc[t] += fisher ? orc_fisher(r) : r;
+    }
+}
+
+/* CorMat.cpp:548-558. */
+void orc_group_finish(float *acc, int64_t M1, int S, int fisher)
+{
+    for (int64_t t = 0; t < M1; t++) {
+        acc[t] /= (float)S;
+        if (fisher) acc[t] = orc_inv_fisher(acc[t]);
+    }
+}
+
+/* FormFullGraph (CorMat.cpp:683-703) over packed values: pair (i, j) is an edge
+ * iff v > thr; then FormCSRGraph with V = v. */
+orc_graph *orc_threshold_packed(const double *packed, int N, double thr)
+{
+    edge_vec e = {0};
+    int64_t idx = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = i + 1; j < N; j++, idx++)
+            if (packed[idx] > thr) ev_push(&e, i, j, packed[idx]);
+    orc_graph *g = graph_from_edges(N, &e);
+    g->r_star = thr;
+    ev_free(&e);
     return g;
 }
 

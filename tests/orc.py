@@ -44,6 +44,11 @@ def olib() -> C.CDLL:
         "orc_degree": (None, [vp, vp, i32, vp]),
         "orc_eigcent": (dbl, [vp, vp, vp, i32, dbl, i32, i32, vp, C.POINTER(C.c_int)]),
         "orc_ref_cpu_rows": (i64, [vp, i32, i32, i32, i32, C.c_float, i32, C.POINTER(C.c_double)]),
+        "orc_fisher": (C.c_float, [C.c_float]),
+        "orc_inv_fisher": (C.c_float, [C.c_float]),
+        "orc_group_add": (None, [vp, vp, i64, i32]),
+        "orc_group_finish": (None, [vp, i64, i32, i32]),
+        "orc_threshold_packed": (vp, [vp, i32, dbl]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -163,3 +168,29 @@ def ref_corr_rows(z32, i0, i1):
     out = np.empty(n, np.float32)
     olib().orc_ref_corr_rows(_p(np.ascontiguousarray(z32, np.float32)), N, L, i0, i1, _p(out))
     return out
+
+
+def fisher(r: float) -> float:
+    return float(olib().orc_fisher(r))
+
+
+def inv_fisher(z: float) -> float:
+    return float(olib().orc_inv_fisher(z))
+
+
+def group_average(packed_list, N: int, fisher: bool) -> np.ndarray:
+    """post_block over each subject's packed r (float64, NaN for invalid pairs),
+    then the average (float32 packed)."""
+    M1 = N * (N - 1) // 2
+    acc = np.zeros(M1, np.float32)
+    for p in packed_list:
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        assert p.shape == (M1,)
+        olib().orc_group_add(acc.ctypes.data, p.ctypes.data, M1, 1 if fisher else 0)
+    olib().orc_group_finish(acc.ctypes.data, M1, len(packed_list), 1 if fisher else 0)
+    return acc
+
+
+def threshold_packed(packed: np.ndarray, N: int, thr: float) -> OGraph:
+    packed = np.ascontiguousarray(packed, dtype=np.float64)
+    return OGraph(olib().orc_threshold_packed(packed.ctypes.data, N, thr)).load(N)

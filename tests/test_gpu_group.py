@@ -1,0 +1,195 @@
+"""GPU parity of group averaging (avg = y / b) against the oracle.
+
+The oracle (test_group_oracle.py pins it) adds each subject's r64, taken as
+float, or its Fisher z when it lies in [-1, 1], divides by the subject count in
+float and maps Fisher sums back (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:36-50,
+548-558, 824-887); the average is thresholded by FormFullGraph / FormFullGraph_s
+(:683-822) and listed by FormCSRGraph (:889-918).
+
+The device adds the tile kernel's r, so each sum differs from the oracle's by
+the MFMA error of its terms.  Bars: averages within AVG_TOL (plain) or
+AVG_TOL_Z (Fisher, whose slope 1 / (1 - r^2) magnifies the error); edge sets
+identical except pairs whose oracle average lies within that tolerance of the
+threshold, which are counted and must be few; values within the tolerance.
+Where the averages are exact (pairs with an invalid voxel in every subject
+average to exactly 0) the edge set is compared bit for bit, tie rule included.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import cuda_newest20160331_amd as bna
+import orc
+
+pytestmark = pytest.mark.gpu
+
+AVG_TOL = 4e-6
+AVG_TOL_Z = 2e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = bna.Context(0)
+    yield c
+    c.close()
+
+
+def subjects(N, Ls, seed, dead=None):
+    xs = []
+    for q, L in enumerate(Ls):
+        x = bna.synth_factor_model(N, L, 8, 0.7, seed + q)
+        if dead is not None and dead.get(q) is not None:
+            x[:, dead[q]] = 3.0  # constant series: invalid voxels, NaN r
+        xs.append(x)
+    return xs
+
+
+def oracle_packed(x):
+    z64, _, _, _, valid = orc.znorm(x, 0)
+    return orc.all_pairs(z64, valid)
+
+
+def gpu_group(ctx, xs, N, fisher, prec):
+    g = ctx.group(N, fisher)
+    for x in xs:
+        with ctx.series(x) as s:
+            g.add(s, prec)
+    return g
+
+
+def edges(G):
+    rows = np.repeat(np.arange(G.N), np.diff(G.R))
+    keep = rows < G.C
+    return set(zip(rows[keep].tolist(), G.C[keep].tolist()))
+
+
+def check_graph(G, og, avg_ref, N, thr, tol):
+    """Edge sets equal except pairs within tol of thr (by the oracle's average)."""
+    assert np.all(np.diff(G.R) >= 0) and G.R[-1] == G.nnz
+    e_gpu, e_orc = edges(G), edges(og)
+    diff = e_gpu ^ e_orc
+    for (i, j) in diff:
+        p = i * N - (i + 1) * i // 2 + j - i - 1
+        assert abs(float(avg_ref[p]) - thr) <= tol, f"pair ({i},{j}) avg {avg_ref[p]} thr {thr}"
+    return len(diff)
+
+
+@pytest.mark.parametrize("fisher", [False, True])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+def test_group_average_and_threshold(ctx, fisher, prec):
+    N, Ls = 700, [120, 200, 333]
+    say(f"group N={N} L={Ls} fisher={fisher} prec={prec}")
+    xs = subjects(N, Ls, 0xB17A7000, dead={1: [5, 300]})
+    ref = orc.group_average([oracle_packed(x) for x in xs], N, fisher)
+    g = gpu_group(ctx, xs, N, fisher, prec)
+    try:
+        assert g.count == 3
+        g.finish()
+        avg = g.packed()
+        tol = AVG_TOL_Z if fisher else AVG_TOL
+        err = np.abs(avg.astype(np.float64) - ref)
+        say(f"  max |avg - oracle| = {err.max():.3g}")
+        assert err.max() <= tol
+        # voxel 5 is invalid in subject 1 only: its pairs average the other two
+        assert np.all(np.isfinite(avg))
+        thr = 0.25
+        G = g.build(thr)
+        og = orc.threshold_packed(ref, N, thr)
+        nd = check_graph(G, og, ref, N, thr, tol)
+        say(f"  r > {thr}: nnz {G.nnz} oracle {og.nnz}, boundary differences {nd}")
+        assert nd <= 4
+        # values: the averages of the kept pairs
+        rows = np.repeat(np.arange(N), np.diff(G.R))
+        lo, hi = np.minimum(rows, G.C), np.maximum(rows, G.C)
+        p = lo * N - (lo + 1) * lo // 2 + hi - lo - 1
+        assert np.array_equal(G.V, avg[p])
+        assert G.thr_used == pytest.approx(thr)
+        # density: exactly k edges
+        spa = 3.0
+        k = orc.density_k(N, spa)
+        Gd = g.build(density=spa, weighted=False)
+        assert Gd.nnz == 2 * k and Gd.V is None
+        okd = orc.select_packed(ref.astype(np.float64), N, k, literal=False)
+        assert abs(Gd.thr_used - okd.r_star) <= tol
+        nd = check_graph(Gd, okd, ref, N, okd.r_star, 2 * tol)
+        say(f"  density {spa}%: k={k} r*={Gd.thr_used:.7g} oracle {okd.r_star:.7g}, differences {nd}")
+        assert nd <= 8
+    finally:
+        g.close()
+
+
+def test_group_ties_exact(ctx):
+    # Half the voxels are constant in every subject, so their pairs sum nothing
+    # and average exactly 0; the rest share one strong factor (r ~ 0.8).  A 40 %
+    # density takes every valid pair and the first m zero pairs in row-major
+    # order (FormFullGraph_s's tie rule), which the oracle fixes exactly.
+    N, L, S = 600, 150, 2
+    rng = np.random.default_rng(11)
+    dead = rng.choice(N, N // 2, replace=False)
+    xs = []
+    for q in range(S):
+        f = rng.normal(size=(L, 1)).astype(np.float32)
+        x = (f + 0.5 * rng.normal(size=(L, N))).astype(np.float32)
+        x[:, dead] = 2.0
+        xs.append(np.ascontiguousarray(x))
+    ref = orc.group_average([oracle_packed(x) for x in xs], N, False)
+    spa = 40.0
+    k = orc.density_k(N, spa)
+    g = gpu_group(ctx, xs, N, False, "bf16x3")
+    try:
+        g.finish()
+        avg = g.packed()
+        assert np.array_equal(avg == 0, ref == 0)
+        okd = orc.select_packed(ref.astype(np.float64), N, k, literal=False)
+        assert okd.r_star == 0.0 and okd.n_ties > 0
+        G = g.build(density=spa)
+        say(f"  ties: k={k} nnz={G.nnz} oracle {okd.nnz}, ties {okd.n_ties}")
+        assert G.nnz == okd.nnz == 2 * k
+        assert np.array_equal(G.R, okd.R)
+        assert np.array_equal(G.C, okd.C)
+        assert G.thr_used == 0.0
+    finally:
+        g.close()
+
+
+def test_single_subject_cormat_is_r_or_zero(ctx):
+    # one subject, no Fisher: the packed matrix post_block leaves for the
+    # individual .cormat (CorMat.cpp:301-310): r where valid, 0 for invalid pairs
+    N = 300
+    x = subjects(N, [90], 0xB17A7100, dead={0: [7]})[0]
+    ref = oracle_packed(x)
+    g = gpu_group(ctx, [x], N, False, "bf16x3")
+    try:
+        g.finish()
+        avg = g.packed()
+        exp = np.where(np.isnan(ref), 0.0, ref)
+        assert np.max(np.abs(avg - exp)) <= AVG_TOL
+        assert np.all(avg[np.isnan(ref)] == 0.0)
+    finally:
+        g.close()
+
+
+def test_group_errors(ctx):
+    x = subjects(200, [64], 0xB17A7200)[0]
+    g = ctx.group(200)
+    try:
+        with pytest.raises(bna.BnaError):
+            g.finish()  # nothing added
+        with ctx.series(subjects(150, [64], 1)[0]) as s:
+            with pytest.raises(bna.BnaError):
+                g.add(s)  # wrong N
+        with ctx.series(x) as s:
+            g.add(s)
+            g.finish()
+            with pytest.raises(bna.BnaError):
+                g.add(s)  # already averaged
+        with pytest.raises(bna.BnaError):
+            g.build(density=0.0)
+    finally:
+        g.close()
