@@ -36,7 +36,7 @@ EXPORTS = [
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
-    "bna_group_build_csr",
+    "bna_group_build_csr", "bna_write_cormat",
 ]
 
 
@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         "bna_group_finish": (i32, [vp, vp]),
         "bna_group_download": (i32, [vp, vp, fp]),
         "bna_group_build_csr": (i32, [vp, vp, C.POINTER(Opts), C.POINTER(CSR)]),
+        "bna_write_cormat": (i32, [C.c_char_p, fp, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -130,6 +130,21 @@ extern "C" int bna_write_csr(const char *path, const bna_csr *g)
     return ok ? BNA_OK : BNA_ERR_IO;
 }
 
+// .cormat (CorMat.cpp:301-310, 570-579): the pair count as a 4-byte int (the
+// low 32 bits of M1, which the reference writes with sizeof(int)), then the
+// packed upper triangle as floats.
+extern "C" int bna_write_cormat(const char *path, const float *packed, int N)
+{
+    if (!path || !packed || N < 2) return BNA_ERR_ARG;
+    const int64_t M1 = (int64_t)N * (N - 1) / 2;
+    const uint32_t head = (uint32_t)((uint64_t)M1 & 0xffffffffu);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return BNA_ERR_IO;
+    bool ok = fwrite(&head, 4, 1, fp) == 1 && fwrite(packed, 4, (size_t)M1, fp) == (size_t)M1;
+    ok = (fclose(fp) == 0) && ok;
+    return ok ? BNA_OK : BNA_ERR_IO;
+}
+
 extern "C" int bna_read_csr(const char *path, bna_csr *out)
 {
     if (!path || !out) return BNA_ERR_ARG;

@@ -231,6 +231,9 @@ int bna_write_csr(const char *path, const bna_csr *g);
 int bna_read_csr(const char *path, bna_csr *out);
 /* .nm node maps (Degree.cpp:95-99, weighted-net.cu:305-322): int32 N, float[N]. */
 int bna_write_nm(const char *path, const float *v, int N);
+/* .cormat (CorMat.cpp:301-310): int32 low 32 bits of M1 = N(N-1)/2, then the
+ * M1 packed floats (bna_group_download's layout). */
+int bna_write_cormat(const char *path, const float *packed, int N);
 
 /* NIfTI-1 ingest as CUCorMat does it (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:141-238):
  * the 352-byte header read as short[176] (dims hdr[21..23], time points hdr[24],

@@ -110,3 +110,67 @@ def test_degree_and_ec_tools(subjects):
         if sub == "unweighted":
             lines = (d / sub / "isolated_v_mark.txt").read_text().split()
             assert list(map(int, lines)) == iso
+
+
+def read_cormat(path, N):
+    raw = np.fromfile(path, dtype=np.uint8)
+    head = int(raw[:4].view(np.uint32)[0])
+    M1 = N * (N - 1) // 2
+    assert head == M1 & 0xFFFFFFFF
+    vals = raw[4:].view(np.float32)
+    assert vals.size == M1
+    return vals
+
+
+def edge_set(R, Cc):
+    rows = np.repeat(np.arange(len(R) - 1), np.diff(R))
+    k = rows < Cc
+    return set(zip(rows[k].tolist(), Cc[k].tolist()))
+
+
+def test_cormat_group_average_fisher(subjects):
+    d, mask, Ls = subjects
+    say("cli: group average (Fisher), .cormat")
+    run("bna_cormat", d, d / "mask.nii", 0.5, "yf", "y", "r", 0.3)
+    packs, N = [], None
+    for name in sorted(Ls):
+        x, N = ingest(d, name)
+        z64, _, _, _, valid = orc.znorm(x, 0)
+        packs.append(orc.all_pairs(z64, valid))
+    ref = orc.group_average(packs, N, fisher=True)
+    base = f"{d.name}_{len(Ls)}_{N}"
+    cm = read_cormat(d / f"{base}.cormat", N)
+    assert np.abs(cm.astype(np.float64) - ref).max() <= 2e-5
+    files = sorted(d.glob(f"{base}_spa*_cor0.300.csr"))
+    assert len(files) == 1, sorted(p.name for p in d.iterdir())
+    g = bna.read_csr(files[0])
+    og = orc.threshold_packed(ref, N, np.float32(0.3))
+    for (i, j) in edge_set(g.R, g.C) ^ edge_set(og.R, og.C):
+        p = i * N - (i + 1) * i // 2 + j - i - 1
+        assert abs(float(ref[p]) - 0.3) <= 2e-5
+    spa = np.float32(100.0 * g.nnz / (N * (N - 1) / 2) / 2)
+    assert files[0].name == f"{base}_spa{spa:.3f}%_cor0.300.csr"
+
+
+def test_cormat_both_modes_with_subject_cormat(subjects):
+    d, mask, Ls = subjects
+    say("cli: avg bn (each subject and the group mean), density, .cormat")
+    run("bna_cormat", d, 0.5, "bn", "y", "s", 2.0)
+    packs, N = [], None
+    for name in sorted(Ls):
+        x, N = ingest(d, name)
+        z64, _, _, _, valid = orc.znorm(x, 0)
+        p = orc.all_pairs(z64, valid)
+        packs.append(p)
+        cm = read_cormat(d / "weighted" / f"{name}_{N}.cormat", N)
+        exp = np.where(np.isnan(p), 0.0, p)
+        assert np.abs(cm - exp).max() <= 4e-6
+        assert any((d / "weighted").glob(f"{name}_{N}_spa2.000%_cor*_weighted.csr"))
+    ref = orc.group_average(packs, N, fisher=False)
+    base = f"{d.name}_{len(Ls)}_{N}"
+    assert np.abs(read_cormat(d / f"{base}.cormat", N) - ref).max() <= 4e-6
+    files = sorted(d.glob(f"{base}_spa2.000%_cor*.csr"))
+    assert len(files) == 1
+    g = bna.read_csr(files[0])
+    k = orc.density_k(N, np.float32(2.0))
+    assert g.nnz == 2 * k and g.V is not None

@@ -80,11 +80,13 @@ def test_cli_argument_handling(tmp_path):
     r = _run("bna_cormat", tmp_path)
     assert r.returncode == 1 and "usage" in r.stderr
     write_nii(tmp_path / "mask.nii", np.ones((2, 2, 2), np.uint8))
-    # group averaging and the dense dump are refused before any device work
-    r = _run("bna_cormat", tmp_path, tmp_path / "mask.nii", 0.5, "yf", "n", "r", 0.3)
-    assert r.returncode == 2 and "averaging" in r.stderr
-    r = _run("bna_cormat", tmp_path, 0.5, "n", "y", "r", 0.3)
-    assert r.returncode == 2 and "cormat" in r.stderr
+    # malformed avg / save_cormat / threshold letters are refused before any device work
+    r = _run("bna_cormat", tmp_path, tmp_path / "mask.nii", 0.5, "q", "n", "r", 0.3)
+    assert r.returncode == 1 and "avg type" in r.stderr
+    r = _run("bna_cormat", tmp_path, tmp_path / "mask.nii", 0.5, "y", "n", "r", 0.3)
+    assert r.returncode == 1 and "second letter" in r.stderr
+    r = _run("bna_cormat", tmp_path, 0.5, "n", "x", "r", 0.3)
+    assert r.returncode == 1 and "to_save_cor_matrix" in r.stderr
     r = _run("bna_cormat", tmp_path, 0.5, "n", "n", "q", 0.3)
     assert r.returncode == 1 and "threshold type" in r.stderr
     r = _run("bna_cormat", "--precision=fp64", tmp_path, 0.5, "n", "n", "r", 0.3)
