@@ -269,6 +269,46 @@ def batch_form(n_subj: int, seed: int) -> dict:
             "config": "density 1 % each, split-BF16, weighted CSR left in device memory, series resident"}
 
 
+def group_form(n_subj: int, seed: int, N: int = 45000, L: int = 240) -> dict:
+    """Group averaging (the avg = yf path): n_subj subjects of the C2 shape
+    added into device-resident Fisher-z sums by the tile kernel's accumulation
+    pass, averaged, then a 1 % density graph of the averages."""
+    xs = [bna.synth_factor_model(N, L, 20, 0.6, seed + q) for q in range(n_subj)]
+    M1 = N * (N - 1) // 2
+    with bna.Context(0) as ctx:
+        ser = [ctx.series(x) for x in xs]
+        del xs
+        for s_ in ser:
+            s_.normalize(bna.BF16X3)
+        with ctx.group(N, fisher=True) as warm:  # kernels and workspaces
+            warm.add(ser[0])
+            warm.finish()
+            warm.build(density=1.0)
+        g = ctx.group(N, fisher=True)
+        ctx._L.bna_synchronize(ctx._h)
+        t0 = time.perf_counter()
+        add_ms = []
+        for s_ in ser:
+            g.add(s_)
+            add_ms.append(ctx.timings()["corr_ms"])
+        t1 = time.perf_counter()
+        g.finish()
+        t2 = time.perf_counter()
+        G = g.build(density=1.0)
+        t3 = time.perf_counter()
+        g.close()
+        for s_ in ser:
+            s_.close()
+    add_dt = t1 - t0
+    return {"subjects": n_subj, "N": N, "L": L, "add_ms_per_subject": add_dt * 1e3 / n_subj,
+            "accumulate_kernel_ms": float(np.mean(add_ms)),
+            "voxel_pairs_per_s": M1 * n_subj / add_dt,
+            "finish_ms": (t2 - t1) * 1e3, "graph_ms": (t3 - t2) * 1e3, "edges": G.nnz // 2,
+            "r_star": G.thr_used,
+            "config": "Fisher-z group mean (avg yf), split-BF16, 1 % density of the averages, "
+                      "series resident; graph_ms includes the CSR copy to the host"}
+
+
 def measured_traffic(N: int, L: int):
     """HBM bytes per launch of the split-BF16 tile kernel at N=45000, L=1200, from
     the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a
@@ -317,6 +357,8 @@ def main() -> int:
     ap.add_argument("--no-maps", action="store_true")
     ap.add_argument("--batch-subjects", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--group-subjects", type=int, default=8)
+    ap.add_argument("--no-group", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
     a = ap.parse_args()
     if a.calibrate_copy:
@@ -401,6 +443,10 @@ def main() -> int:
         log("bench: batched form")
         line["batch"] = batch_form(a.batch_subjects, 0xB17A0000 + 4000)
         log(f"bench: batch {line['batch']}")
+    if not a.no_group:
+        log("bench: group average")
+        line["group"] = group_form(a.group_subjects, 0xB17A0000 + 6000)
+        log(f"bench: group {line['group']}")
     if not a.no_serve:
         log("bench: serving stream")
         sv = serve_stream(a.serve_subjects, a.serve_rate, 0xB17A0000 + 5000, a.serve_workers,

@@ -193,10 +193,32 @@ __device__ __forceinline__ float fisher_z(float r) {
 // packed upper triangle, row-major with i < j (CorMat.cpp:851-866).  Lanes 0-31
 // and 32-63 touch two runs of 32 neighbouring slots.  Each pair belongs to one
 // tile, so the read-modify-write needs no atomic.
+// A tile inside the matrix and off the diagonal updates every slot of the
+// block without a branch (a value outside [-1, 1] adds +0, which leaves the sum
+// as it is), so its sixteen loads can be in flight together; edge and diagonal
+// tiles test each element first.
 template <int RB, int CB>
 __device__ __forceinline__ void epi_accum(const f32x16_t &a, const EpiCtx &c, float *dense, int fisher) {
     const int lc = c.wc0 + CB * 32 + c.lrow;
     const int64_t j = c.gj0 + lc;
+    const bool inner = !c.diag && c.gi0 + TILE <= c.N && c.gj0 + TILE <= c.N;
+    if (inner) {
+        float add[16];
+        float *slot[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int64_t i = c.gi0 + c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
+            const float v = a[e];
+            add[e] = (v >= -1.0f && v <= 1.0f) ? (fisher ? fisher_z(v) : v) : 0.0f;
+            slot[e] = dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
+        }
+        float old[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) old[e] = *slot[e];
+#pragma unroll
+        for (int e = 0; e < 16; e++) *slot[e] = old[e] + add[e];
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
