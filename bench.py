@@ -296,6 +296,7 @@ def group_form(n_subj: int, seed: int, N: int = 45000, L: int = 240) -> dict:
         t2 = time.perf_counter()
         G = g.build(density=1.0)
         t3 = time.perf_counter()
+        gt = ctx.timings()
         g.close()
         for s_ in ser:
             s_.close()
@@ -303,7 +304,8 @@ def group_form(n_subj: int, seed: int, N: int = 45000, L: int = 240) -> dict:
     return {"subjects": n_subj, "N": N, "L": L, "add_ms_per_subject": add_dt * 1e3 / n_subj,
             "accumulate_kernel_ms": float(np.mean(add_ms)),
             "voxel_pairs_per_s": M1 * n_subj / add_dt,
-            "finish_ms": (t2 - t1) * 1e3, "graph_ms": (t3 - t2) * 1e3, "edges": G.nnz // 2,
+            "finish_ms": (t2 - t1) * 1e3, "graph_ms": (t3 - t2) * 1e3,
+            "graph_select_ms": gt["select_ms"], "graph_csr_ms": gt["csr_ms"], "edges": G.nnz // 2,
             "r_star": G.thr_used,
             "config": "Fisher-z group mean (avg yf), split-BF16, 1 % density of the averages, "
                       "series resident; graph_ms includes the CSR copy to the host"}

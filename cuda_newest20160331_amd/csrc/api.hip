@@ -5,6 +5,7 @@
 #include "bna.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1646,6 +1647,8 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
     float thr = (float)o->thr;
     int64_t p_cut = 0;
     int rc;
+    memset(&ctx->tm, 0, sizeof(ctx->tm));
+    const auto t0 = std::chrono::steady_clock::now();
     if (dens) {
         const int64_t k = bna_density_to_k(N, (float)o->thr);
         if (k < 1) return fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
@@ -1677,6 +1680,7 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
             p_cut = packed_pos(N, i, i + 1 + (int64_t)q);
         }
     }
+    const auto t1 = std::chrono::steady_clock::now();
     uint8_t *keep = nullptr;
     unsigned int *flag = nullptr;
     int64_t *cnt = nullptr;
@@ -1725,6 +1729,10 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
     r.thr_used = thr;
     r.density_pct = 100.0 * (double)nnz / (double)g->M1 / 2.0;
     *out = r;
+    const auto t2 = std::chrono::steady_clock::now();
+    ctx->tm.select_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->tm.csr_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    ctx->tm.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
     return BNA_OK;
 }
 

@@ -51,10 +51,21 @@ __global__ __launch_bounds__(256) void group_digits_kernel(const float *__restri
     h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // a run of equal digits (the common case for the top digit, where most
+    // averages share an exponent) costs one LDS atomic
+    uint32_t cur = 0, run = 0;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < M1; t += stride) {
         const uint32_t k = fkey(acc[t]);
-        if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & 255u], 1u);
+        if ((k & mask) != prefix) continue;
+        const uint32_t d = (k >> shift) & 255u;
+        if (d != cur && run) {
+            atomicAdd(&h[cur], run);
+            run = 0;
+        }
+        cur = d;
+        run++;
     }
+    if (run) atomicAdd(&h[cur], run);
     __syncthreads();
     if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
