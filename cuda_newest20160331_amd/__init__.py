@@ -36,7 +36,7 @@ EXPORTS = [
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
-    "bna_group_build_csr", "bna_write_cormat",
+    "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti",
 ]
 
 
@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         "bna_group_download": (i32, [vp, vp, fp]),
         "bna_group_build_csr": (i32, [vp, vp, C.POINTER(Opts), C.POINTER(CSR)]),
         "bna_write_cormat": (i32, [C.c_char_p, fp, i32]),
+        "bna_nm_to_nifti": (i32, [C.c_char_p, dbl, fp, i32, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -324,6 +324,51 @@ int bna_nifti_bold(const char *path, const uint8_t *selected, int64_t total_voxe
     return BNA_OK;
 }
 
+// ConvertNII (W/ConvertNII/ConvertNII/ConvertNII.cpp:65-167): the mask's own
+// 352-byte header with datatype 16 and 32 bits per value, then one float per
+// voxel of the mask volume: the map's next value where mask >= thr (ascending
+// linear voxel index, the order of bna_nifti_bold), 0 elsewhere.  The threshold
+// compares in double, as the tool's ProbCut does.
+int bna_nm_to_nifti(const char *mask_path, double mask_thr, const float *values, int N, const char *out_path)
+{
+    if (!mask_path || !values || !out_path || N < 0) return BNA_ERR_ARG;
+    FILE *fp = fopen(mask_path, "rb");
+    if (!fp) return BNA_ERR_IO;
+    int16_t h[HDR_BYTES / 2];
+    int rc = read_hdr(fp, h);
+    const int64_t total = (int64_t)h[21] * h[22] * h[23];
+    const int dt = h[35];
+    const int esize = dt == 2 ? 1 : (dt == 16 ? 4 : 0);
+    if (rc == BNA_OK && (total <= 0 || esize == 0)) rc = BNA_ERR_FORMAT;
+    if (rc == BNA_OK && file_bytes(fp) < HDR_BYTES + total * esize) rc = BNA_ERR_FORMAT;
+    std::vector<unsigned char> raw;
+    if (rc == BNA_OK) {
+        raw.resize((size_t)(total * esize));
+        if (fread(raw.data(), 1, raw.size(), fp) != raw.size()) rc = BNA_ERR_FORMAT;
+    }
+    fclose(fp);
+    if (rc != BNA_OK) return rc;
+    std::vector<float> vol((size_t)total, 0.0f);
+    int64_t l = 0;
+    for (int64_t k = 0; k < total; k++) {
+        float m;
+        if (dt == 2) m = (float)raw[(size_t)k];
+        else memcpy(&m, &raw[(size_t)k * 4], 4);
+        if ((double)m >= mask_thr) {
+            if (l >= N) return BNA_ERR_FORMAT;  // the map is shorter than the mask
+            vol[(size_t)k] = values[l++];
+        }
+    }
+    if (l != N) return BNA_ERR_FORMAT;  // the mask selects fewer voxels than the map holds
+    h[35] = 16;
+    h[36] = 32;
+    FILE *fo = fopen(out_path, "wb");
+    if (!fo) return BNA_ERR_IO;
+    bool ok = fwrite(h, 1, HDR_BYTES, fo) == (size_t)HDR_BYTES && fwrite(vol.data(), 4, vol.size(), fo) == vol.size();
+    ok = (fclose(fo) == 0) && ok;
+    return ok ? BNA_OK : BNA_ERR_IO;
+}
+
 void bna_free(void *p) { free(p); }
 
 }  // extern "C"

@@ -231,6 +231,12 @@ int bna_write_csr(const char *path, const bna_csr *g);
 int bna_read_csr(const char *path, bna_csr *out);
 /* .nm node maps (Degree.cpp:95-99, weighted-net.cu:305-322): int32 N, float[N]. */
 int bna_write_nm(const char *path, const float *v, int N);
+/* A vertex map (N floats, e.g. a _deg.nm or _ec.nm payload) as a NIfTI volume
+ * on the mask's grid, the ConvertNII tool (W/ConvertNII/ConvertNII/
+ * ConvertNII.cpp:65-167): the mask's header with float data, the map's values at
+ * the voxels with mask >= mask_thr in ascending order, 0 elsewhere.
+ * BNA_ERR_FORMAT if the mask selects a different number of voxels than N. */
+int bna_nm_to_nifti(const char *mask_path, double mask_thr, const float *values, int N, const char *out_path);
 /* .cormat (CorMat.cpp:301-310): int32 low 32 bits of M1 = N(N-1)/2, then the
  * M1 packed floats (bna_group_download's layout). */
 int bna_write_cormat(const char *path, const float *packed, int N);

@@ -114,3 +114,46 @@ def test_request_footprint_and_server_without_device():
     if not has_gpu:
         with pytest.raises(bna.BnaError):
             bna.Server()
+
+
+@pytest.mark.parametrize("mask_dtype", [np.uint8, np.float32])
+def test_convert_nii_scatters_maps_onto_the_mask(tmp_path, mask_dtype):
+    # ConvertNII (W/ConvertNII/ConvertNII/ConvertNII.cpp:65-167): the inverse of
+    # the ingest's gather.  A volume built by the tool and read back through
+    # bna_nifti_bold with the same mask gives the map again; voxels outside the
+    # mask are 0; the header is the mask's with float data.
+    rng = np.random.default_rng(9)
+    X, Y, Z = 7, 6, 5
+    if mask_dtype == np.uint8:
+        mask = (rng.random((X, Y, Z)) < 0.6).astype(np.uint8)
+        thr = 0.5
+    else:
+        mask = rng.random((X, Y, Z)).astype(np.float32)
+        thr = 0.4
+    write_nii(tmp_path / "mask.nii", mask)
+    sel, N = bna.nifti_mask(str(tmp_path / "mask.nii"), thr)
+    d = tmp_path / "maps"
+    d.mkdir()
+    v = rng.normal(size=N).astype(np.float32)
+    assert bna.lib().bna_write_nm(str(d / "s_deg.nm").encode(), bna._f32p(v), N) == 0
+    w = rng.normal(size=N + 3).astype(np.float32)
+    assert bna.lib().bna_write_nm(str(d / "t_ec.nm").encode(), bna._f32p(w), N + 3) == 0
+    r = _run("bna_convert_nii", d, tmp_path / "mask.nii", thr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "doesn't match" in r.stdout
+    assert not (d / "t_ec.nii").exists()
+    out = (d / "s_deg.nii").read_bytes()
+    mh = (tmp_path / "mask.nii").read_bytes()[:352]
+    h = np.frombuffer(out[:352], np.int16)
+    m16 = np.frombuffer(mh, np.int16)
+    assert h[35] == 16 and h[36] == 32
+    keep = np.ones(176, bool)
+    keep[[35, 36]] = False
+    assert np.array_equal(h[keep], m16[keep])
+    vol = np.frombuffer(out[352:], np.float32)
+    flat_mask = mask.reshape(-1, order="F").astype(np.float64) >= thr
+    assert vol.size == X * Y * Z
+    assert np.array_equal(vol[flat_mask], v)
+    assert np.all(vol[~flat_mask] == 0)
+    back = bna.nifti_bold(str(d / "s_deg.nii"), sel)
+    assert np.array_equal(back.reshape(-1), v)
