@@ -205,11 +205,17 @@ def density_and_maps(ctx, steps: int) -> dict:
         ms = (time.perf_counter() - t0) * 1e3 / steps
         st = ctx.timings()
         nnz = g.nnz
-        ctx.degree_device(g)
-        deg_ms = ctx.timings()["degree_ms"]
-        _, lam, it = ctx.eigcent_device(g)
-        t = ctx.timings()
-        ec_ms, iters = t["ec_ms"], t["ec_iters"]
+        deg_ms = []
+        for _ in range(3):  # the first call also pays the kernel's first launch
+            ctx.degree_device(g)
+            deg_ms.append(ctx.timings()["degree_ms"])
+        deg_ms = min(deg_ms)
+        ec_ms = None
+        for _ in range(2):
+            _, lam, it = ctx.eigcent_device(g)
+            t = ctx.timings()
+            if ec_ms is None or t["ec_ms"] < ec_ms:
+                ec_ms, iters = t["ec_ms"], t["ec_iters"]
         s.free_device(g)
     deg_bytes = 4 * (N + 1) + 4 * nnz + 4 * N
     spmv_bytes = 4 * (N + 1) + 8 * nnz + 16 * N
@@ -219,11 +225,12 @@ def density_and_maps(ctx, steps: int) -> dict:
                                 "weighted CSR", "ms_per_step": ms, "voxel_pairs_per_s": M1 / (ms / 1e3),
                     "edges_per_s": (nnz // 2) / (ms / 1e3), "corr_ms": st["corr_ms"], "select_ms": st["select_ms"],
                     "recheck_ms": st["recheck_ms"], "refine_ms": st["refine_ms"]}
-    out["degree"] = {"ms": deg_ms, "GB_per_s": deg_gbs, "roofline_frac": deg_gbs / 8000.0, "bytes": deg_bytes}
+    out["degree"] = {"ms": deg_ms, "GB_per_s": deg_gbs, "roofline_frac": deg_gbs / 8000.0, "bytes": deg_bytes,
+                     "note": "best of 3 calls"}
     out["eigcent"] = {"ms": ec_ms, "iterations": iters, "lambda": lam, "spmv_GB_per_s": spmv_gbs,
                       "roofline_frac": spmv_gbs / 8000.0, "bytes_per_iteration": spmv_bytes,
                       "note": "whole power iteration on the device (SpMV, FP64 norms, stopping test); "
-                              "GB/s = algorithmic bytes x iterations / time"}
+                              "GB/s = algorithmic bytes x iterations / time; best of 2 runs"}
     return out
 
 
