@@ -2,10 +2,9 @@
 // over a device CSR.
 //
 // Degree follows W/Degree/Degree/Degree.cpp:83-87: a float running sum of V in
-// CSR order, so the order (and every rounding) is the reference's.  One wave per
-// row: the lanes load 64 consecutive values (coalesced) and every lane adds them
-// in order, taking each from its lane with readlane; binary graphs take the row
-// length (B/Degree/Degree/Degree.cpp:84-91).
+// CSR order, so the order (and every rounding) is the reference's.  One thread
+// per row walks the row in order; binary graphs take the row length
+// (B/Degree/Degree/Degree.cpp:84-91).
 //
 // Eigenvector centrality follows Lead_Vector_GPU (W/CUEC/weighted-net.cu:55-221):
 //   x0 <- x; n = ||x||; x <- x * (1/n); y = A x; x <- y;
@@ -22,31 +21,37 @@
 
 namespace bna {
 
+// One thread per row: the row's values are added one after another in CSR
+// order, the reference's float loop (W/Degree/Degree/Degree.cpp:83-87), so the
+// sum is the same float bit for bit.  A wave's 64 rows run side by side; each
+// lane walks its own row, four loads ahead of its additions.
 __global__ __launch_bounds__(256) void degree_kernel(const int32_t *__restrict__ R, const float *__restrict__ V, int N,
                                                      float *__restrict__ deg)
 {
-    const int lane = threadIdx.x & 63;
-    const int i = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int b = R[i], e = R[i + 1];
     if (!V) {
-        if (lane == 0) deg[i] = (float)(e - b);
+        deg[i] = (float)(e - b);
         return;
     }
     float s = 0.0f;
-    for (int base = b; base < e; base += 64) {
-        const float v = base + lane < e ? V[base + lane] : 0.0f;
-        const int n = e - base < 64 ? e - base : 64;
-        const int vb = __builtin_bit_cast(int, v);
-        for (int k = 0; k < n; k++) s += __builtin_bit_cast(float, __builtin_amdgcn_readlane(vb, k));
+    int k = b;
+    for (; k + 4 <= e; k += 4) {
+        const float v0 = V[k], v1 = V[k + 1], v2 = V[k + 2], v3 = V[k + 3];
+        s += v0;
+        s += v1;
+        s += v2;
+        s += v3;
     }
-    if (lane == 0) deg[i] = s;
+    for (; k < e; k++) s += V[k];
+    deg[i] = s;
 }
 
 hipError_t launch_degree(const int32_t *R, const float *V, int N, float *deg, hipStream_t st)
 {
-    const int64_t threads = (int64_t)N * 64;
-    hipLaunchKernelGGL(degree_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, R, V, N, deg);
+    if (N < 1) return hipSuccess;
+    hipLaunchKernelGGL(degree_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, R, V, N, deg);
     return hipGetLastError();
 }
 
