@@ -1684,12 +1684,13 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
     uint8_t *keep = nullptr;
     unsigned int *flag = nullptr;
     int64_t *cnt = nullptr;
-    if ((rc = ws_get_t(ctx, "group_keep", (size_t)N * N, &keep))) return rc;
+    const int64_t ld = ((int64_t)N + 63) / 64 * 64;
+    if ((rc = ws_get_t(ctx, "group_keep", (size_t)N * (size_t)ld, &keep))) return rc;
     if ((rc = ws_get_t(ctx, "group_flag", 1, &flag))) return rc;
     if ((rc = ws_get_t(ctx, "group_cnt", (size_t)N, &cnt))) return rc;
     HIPCHK(ctx, hipMemsetAsync(flag, 0, sizeof(unsigned int), ctx->stream));
-    HIPCHK(ctx, launch_group_keep(g->acc, N, thr, dens ? 1 : 0, p_cut, keep, flag, ctx->stream));
-    HIPCHK(ctx, launch_group_rowcount(keep, N, cnt, ctx->stream));
+    HIPCHK(ctx, launch_group_keep(g->acc, N, ld, thr, dens ? 1 : 0, p_cut, keep, flag, ctx->stream));
+    HIPCHK(ctx, launch_group_rowcount(keep, N, ld, cnt, ctx->stream));
     std::vector<int64_t> c(N);
     unsigned int bad = 0;
     HIPCHK(ctx, hipMemcpyAsync(c.data(), cnt, sizeof(int64_t) * N, hipMemcpyDeviceToHost, ctx->stream));
@@ -1720,7 +1721,7 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
         return rc;
     }
     hipError_t e = hipMemcpyAsync(r.R, R.data(), sizeof(int32_t) * R.size(), hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = launch_group_fill(keep, g->acc, N, r.R, r.C, r.V, ctx->stream);
+    if (e == hipSuccess) e = launch_group_fill(keep, ld, g->acc, N, r.R, r.C, r.V, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         bna_csr_release(ctx, &r);

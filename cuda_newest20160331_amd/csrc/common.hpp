@@ -119,16 +119,17 @@ hipError_t launch_group_digits(const float *acc, int64_t M1, uint32_t prefix, ui
                                unsigned long long *hist, hipStream_t st);
 // Per packed row i: how many of its values equal `val` (ties of a density threshold).
 hipError_t launch_group_ties(const float *acc, int N, float val, int64_t *tie_rows, hipStream_t st);
-// Full N x N keep matrix (bytes, both triangles, zero diagonal): density mode
-// (dens = 1) keeps v > thr and the ties v == thr at packed index <= p_cut;
-// r mode keeps v > thr.  flag[0] gets 1 if any value exceeds 1.
-hipError_t launch_group_keep(const float *acc, int N, float thr, int dens, int64_t p_cut, uint8_t *keep,
+// Full N x N keep matrix (bytes, both triangles, zero diagonal, row stride ld =
+// N rounded up to 64, padding columns 0): density mode (dens = 1) keeps v > thr
+// and the ties v == thr at packed index <= p_cut; r mode keeps v > thr.
+// flag[0] gets 1 if any value exceeds 1.
+hipError_t launch_group_keep(const float *acc, int N, int64_t ld, float thr, int dens, int64_t p_cut, uint8_t *keep,
                              unsigned int *flag, hipStream_t st);
 // Row lengths of the keep matrix, then the CSR rows (columns ascending, values
 // from the packed averages when V is not null).
-hipError_t launch_group_rowcount(const uint8_t *keep, int N, int64_t *cnt, hipStream_t st);
-hipError_t launch_group_fill(const uint8_t *keep, const float *acc, int N, const int32_t *R, int32_t *C, float *V,
-                             hipStream_t st);
+hipError_t launch_group_rowcount(const uint8_t *keep, int N, int64_t ld, int64_t *cnt, hipStream_t st);
+hipError_t launch_group_fill(const uint8_t *keep, int64_t ld, const float *acc, int N, const int32_t *R, int32_t *C,
+                             float *V, hipStream_t st);
 
 // FP64 recheck of candidates.
 // Slots at or past entry_cap (overflowed lists) are skipped.

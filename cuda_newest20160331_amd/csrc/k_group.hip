@@ -8,12 +8,15 @@
 // :683-703 or the exact-k FormFullGraph_s :743-822) and lists each row's
 // neighbours in column order (FormCSRGraph :889-918).  Here the sums stay in
 // HBM (the tile kernel's accumulation pass adds each subject), and the
-// threshold writes a full N x N keep matrix of bytes (2 GB at N = 45000), so
+// threshold writes a full N x N keep matrix of bytes (2 GB at N = 45000; rows
+// padded to a multiple of 64 bytes), so
 // that the symmetric CSR is read row by row with coalesced loads instead of
 // gathering column entries from the packed triangle.
 #include "common.hpp"
 
 namespace bna {
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int64_t packed_at(int64_t N, int64_t i, int64_t j)
 {
@@ -89,63 +92,88 @@ __global__ __launch_bounds__(256) void group_ties_kernel(const float *__restrict
 
 // One 64 x 64 block (bi <= bj) of the upper triangle per workgroup: decide
 // each pair, write the block to its place in the keep matrix and its
-// transpose through LDS, so both writes are row runs.
-__global__ __launch_bounds__(256) void group_keep_kernel(const float *__restrict__ acc, int N, float thr, int dens,
-                                                         int64_t p_cut, uint8_t *__restrict__ keep,
+// transpose through LDS.  Rows of the keep matrix are ld bytes apart, ld a
+// multiple of 64, so every thread's 16 bytes of a row go out as one store.
+__global__ __launch_bounds__(256) void group_keep_kernel(const float *__restrict__ acc, int N, int64_t ld, float thr,
+                                                         int dens, int64_t p_cut, uint8_t *__restrict__ keep,
                                                          unsigned int *__restrict__ flag)
 {
     const int bj = blockIdx.x, bi = blockIdx.y;
     if (bi > bj) return;
-    __shared__ uint8_t tile[64][65];
+    __shared__ __align__(16) uint8_t tile[64][80];  // row pitch: a multiple of 16 bytes
     const int t = threadIdx.x;
     const int64_t i0 = (int64_t)bi * 64, j0 = (int64_t)bj * 64;
     {
         const int r = t >> 2, cs = (t & 3) * 16;
         const int64_t i = i0 + r;
         bool bad = false;
-#pragma unroll 4
+        uint8_t k[16];
+#pragma unroll
         for (int c = 0; c < 16; c++) {
             const int64_t j = j0 + cs + c;
-            uint8_t k = 0;
+            k[c] = 0;
             if (i < N && j < N && i < j) {
                 const int64_t p = packed_at(N, i, j);
                 const float v = acc[p];
-                k = dens ? (uint8_t)(v > thr || (v == thr && p <= p_cut)) : (uint8_t)(v > thr);
+                k[c] = dens ? (uint8_t)(v > thr || (v == thr && p <= p_cut)) : (uint8_t)(v > thr);
                 bad |= v > 1.0f;
             }
-            tile[r][cs + c] = k;
-            if (i < N && j < N && i <= j) keep[i * N + j] = k;  // the diagonal gets 0
         }
+        u32x4_t w;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            w[q] = (uint32_t)k[4 * q] | ((uint32_t)k[4 * q + 1] << 8) | ((uint32_t)k[4 * q + 2] << 16) |
+                   ((uint32_t)k[4 * q + 3] << 24);
+        *(u32x4_t *)&tile[r][cs] = w;
+        // the block's own rows: pairs (i, j) with i <= j; below the diagonal of
+        // a diagonal block the transpose below writes the same bytes' partners,
+        // and here those positions hold 0 until it does
+        if (i < N && bi < bj) *(u32x4_t *)(keep + i * ld + j0 + cs) = w;
         if (bad && !dens) atomicOr(flag, 1u);
     }
     __syncthreads();
     {
         const int jj = t >> 2, is = (t & 3) * 16;
         const int64_t j = j0 + jj;
-#pragma unroll 4
-        for (int c = 0; c < 16; c++) {
-            const int64_t i = i0 + is + c;
-            if (i < N && j < N && i < j) keep[j * N + i] = tile[is + c][jj];
+        if (j < N) {
+            uint8_t k[16];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                const int ii = is + c;
+                // diagonal block: (j, i) with i < j comes from tile[ii][jj]; i >= j from tile[jj][ii]
+                k[c] = (bi < bj || ii < jj) ? tile[ii][jj] : tile[jj][ii];
+            }
+            u32x4_t w;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                w[q] = (uint32_t)k[4 * q] | ((uint32_t)k[4 * q + 1] << 8) | ((uint32_t)k[4 * q + 2] << 16) |
+                       ((uint32_t)k[4 * q + 3] << 24);
+            *(u32x4_t *)(keep + j * ld + i0 + is) = w;
         }
     }
 }
 
-// One wave per row of the keep matrix.
-__global__ __launch_bounds__(256) void group_rowcount_kernel(const uint8_t *__restrict__ keep, int N,
+// One wave per row of the keep matrix: 16 bytes per lane per step.
+__global__ __launch_bounds__(256) void group_rowcount_kernel(const uint8_t *__restrict__ keep, int N, int64_t ld,
                                                              int64_t *__restrict__ cnt)
 {
     const int lane = threadIdx.x & 63;
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (i >= N) return;
-    const uint8_t *row = keep + i * N;
+    const uint8_t *row = keep + i * ld;
     int64_t c = 0;
-    for (int64_t j = lane; j < N; j += 64) c += row[j];
+    for (int64_t j = (int64_t)lane * 16; j < N; j += 1024) {
+        const u32x4_t w = *(const u32x4_t *)(row + j);
+        // bytes are 0 or 1; columns at or past N are 0
+#pragma unroll
+        for (int q = 0; q < 4; q++) c += (int64_t)((w[q] * 0x01010101u) >> 24);
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
     if (lane == 0) cnt[i] = c;
 }
 
-__global__ __launch_bounds__(256) void group_fill_kernel(const uint8_t *__restrict__ keep,
+__global__ __launch_bounds__(256) void group_fill_kernel(const uint8_t *__restrict__ keep, int64_t ld,
                                                          const float *__restrict__ acc, int N,
                                                          const int32_t *__restrict__ R, int32_t *__restrict__ C,
                                                          float *__restrict__ V)
@@ -153,19 +181,36 @@ __global__ __launch_bounds__(256) void group_fill_kernel(const uint8_t *__restri
     const int lane = threadIdx.x & 63;
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (i >= N) return;
-    const uint8_t *row = keep + i * N;
-    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint8_t *row = keep + i * ld;
     int64_t base = R[i];
-    for (int64_t j0 = 0; j0 < N; j0 += 64) {
-        const int64_t j = j0 + lane;
-        const bool k = j < N && row[j] != 0;
-        const uint64_t m = __ballot(k);
-        if (k) {
-            const int64_t pos = base + __popcll(m & lt);
-            C[pos] = (int32_t)j;
-            if (V) V[pos] = acc[i < j ? packed_at(N, i, j) : packed_at(N, j, i)];
+    for (int64_t j0 = 0; j0 < N; j0 += 1024) {
+        const int64_t jl = j0 + (int64_t)lane * 16;
+        u32x4_t w = {0u, 0u, 0u, 0u};
+        if (jl < N) w = *(const u32x4_t *)(row + jl);
+        int n = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) n += (int)((w[q] * 0x01010101u) >> 24);
+        // this lane's first slot: the kept columns of the lanes before it
+        int pre = n;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(pre, d, 64);
+            if (lane >= d) pre += o;
         }
-        base += __popcll(m);
+        const int total = __shfl(pre, 63, 64);
+        int64_t pos = base + pre - n;
+        if (n) {
+#pragma unroll
+            for (int b = 0; b < 16; b++) {
+                if ((w[b >> 2] >> (8 * (b & 3))) & 0xffu) {
+                    const int64_t j = jl + b;
+                    C[pos] = (int32_t)j;
+                    if (V) V[pos] = acc[i < j ? packed_at(N, i, j) : packed_at(N, j, i)];
+                    pos++;
+                }
+            }
+        }
+        base += total;
     }
 }
 
@@ -200,28 +245,31 @@ hipError_t launch_group_ties(const float *acc, int N, float val, int64_t *tie_ro
     return hipGetLastError();
 }
 
-hipError_t launch_group_keep(const float *acc, int N, float thr, int dens, int64_t p_cut, uint8_t *keep,
+hipError_t launch_group_keep(const float *acc, int N, int64_t ld, float thr, int dens, int64_t p_cut, uint8_t *keep,
                              unsigned int *flag, hipStream_t st)
 {
     if (N < 1) return hipSuccess;
     const unsigned nbk = (unsigned)((N + 63) / 64);
-    if (nbk > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(group_keep_kernel, dim3(nbk, nbk), dim3(256), 0, st, acc, N, thr, dens, p_cut, keep, flag);
+    if (nbk > 65535 || ld % 64 || ld < (int64_t)nbk * 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_keep_kernel, dim3(nbk, nbk), dim3(256), 0, st, acc, N, ld, thr, dens, p_cut, keep, flag);
     return hipGetLastError();
 }
 
-hipError_t launch_group_rowcount(const uint8_t *keep, int N, int64_t *cnt, hipStream_t st)
+hipError_t launch_group_rowcount(const uint8_t *keep, int N, int64_t ld, int64_t *cnt, hipStream_t st)
 {
     if (N < 1) return hipSuccess;
-    hipLaunchKernelGGL(group_rowcount_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, N, cnt);
+    if (ld % 64 || ld < N) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_rowcount_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, N, ld, cnt);
     return hipGetLastError();
 }
 
-hipError_t launch_group_fill(const uint8_t *keep, const float *acc, int N, const int32_t *R, int32_t *C, float *V,
-                             hipStream_t st)
+hipError_t launch_group_fill(const uint8_t *keep, int64_t ld, const float *acc, int N, const int32_t *R, int32_t *C,
+                             float *V, hipStream_t st)
 {
     if (N < 1) return hipSuccess;
-    hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, acc, N, R, C, V);
+    if (ld % 64 || ld < N) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, ld, acc, N, R, C,
+                       V);
     return hipGetLastError();
 }
 
