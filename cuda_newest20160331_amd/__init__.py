@@ -36,7 +36,7 @@ EXPORTS = [
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
-    "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti",
+    "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
 ]
 
 
@@ -128,6 +128,8 @@ def lib() -> C.CDLL:
         "bna_group_build_csr": (i32, [vp, vp, C.POINTER(Opts), C.POINTER(CSR)]),
         "bna_write_cormat": (i32, [C.c_char_p, fp, i32]),
         "bna_nm_to_nifti": (i32, [C.c_char_p, dbl, fp, i32, C.c_char_p]),
+        "bna_corr_packed": (i32, [vp, vp, i32, fp]),
+        "bna_select_density": (i32, [vp, vp, i32, dbl, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -435,6 +437,21 @@ class Series:
 
     def free_device(self, g: CSR):
         self.ctx._L.bna_csr_release(self.ctx._h, C.byref(g))
+
+    def corr_packed(self, precision: str = "bf16x3") -> np.ndarray:
+        """The packed r matrix (M1 floats; 0 outside [-1, 1] and for invalid voxels)."""
+        out = np.empty(self.N * (self.N - 1) // 2, np.float32)
+        self.ctx._check(self.ctx._L.bna_corr_packed(self.ctx._h, self._h, PRECISIONS[precision], _f32p(out)),
+                        "bna_corr_packed")
+        return out
+
+    def select_density(self, density: float, precision: str = "bf16x3") -> tuple[float, int]:
+        """(r*, k): the k-th largest r for a density in percent."""
+        r = C.c_double()
+        k = C.c_int64()
+        self.ctx._check(self.ctx._L.bna_select_density(self.ctx._h, self._h, PRECISIONS[precision], density,
+                                                       C.byref(r), C.byref(k)), "bna_select_density")
+        return r.value, k.value
 
     def build(self, **kw) -> Graph:
         g = self.build_device(**kw)

@@ -1635,6 +1635,35 @@ int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed)
     return copy_d2h(ctx, packed, g->acc, (size_t)g->M1 * 4);
 }
 
+int bna_corr_packed(bna_ctx *ctx, bna_series *s, int precision, float *packed)
+{
+    if (!ctx || !s || !packed) return fail(ctx, BNA_ERR_ARG, "bna_corr_packed: NULL argument");
+    bna_group *g = nullptr;
+    int rc = bna_group_create(ctx, s->N, 0, &g);
+    if (rc) return rc;
+    if (!(rc = bna_group_add(ctx, g, s, precision)) && !(rc = bna_group_finish(ctx, g)))
+        rc = bna_group_download(ctx, g, packed);
+    bna_group_destroy(ctx, g);
+    return rc;
+}
+
+int bna_select_density(bna_ctx *ctx, bna_series *s, int precision, double density_pct, double *r_star, int64_t *k)
+{
+    if (!ctx || !s || !r_star) return fail(ctx, BNA_ERR_ARG, "bna_select_density: NULL argument");
+    bna_opts o;
+    bna_opts_default(&o);
+    o.thr_type = BNA_THR_DENSITY;
+    o.thr = density_pct;
+    o.weighted = 0;
+    o.precision = precision;
+    bna_csr g{};
+    int rc = bna_build_csr_series(ctx, s, &o, &g);
+    if (rc) return rc;
+    *r_star = g.thr_used;
+    if (k) *k = g.nnz / 2;
+    return bna_csr_release(ctx, &g);
+}
+
 int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *out)
 {
     if (!ctx || !g || !o || !out) return fail(ctx, BNA_ERR_ARG, "bna_group_build_csr: NULL argument");

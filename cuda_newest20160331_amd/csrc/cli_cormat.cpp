@@ -151,7 +151,7 @@ int main(int argc, char **argv)
         snprintf(tag, sizeof tag, "_spa%.3f%%_cor%.3f", (double)s_val, (double)r_val);
         return std::string(tag);
     };
-    // the packed matrix of a group (or of one subject) to a .cormat file
+    // the packed averages of a group to a .cormat file
     auto dump_cormat = [&](bna_group *grp, const std::string &path) {
         std::vector<float> packed((size_t)((int64_t)N * (N - 1) / 2));
         int r = bna_group_download(ctx, grp, packed.data());
@@ -178,16 +178,15 @@ int main(int argc, char **argv)
             continue;
         }
         const std::string base = stem(join(outdir, f)) + "_" + std::to_string(N);
-        if (save_cormat) {
-            bna_group *one = nullptr;
-            rc = bna_group_create(ctx, N, 0, &one);
-            if (!rc) rc = bna_group_add(ctx, one, s, o.precision);
-            if (!rc) rc = bna_group_finish(ctx, one);
-            if (rc)
+        if (save_cormat) {  // CorMat.cpp:301-310
+            std::vector<float> packed((size_t)((int64_t)N * (N - 1) / 2));
+            const std::string path = base + ".cormat";
+            if ((rc = bna_corr_packed(ctx, s, o.precision, packed.data()))) {
                 status = die(ctx, rc, "r matrix of " + in);
-            else if (dump_cormat(one, base + ".cormat"))
-                status = 1;
-            bna_group_destroy(ctx, one);
+            } else {
+                printf("generating %s...\n", path.c_str());
+                if ((rc = bna_write_cormat(path.c_str(), packed.data(), N))) status = die(ctx, rc, "write " + path);
+            }
         }
         for (int k = 0; k < nthr; k++) {
             const bna_opts op = opts_for(k);

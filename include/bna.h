@@ -197,6 +197,17 @@ int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed);
  * largest average. */
 int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *opts, bna_csr *out);
 
+/* The packed r matrix of one subject, M1 floats in row-major upper-triangle
+ * order with 0 for pairs outside [-1, 1] or with an invalid voxel: what
+ * CorMat_gpu followed by post_block leaves (CorMat.cu:17, CorMat.cpp:259-262),
+ * the payload of a subject's .cormat.  r carries the precision's MFMA error. */
+int bna_corr_packed(bna_ctx *ctx, bna_series *s, int precision, float *packed);
+/* The r threshold of a density: the k-th largest r of the subject, with k from
+ * bna_density_to_k, as CorMat_spa2rth returns it
+ * (CorMat-2mm-transition-GPUhisto.cu:70-214) but exact (FP64 recheck of the
+ * pairs at the boundary).  k may be NULL. */
+int bna_select_density(bna_ctx *ctx, bna_series *s, int precision, double density_pct, double *r_star, int64_t *k);
+
 /* ------------------------------------------------------------ graph maps */
 /* Degree.  Weighted: float sum of V along each row in CSR order
  * (W/Degree/Degree/Degree.cpp:83-87); binary or V == NULL: row length as float

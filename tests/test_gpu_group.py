@@ -193,3 +193,21 @@ def test_group_errors(ctx):
             g.build(density=0.0)
     finally:
         g.close()
+
+
+def test_corr_packed_and_select_density(ctx):
+    # bna_corr_packed: CorMat_gpu + post_block's packed matrix; bna_select_density:
+    # CorMat_spa2rth's r*, here exact (the k-th largest r64)
+    N = 900
+    x = subjects(N, [140], 0xB17A7300, dead={0: [11]})[0]
+    ref = oracle_packed(x)
+    with ctx.series(x) as s:
+        p = s.corr_packed("bf16x3")
+        exp = np.where(np.isnan(ref), 0.0, ref)
+        assert np.max(np.abs(p - exp)) <= AVG_TOL
+        r_star, k = s.select_density(2.5)
+    z64, _, _, _, valid = orc.znorm(x, 0)
+    kk = orc.density_k(N, np.float32(2.5))
+    og = orc.build_density(z64, valid, kk)
+    assert k == kk
+    assert r_star == og.r_star
