@@ -14,8 +14,8 @@
 // (workgroup b runs on XCD b % 8), so tiles that share operand panels run on the
 // same L2; a workgroup whose range is drained takes tiles from the others.  Each
 // workgroup knows its next tile one tile ahead, and the operand stream runs on
-// across the tile boundary: the last three stages of a tile fetch the next
-// tile's first three stages, so those loads land while the epilogue runs.  A batch of
+// across the tile boundary: the last four stages of a tile fetch the next
+// tile's first four stages, so those loads land while the epilogue runs.  A batch of
 // subjects (different N, L, thresholds) shares one launch: every tile item
 // carries its subject's operand panel and stage count.
 //
@@ -26,9 +26,12 @@
 // voxel row r there is one UNIT-byte record at ((kb * N_pad) + r) * UNIT, so
 // one stage of a 256-row panel is a contiguous 256 * UNIT bytes.  Stages move
 // global -> LDS with global_load_lds_dwordx4 (no registers in between) into a
-// ring of 4 slots; the load for stream position p + 3 is issued while position
-// p is multiplied, so every load has two to three stages of MFMA work to
-// arrive in.
+// ring of 4 slots.  Stage p's fragments are read from LDS into registers while
+// stage p - 1 is multiplied, and the load for stream position p + 4 is issued
+// when stage p's multiply starts, so every load has about three stages of MFMA
+// work to arrive in and no LDS read waits in front of an MFMA.  (The dense
+// accumulation pass keeps the one-set form: position p + 3 while p is read and
+// multiplied.)
 //
 // Precisions (template PREC), all accumulating in FP32:
 //   0 FP32    v_mfma_f32_32x32x2_f32,   UNIT 64: z[16] fp32
@@ -76,6 +79,7 @@ struct CorrCfg {
     static constexpr int LIST_LDS = (LIST_RAW > 2048 ? 2048 : LIST_RAW) & ~7;  // kept pairs per wave
 };
 static_assert(CorrCfg<2>::LIST_LDS >= 256, "epilogue lists do not fit beside the ring");
+static_assert(LPAD_MIN >= 4 * KPAD, "the operand stream fetches four stages of the next tile");
 static_assert(CorrCfg<0>::RING + EPI_FIXED + CTL_BYTES <= LDS_MAX, "LDS layout");
 static_assert(HIST_BINS * 4 <= EPI_FIXED + 4 * 8 * CorrCfg<2>::LIST_LDS, "histogram region");
 
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     int g = 0;  // stream position of the current tile's stage 0; stage s lives in slot (g + s) % NSLOT
 
     // Stream position s relative to the current tile: its own stage s, then the
-    // next tile's stages (s - nk < 3 < the next tile's nk) from xA/xB with
+    // next tile's stages (s - nk < 4 <= the next tile's nk) from xA/xB with
     // stride xK, or, after the last tile, a repeat of the last stage (stride 0)
     // so every wave issues the same count.
 #define BNA_DMA(S)                                                                                 \
@@ -464,11 +468,12 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     } while (0)
 
     // Fragment registers: bf16 hi and lo 8-element vectors per block; fp32 two
-    // chunks (8 consecutive k) per block for lane half h.  One set: the LDS
-    // reads of a stage are scheduled among its MFMAs by the compiler, and the
-    // registers a second set would take are what keeps the stage loop free of
-    // spills.
-    u32x4 fa[4][2], fb[4][2];
+    // chunks (8 consecutive k) per block for lane half h.  Two sets: stage kt + 1
+    // is read into one while stage kt's MFMAs run from the other, so the LDS
+    // reads never stall the matrix pipe.  (The accumulation pass keeps one set:
+    // with a second one its epilogue spills inside the stage loop.)
+    u32x4 fa0[4][2], fb0[4][2], fa1[4][2], fb1[4][2];
+    constexpr int AHEAD = (HIST == 2) ? 3 : 4;  // stream positions in flight ahead of the stage being read
     // logical chunks a lane reads: bf16 (hi h, lo h) = (h, 2 + h); bf16 plain (h);
     // fp32 (2h, 2h + 1)
     const int c0 = (PREC == 0) ? 2 * half : half;
@@ -523,7 +528,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         }                                                                                          \
     } while (0)
 
-    // Prologue of the first tile: stream positions 0..2.
+    // Prologue of the first tile: stream positions 0 .. AHEAD - 1.
     {
         const int nk = ic.nk;
         const unsigned char *cA = (const unsigned char *)ic.panel + (int64_t)ic.I * C::PANEL;
@@ -533,6 +538,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         BNA_DMA(0);
         BNA_DMA(1);
         BNA_DMA(2);
+        if constexpr (AHEAD == 4) BNA_DMA(3);
     }
 
     for (;;) {
@@ -559,17 +565,49 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         // counter proves that the stages issued before them have landed.  (Those
         // stages, this tile's first three, had the epilogue's time to arrive.)
         BNA_WAIT_VMCNT(0);
-
-        // Stage kt: wait for it (positions kt + 1 and kt + 2 may stay in flight),
-        // barrier (every wave is done with stage kt - 1, so its slot is free),
-        // refill that slot with position kt + 3, read stage kt's fragments and
-        // multiply them.
-        for (int kt = 0; kt < nk; kt++) {
-            BNA_WAIT_VMCNT(2 * C::DPS);
+        if constexpr (AHEAD == 4) {
+            // Iteration kt: stage kt's fragments are in set kt & 1.  Wait for stage
+            // kt + 1 (positions kt + 2 and kt + 3 may stay in flight; this wave's
+            // reads of stage kt are complete), barrier (every wave holds stage kt,
+            // so its slot is free), refill that slot with position kt + 4, read
+            // stage kt + 1 into the other set, multiply stage kt.  The scheduling
+            // barrier keeps the MFMAs ahead of the next wait, so the reads have a
+            // whole stage of MFMAs to land.
             __builtin_amdgcn_s_barrier();
-            BNA_DMA(kt + 3);
-            BNA_LDFRAG(fa, fb, kt);
-            BNA_MFMA(fa, fb);
+            BNA_LDFRAG(fa0, fb0, 0);
+            int kt = 0;
+            for (; kt + 1 < nk; kt += 2) {
+                BNA_WAIT_VMCNT(2 * C::DPS);
+                __builtin_amdgcn_s_barrier();
+                BNA_DMA(kt + 4);
+                BNA_LDFRAG(fa1, fb1, kt + 1);
+                BNA_MFMA(fa0, fb0);
+                __builtin_amdgcn_sched_barrier(0);
+                BNA_WAIT_VMCNT(2 * C::DPS);
+                __builtin_amdgcn_s_barrier();
+                BNA_DMA(kt + 5);
+                if (kt + 2 < nk) BNA_LDFRAG(fa0, fb0, kt + 2);
+                BNA_MFMA(fa1, fb1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (nk & 1) {  // the last stage, nk - 1, is in set 0
+                BNA_WAIT_VMCNT(2 * C::DPS);
+                __builtin_amdgcn_s_barrier();
+                BNA_DMA(kt + 4);
+                BNA_MFMA(fa0, fb0);
+            }
+        } else {
+            // Stage kt: wait for it (positions kt + 1 and kt + 2 may stay in
+            // flight), barrier (every wave is done with stage kt - 1, so its slot
+            // is free), refill that slot with position kt + 3, read stage kt's
+            // fragments and multiply them.
+            for (int kt = 0; kt < nk; kt++) {
+                BNA_WAIT_VMCNT(2 * C::DPS);
+                __builtin_amdgcn_s_barrier();
+                BNA_DMA(kt + 3);
+                BNA_LDFRAG(fa0, fb0, kt);
+                BNA_MFMA(fa0, fb0);
+            }
         }
 
         // ------------------------------------------------------------ epilogue
