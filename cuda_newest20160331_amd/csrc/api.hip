@@ -1024,7 +1024,7 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
             const int64_t per = (int64_t)std::max(64.0, s->N * (ctx->chunk_ms * REFINE_BYTES_PER_MS / bytes));
             if ((rc = run_chunks(ctx, s->N, per, [&](int64_t r0, int64_t n) {
                      return launch_refine(j.R32, j.Cd, j.Vd, s->xv ? s->xv : s->x, s->xv ? s->L_pad : s->L, s->mu,
-                                          s->sd, s->N, s->L, (int)r0, (int)(r0 + n), ctx->stream);
+                                          s->sd, s->N, s->L, (int)r0, (int)(r0 + n), j.nnz, ctx->stream);
                  })))
                 return rc;
         }

@@ -154,9 +154,10 @@ hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, cons
 hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32, int64_t *bsum, hipStream_t st);
 int scan_blocks(int N);
 // Recomputes every value of a weighted CSR in double from the voxel-major series.
-// Rows [row0, row1) of the CSR.
+// Rows [row0, row1) of the CSR; nnz is the whole CSR's entry count (sizes the grid).
 hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
-                         const double *mu, const double *sd, int N, int L, int row0, int row1, hipStream_t st);
+                         const double *mu, const double *sd, int N, int L, int row0, int row1, int64_t nnz,
+                         hipStream_t st);
 // frag_* here are the row-major tables.
 hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, const int64_t *frag_off,
                         const Entry *entries, const int64_t *R64, int N, int nb, int32_t *C,

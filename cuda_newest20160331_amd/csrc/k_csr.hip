@@ -12,6 +12,8 @@
 // is a copy with the recheck's rejected entries squeezed out.
 #include "common.hpp"
 
+#include <algorithm>
+
 namespace bna {
 
 // One wave per candidate.  z[l] = (x[l] - mean) / norm is recomputed in double,
@@ -388,17 +390,24 @@ __global__ __launch_bounds__(256) void refine_kernel(const int32_t *__restrict__
     }
 }
 
-// Row-resident form of the refinement for 16-byte aligned rows of at most
-// 256*T floats.  The wave keeps row i as double z_i = (x_i - mean_i) / norm_i in
-// registers (T pieces of four per lane), so only the rows j are read: four
-// entries at a time, each as 64-lane 16-byte pieces, all T*4 loads issued before
-// any is used.  The sum is taken over the raw x_j,
+// Chunked, row-resident form of the refinement for 16-byte aligned rows of at
+// most 256*T floats.  The CSR entries of rows [row0, row1) are cut into chunks of
+// REFINE_CHUNK entries and each wave takes chunks in turn, so a row with
+// thousands of neighbours is spread over many waves instead of keeping one busy
+// after the others are done.  For each row segment of its chunk the wave keeps
+// row i as double z_i = (x_i - mean_i) / norm_i in registers (T pieces of four
+// per lane), so only the rows j are read: four entries at a time, each as
+// 64-lane 16-byte pieces, all T*4 loads issued before any is used.  The sum is
+// taken over the raw x_j,
 //   r = (sum_l z_i[l] x_j[l] - mean_j sum_l z_i[l]) / norm_j,
 // which equals sum_l z_i[l] z_j[l] and needs one conversion and one FMA per
 // element.  The four sums are folded across the wave with a butterfly that
 // leaves entry b's total in lanes 16b..16b+15, the group that then finds and
 // writes its mirror.  Every double sum has relative error below 1e-13, so the
-// stored float differs from r64 by its own rounding only.
+// stored float differs from r64 by its own rounding only.  Only entries with
+// j > i are computed; their mirrors (j, i) are written with them.
+constexpr int REFINE_CHUNK = 128;
+
 template <int T>
 __global__ __launch_bounds__(256) void refine_rows_kernel(const int32_t *__restrict__ R, const int32_t *__restrict__ C,
                                                           float *__restrict__ V, const float *__restrict__ xs,
@@ -410,96 +419,118 @@ __global__ __launch_bounds__(256) void refine_rows_kernel(const int32_t *__restr
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, gl = lane & 15, gbase = g * 16;
     const uint64_t gmask = 0xffffull << gbase;
-    const int64_t i = row0 + __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-    if (i >= row1) return;
-    const int64_t r0 = R[i], r1 = R[i + 1];
-    if (r0 == r1) return;
-    int64_t e0 = r1;
-    for (int64_t b = r0; b < r1; b += 64) {
-        const uint64_t m = __ballot(b + lane < r1 && C[b + lane] > (int)i);
-        if (m) {
-            e0 = b + __builtin_ctzll(m);
-            break;
-        }
-    }
-    if (e0 == r1) return;
-    const int L4 = L >> 2, Lt = L & 3;
-    const float *xi = xs + i * ld;
-    const double mi = mu[i], ii = 1.0 / sd[i];
-    double zi[T][4], zt = 0.0, s = 0.0;
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-        const int q = lane + 64 * t;
-        f4 a = {0.f, 0.f, 0.f, 0.f};
-        if (q < L4) a = *(const f4 *)(xi + 4 * q);
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            zi[t][c] = q < L4 ? ((double)a[c] - mi) * ii : 0.0;
-            s += zi[t][c];
-        }
-    }
-    if (lane < Lt) {
-        zt = ((double)xi[4 * L4 + lane] - mi) * ii;
-        s += zt;
-    }
-#pragma unroll
-    for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k, 64);
     const bool h5 = lane & 32, h4 = lane & 16;
-    for (int64_t e = e0; e < r1; e += EB) {
-        int j[EB];
-        f4 xj[EB][T];
-#pragma unroll
-        for (int b = 0; b < EB; b++) {
-            j[b] = e + b < r1 ? C[e + b] : -1;
-            const float *xr = xs + (int64_t)max(j[b], 0) * ld;
+    const int L4 = L >> 2, Lt = L & 3;
+    const int64_t base = R[row0], end = R[row1];
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t w0 = __builtin_amdgcn_readfirstlane((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    for (int64_t w = w0;; w += nwaves) {
+        const int64_t a = base + w * REFINE_CHUNK;
+        if (a >= end) break;
+        const int64_t b = min(end, a + REFINE_CHUNK);
+        // the row of entry a: R[lo] <= a < R[hi], hi = lo + 1
+        int lo = row0, hi = row1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)R[mid] <= a) lo = mid;
+            else hi = mid;
+        }
+        int64_t s0 = a;
+        for (int64_t i = lo; s0 < b; i++) {
+            const int64_t s1 = min(b, (int64_t)R[i + 1]);
+            // first entry of the segment with column > i (a row's columns ascend)
+            int64_t e0 = s1;
+            for (int64_t bb = s0; bb < s1; bb += 64) {
+                const uint64_t m = __ballot(bb + lane < s1 && C[bb + lane] > (int)i);
+                if (m) {
+                    e0 = bb + __builtin_ctzll(m);
+                    break;
+                }
+            }
+            s0 = s1;
+            if (e0 == s1) continue;
+            const float *xi = xs + i * ld;
+            const double mi = mu[i], ii = 1.0 / sd[i];
+            double zi[T][4], zt = 0.0, s = 0.0;
 #pragma unroll
             for (int t = 0; t < T; t++) {
                 const int q = lane + 64 * t;
-                xj[b][t] = (f4){0.f, 0.f, 0.f, 0.f};
-                if (j[b] >= 0 && q < L4) xj[b][t] = *(const f4 *)(xr + 4 * q);
+                f4 x4 = {0.f, 0.f, 0.f, 0.f};
+                if (q < L4) x4 = *(const f4 *)(xi + 4 * q);
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    zi[t][c] = q < L4 ? ((double)x4[c] - mi) * ii : 0.0;
+                    s += zi[t][c];
+                }
             }
-        }
-        double acc[EB];
+            if (lane < Lt) {
+                zt = ((double)xi[4 * L4 + lane] - mi) * ii;
+                s += zt;
+            }
 #pragma unroll
-        for (int b = 0; b < EB; b++) {
-            double a = 0.0;
-            if (lane < Lt && j[b] >= 0) a = zt * (double)xs[(int64_t)j[b] * ld + 4 * L4 + lane];
+            for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k, 64);
+            for (int64_t e = e0; e < s1; e += EB) {
+                int j[EB];
+                f4 xj[EB][T];
 #pragma unroll
-            for (int t = 0; t < T; t++)
+                for (int q4 = 0; q4 < EB; q4++) {
+                    j[q4] = e + q4 < s1 ? C[e + q4] : -1;
+                    const float *xr = xs + (int64_t)max(j[q4], 0) * ld;
 #pragma unroll
-                for (int c = 0; c < 4; c++) a = __builtin_fma(zi[t][c], (double)xj[b][t][c], a);
-            acc[b] = a;
-        }
-        // fold: after xor 32 lane half h5 holds entries {2h5, 2h5+1}; after xor
-        // 16 quarter (h5, h4) holds entry 2h5 + h4; xor 8..1 finish the sum
-        double v0 = (h5 ? acc[2] : acc[0]) + __shfl_xor(h5 ? acc[0] : acc[2], 32, 64);
-        double v1 = (h5 ? acc[3] : acc[1]) + __shfl_xor(h5 ? acc[1] : acc[3], 32, 64);
-        double w = (h4 ? v1 : v0) + __shfl_xor(h4 ? v0 : v1, 16, 64);
+                    for (int t = 0; t < T; t++) {
+                        const int q = lane + 64 * t;
+                        xj[q4][t] = (f4){0.f, 0.f, 0.f, 0.f};
+                        if (j[q4] >= 0 && q < L4) xj[q4][t] = *(const f4 *)(xr + 4 * q);
+                    }
+                }
+                double acc[EB];
 #pragma unroll
-        for (int k = 8; k >= 1; k >>= 1) w += __shfl_xor(w, k, 64);
-        const int jg = g == 0 ? j[0] : g == 1 ? j[1] : g == 2 ? j[2] : j[3];
-        const int64_t q = jg >= 0 ? group_find_col(C, R[jg], R[jg + 1], (int)i, gl, gmask, gbase)
-                                  : group_find_col(C, 0, 0, 0, gl, gmask, gbase);
-        if (jg >= 0 && gl == 0) {
-            const float v = (float)((w - mu[jg] * s) / sd[jg]);
-            V[e + g] = v;
-            if (q >= 0) V[q] = v;
+                for (int q4 = 0; q4 < EB; q4++) {
+                    double av = 0.0;
+                    if (lane < Lt && j[q4] >= 0) av = zt * (double)xs[(int64_t)j[q4] * ld + 4 * L4 + lane];
+#pragma unroll
+                    for (int t = 0; t < T; t++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) av = __builtin_fma(zi[t][c], (double)xj[q4][t][c], av);
+                    acc[q4] = av;
+                }
+                // fold: after xor 32 lane half h5 holds entries {2h5, 2h5+1}; after xor
+                // 16 quarter (h5, h4) holds entry 2h5 + h4; xor 8..1 finish the sum
+                double v0 = (h5 ? acc[2] : acc[0]) + __shfl_xor(h5 ? acc[0] : acc[2], 32, 64);
+                double v1 = (h5 ? acc[3] : acc[1]) + __shfl_xor(h5 ? acc[1] : acc[3], 32, 64);
+                double wv = (h4 ? v1 : v0) + __shfl_xor(h4 ? v0 : v1, 16, 64);
+#pragma unroll
+                for (int k = 8; k >= 1; k >>= 1) wv += __shfl_xor(wv, k, 64);
+                const int jg = g == 0 ? j[0] : g == 1 ? j[1] : g == 2 ? j[2] : j[3];
+                const int64_t q = jg >= 0 ? group_find_col(C, R[jg], R[jg + 1], (int)i, gl, gmask, gbase)
+                                          : group_find_col(C, 0, 0, 0, gl, gmask, gbase);
+                if (jg >= 0 && gl == 0) {
+                    const float v = (float)((wv - mu[jg] * s) / sd[jg]);
+                    V[e + g] = v;
+                    if (q >= 0) V[q] = v;
+                }
+            }
         }
     }
 }
 
 hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
-                         const double *mu, const double *sd, int N, int L, int row0, int row1, hipStream_t st)
+                         const double *mu, const double *sd, int N, int L, int row0, int row1, int64_t nnz,
+                         hipStream_t st)
 {
     if (row0 < 0 || row1 > N || row0 >= row1) return row0 == row1 ? hipSuccess : hipErrorInvalidValue;
     const int64_t threads = (int64_t)(row1 - row0) * 64;
     const dim3 grid((unsigned)((threads + 255) / 256));
+    // chunked form: one wave per chunk of the whole CSR; the waves of a launch
+    // over fewer rows find their chunks gone and leave
+    const int64_t cw = std::max<int64_t>(4, (nnz + REFINE_CHUNK - 1) / REFINE_CHUNK);
+    const dim3 cgrid((unsigned)((cw + 3) / 4));
     // 16-byte loads need 16-byte aligned rows
     const bool vec = ld % 4 == 0 && ((uintptr_t)xs & 15) == 0;
     const int T = ((L >> 2) + 63) / 64;
     if (vec && T >= 1 && T <= 8) {
 #define BNA_REFINE_ROWS(TT) \
-    case TT: hipLaunchKernelGGL(refine_rows_kernel<TT>, grid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, L, row0, row1); break;
+    case TT: hipLaunchKernelGGL(refine_rows_kernel<TT>, cgrid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, L, row0, row1); break;
         switch (T) {
             BNA_REFINE_ROWS(1) BNA_REFINE_ROWS(2) BNA_REFINE_ROWS(3) BNA_REFINE_ROWS(4)
             BNA_REFINE_ROWS(5) BNA_REFINE_ROWS(6) BNA_REFINE_ROWS(7) BNA_REFINE_ROWS(8)
