@@ -568,7 +568,7 @@ class Result(C.Structure):
                 ("lam", C.c_double), ("ec_iters", C.c_int), ("batch_size", C.c_int),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
                 ("upload_ms", C.c_double), ("build_ms", C.c_double), ("maps_ms", C.c_double),
-                ("download_ms", C.c_double), ("release_ms", C.c_double), ("build", Timings)]
+                ("download_ms", C.c_double), ("build", Timings)]
 
 
 class ServerStats(C.Structure):
@@ -593,7 +593,7 @@ class Served:
         self.batch_size = r.batch_size
         self.queue_ms, self.run_ms, self.latency_ms = r.queue_ms, r.run_ms, r.latency_ms
         self.phases_ms = {"upload": r.upload_ms, "build": r.build_ms, "maps": r.maps_ms,
-                          "download": r.download_ms, "release": r.release_ms}
+                          "download": r.download_ms}
         self.build = {k: getattr(r.build, k) for k, _ in Timings._fields_}
 
 

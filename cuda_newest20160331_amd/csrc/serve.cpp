@@ -102,14 +102,6 @@ extern "C" int64_t bna_request_footprint(int N, int L, const bna_opts *o)
     return b;
 }
 
-static void finish(bna_server *srv, const std::shared_ptr<Req> &r)
-{
-    r->done = true;
-    srv->reserved -= r->footprint;
-    srv->st.completed++;
-    r->x.clear();
-    r->x.shrink_to_fit();
-}
 
 // Requests of one precision, oldest first, that fit in what the budget has
 // left; a request larger than the whole budget runs alone on an idle server.
@@ -232,20 +224,28 @@ static void run_round(bna_server *srv, bna_ctx *ctx, std::vector<std::shared_ptr
             strncpy(res.error, bna_last_error(ctx), sizeof(res.error) - 1);
             bna_csr_free(&res.csr);
         }
-        bna_csr_release(ctx, &dev[q]);
-        if (ser[q]) bna_series_destroy(ctx, ser[q]);
-        ser[q] = nullptr;
-        const Clock::time_point t1 = Clock::now();
-        res.release_ms = ms_between(t_dl, t1);
+        const Clock::time_point t1 = t_dl;
         res.queue_ms = ms_between(r.t_submit, r.t_start);
         res.run_ms = ms_between(r.t_start, t1);
         res.latency_ms = ms_between(r.t_submit, t1);
         {
             std::lock_guard<std::mutex> lk(srv->mu);
-            finish(srv, rs[q]);
+            r.done = true;  // the worker does not touch r.res from here on
+            srv->st.completed++;
             if (res.status != BNA_OK) srv->st.failed++;
         }
         srv->cv_done.notify_all();
+        // The request's device memory goes back to the context's pool after its
+        // result is out: a free can wait behind another context's
+        // synchronisation, and the caller should not wait with it.  Its footprint
+        // is returned to the budget once the memory is.
+        bna_csr_release(ctx, &dev[q]);
+        if (ser[q]) bna_series_destroy(ctx, ser[q]);
+        ser[q] = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->reserved -= rs[q]->footprint;
+        }
         srv->cv_work.notify_all();  // memory was released: more may be admitted
     }
     if (big) {

@@ -308,7 +308,6 @@ typedef struct bna_result {
     double run_ms;    /* round start -> result ready */
     double latency_ms;/* submit -> result ready */
     double upload_ms, build_ms, maps_ms, download_ms; /* phases of its round (build: the whole batch) */
-    double release_ms; /* returning its device memory to the context's pool */
     bna_timings build; /* the batch's device timings */
 } bna_result;
 

@@ -109,12 +109,16 @@ def test_admission_respects_budget():
 
 
 def test_cancel_and_poll():
+    # one worker, one request per round: a long first request (a density
+    # threshold at N = 60000, L = 1200, two tile passes and the refinement) keeps
+    # the small ones queued, so the last can be cancelled whatever the device speed
+    big = bna.synth_factor_model(60000, 1200, 8, 0.7, 76)
     x = bna.synth_factor_model(6000, 300, 8, 0.7, 77)
     with bna.Server(workers=1, max_batch=1) as srv:
-        ids = [srv.submit(x, thr=0.5) for _ in range(4)]
+        ids = [srv.submit(big, density=1.0)] + [srv.submit(x, thr=0.5) for _ in range(3)]
+        assert srv.cancel(ids[-1]) == 0
         first = srv.poll(ids[0])
         assert first == bna.Server.BUSY or first.status == 0
-        assert srv.cancel(ids[-1]) == 0
         res = srv.wait(ids[-1])
         assert res.status == bna.Server.CANCELLED
         for i in ids[1:-1]:
