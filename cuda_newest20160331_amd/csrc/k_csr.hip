@@ -308,8 +308,22 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
 constexpr int RG = 16;  // lanes per entry
 
 __device__ __forceinline__ int64_t group_find_col(const int32_t *__restrict__ C, int64_t lo, int64_t hi, int key,
-                                                  int gl, uint64_t gmask, int gbase)
+                                                  int gl, uint64_t gmask, int gbase, int N = 0)
 {
+    // first probe (N > 0): the RG entries around where key would sit if the
+    // row's columns were spread evenly over [0, N); a row's columns ascend, so
+    // the probe either holds key or tells on which side it lies
+    if (N > 0 && hi - lo > RG) {
+        int64_t p0 = lo + (int64_t)((double)(hi - lo) * (double)key / (double)N) - RG / 2;
+        p0 = p0 < lo ? lo : (p0 > hi - RG ? hi - RG : p0);
+        const int c = C[p0 + gl];
+        const uint64_t eq = (__ballot(c == key) & gmask) >> gbase;
+        if (eq) return p0 + __builtin_ctzll(eq);
+        const uint64_t le = (__ballot(c < key) & gmask) >> gbase;
+        if (le == 0) hi = p0;                      // key is below the probe
+        else if (le == 0xffffull) lo = p0 + RG;    // key is above it
+        else return -1;                            // between two neighbours: absent
+    }
     // narrow [lo, hi) by sampling RG evenly spaced entries until one load covers it
     while (hi - lo > RG) {
         const int64_t step = (hi - lo + RG - 1) / RG;
@@ -412,7 +426,8 @@ template <int T>
 __global__ __launch_bounds__(256) void refine_rows_kernel(const int32_t *__restrict__ R, const int32_t *__restrict__ C,
                                                           float *__restrict__ V, const float *__restrict__ xs,
                                                           int64_t ld, const double *__restrict__ mu,
-                                                          const double *__restrict__ sd, int L, int row0, int row1)
+                                                          const double *__restrict__ sd, int N, int L, int row0,
+                                                          int row1)
 {
     typedef float f4 __attribute__((ext_vector_type(4)));
     constexpr int EB = 4;  // entries per trip, one per 16-lane group at the end
@@ -502,7 +517,7 @@ __global__ __launch_bounds__(256) void refine_rows_kernel(const int32_t *__restr
 #pragma unroll
                 for (int k = 8; k >= 1; k >>= 1) wv += __shfl_xor(wv, k, 64);
                 const int jg = g == 0 ? j[0] : g == 1 ? j[1] : g == 2 ? j[2] : j[3];
-                const int64_t q = jg >= 0 ? group_find_col(C, R[jg], R[jg + 1], (int)i, gl, gmask, gbase)
+                const int64_t q = jg >= 0 ? group_find_col(C, R[jg], R[jg + 1], (int)i, gl, gmask, gbase, N)
                                           : group_find_col(C, 0, 0, 0, gl, gmask, gbase);
                 if (jg >= 0 && gl == 0) {
                     const float v = (float)((wv - mu[jg] * s) / sd[jg]);
@@ -530,7 +545,7 @@ hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const flo
     const int T = ((L >> 2) + 63) / 64;
     if (vec && T >= 1 && T <= 8) {
 #define BNA_REFINE_ROWS(TT) \
-    case TT: hipLaunchKernelGGL(refine_rows_kernel<TT>, cgrid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, L, row0, row1); break;
+    case TT: hipLaunchKernelGGL(refine_rows_kernel<TT>, cgrid, dim3(256), 0, st, R, C, V, xs, ld, mu, sd, N, L, row0, row1); break;
         switch (T) {
             BNA_REFINE_ROWS(1) BNA_REFINE_ROWS(2) BNA_REFINE_ROWS(3) BNA_REFINE_ROWS(4)
             BNA_REFINE_ROWS(5) BNA_REFINE_ROWS(6) BNA_REFINE_ROWS(7) BNA_REFINE_ROWS(8)
