@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void refine_kernel(const int32_t *__restrict__
         }
 #pragma unroll
         for (int s = RG / 2; s >= 1; s >>= 1) p += __shfl_xor(p, s, 64);
-        const int64_t q = act ? group_find_col(C, R[j], R[j + 1], (int)i, gl, gmask, gbase)
+        const int64_t q = act ? group_find_col(C, R[j], R[j + 1], (int)i, gl, gmask, gbase, N)
                               : group_find_col(C, 0, 0, 0, gl, gmask, gbase);
         if (act && gl == 0) {
             const float v = (float)p;

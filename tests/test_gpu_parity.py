@@ -269,3 +269,18 @@ def test_c3_full_size_properties(ctx):
         say(f"  {name}: max |V - r64| over all {len(ii)} edges = {d.max():.3g} "
             f"(> 1e-6: {int(np.sum(d > 1e-6))}), band {g.band:.3g}")
         assert d.max() <= V_TOL
+
+
+@pytest.mark.parametrize("N,L,layout", [(900, 97, bna.VOXEL_MAJOR), (600, 2100, bna.TIME_MAJOR)])
+def test_refine_general_forms(ctx, N, L, layout):
+    # rows that are not 16-byte aligned (voxel-major input, L % 4 != 0) and rows
+    # longer than the register-resident form holds (L > 2048) take the general
+    # refinement kernel; its values and mirror entries meet the same bar
+    say(f"refine general form N={N} L={L} layout={layout}")
+    x = bna.synth_factor_model(N, L, 8, 0.7, 0xB17A0000 + 7000 + L, layout)
+    z64, _, _, _, valid = orc.znorm(x, layout)
+    og = orc.build_threshold(z64, valid, 0.3)
+    g = ctx.build_csr(x, layout, thr=0.3)
+    err = compare(g, og)
+    # refined values are the double sums rounded to float: half a float ulp of r
+    assert err <= 6e-8, f"max |V - r64| = {err}"
