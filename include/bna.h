@@ -194,7 +194,8 @@ int bna_group_finish(bna_ctx *ctx, bna_group *g);
 int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed);
 /* Thresholds the averages: opts->thr_type, thr and weighted are used.  out gets
  * a device CSR; thr_used is the r threshold or, for a density, the k-th
- * largest average. */
+ * largest average.  The context keeps an N x N byte keep matrix (rows padded to
+ * 64 bytes; 2 GB at N = 45000) in its workspace until bna_trim. */
 int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *opts, bna_csr *out);
 
 /* The packed r matrix of one subject, M1 floats in row-major upper-triangle
