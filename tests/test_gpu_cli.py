@@ -174,3 +174,15 @@ def test_cormat_both_modes_with_subject_cormat(subjects):
     g = bna.read_csr(files[0])
     k = orc.density_k(N, np.float32(2.0))
     assert g.nnz == 2 * k and g.V is not None
+
+
+def test_cormat_group_binary(subjects):
+    d, mask, Ls = subjects
+    say("cli: group average, binary graph")
+    run("bna_cormat", "--binary", d, 0.5, "yn", "n", "s", 3.0)
+    x, N = ingest(d, sorted(Ls)[0])
+    base = f"{d.name}_{len(Ls)}_{N}"
+    files = sorted(d.glob(f"{base}_spa3.000%_cor*.csr"))
+    assert len(files) == 1, sorted(p.name for p in d.iterdir())
+    g = bna.read_csr(files[0])
+    assert g.V is None and g.nnz == 2 * orc.density_k(N, np.float32(3.0))

@@ -211,3 +211,23 @@ def test_corr_packed_and_select_density(ctx):
     og = orc.build_density(z64, valid, kk)
     assert k == kk
     assert r_star == og.r_star
+
+
+@pytest.mark.parametrize("N", [2, 50, 257])
+def test_group_small_and_ragged(ctx, N):
+    # one diagonal tile, one partial keep block, and a tile edge just past 256
+    say(f"group small N={N}")
+    xs = subjects(N, [40, 33], 0xB17A7400 + N)
+    ref = orc.group_average([oracle_packed(x) for x in xs], N, True)
+    g = gpu_group(ctx, xs, N, True, "bf16x3")
+    try:
+        g.finish()
+        avg = g.packed()
+        assert avg.shape == ref.shape and np.max(np.abs(avg - ref), initial=0.0) <= AVG_TOL_Z
+        G = g.build(-2.0, weighted=False)  # every pair
+        assert G.nnz == N * (N - 1) and G.V is None
+        assert np.array_equal(G.R, np.arange(N + 1) * (N - 1))
+        cols = np.concatenate([np.delete(np.arange(N), i) for i in range(N)]) if N > 1 else np.zeros(0)
+        assert np.array_equal(G.C, cols)
+    finally:
+        g.close()
