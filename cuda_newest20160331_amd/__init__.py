@@ -37,6 +37,7 @@ EXPORTS = [
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
+    "bna_build_csr_into",
 ]
 
 
@@ -129,6 +130,7 @@ def lib() -> C.CDLL:
         "bna_write_cormat": (i32, [C.c_char_p, fp, i32]),
         "bna_nm_to_nifti": (i32, [C.c_char_p, dbl, fp, i32, C.c_char_p]),
         "bna_corr_packed": (i32, [vp, vp, i32, fp]),
+        "bna_build_csr_into": (i32, [vp, vp, C.POINTER(Opts), vp, vp, vp, i64, C.POINTER(C.c_int64)]),
         "bna_select_density": (i32, [vp, vp, i32, dbl, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
@@ -437,6 +439,18 @@ class Series:
 
     def free_device(self, g: CSR):
         self.ctx._L.bna_csr_release(self.ctx._h, C.byref(g))
+
+    def build_into(self, R, Cc, V, **kw) -> int:
+        """Builds into caller arrays (numpy int32 R[N+1], int32 C[cap], float32 V[cap]
+        or None); R=None is a size query.  Returns nnz."""
+        o = make_opts(**kw)
+        nnz = C.c_int64()
+        cap = 0 if Cc is None else int(Cc.size)
+        self.ctx._check(self.ctx._L.bna_build_csr_into(
+            self.ctx._h, self._h, C.byref(o), None if R is None else R.ctypes.data,
+            None if Cc is None else Cc.ctypes.data, None if V is None else V.ctypes.data, cap, C.byref(nnz)),
+            "bna_build_csr_into")
+        return nnz.value
 
     def corr_packed(self, precision: str = "bf16x3") -> np.ndarray:
         """The packed r matrix (M1 floats; 0 outside [-1, 1] and for invalid voxels)."""

@@ -1635,6 +1635,28 @@ int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed)
     return copy_d2h(ctx, packed, g->acc, (size_t)g->M1 * 4);
 }
 
+int bna_build_csr_into(bna_ctx *ctx, bna_series *s, const bna_opts *o, int32_t *R, int32_t *C, float *V, int64_t cap,
+                       int64_t *nnz)
+{
+    if (!ctx || !s || !o || !nnz) return fail(ctx, BNA_ERR_ARG, "bna_build_csr_into: NULL argument");
+    *nnz = 0;
+    bna_csr g{};
+    int rc = bna_build_csr_series(ctx, s, o, &g);
+    if (rc) return rc;
+    *nnz = g.nnz;
+    if (!R) return bna_csr_release(ctx, &g);  // size query
+    if (!C || cap < g.nnz || (g.V && !V)) {
+        (void)bna_csr_release(ctx, &g);
+        return fail(ctx, BNA_ERR_OVERFLOW, "bna_build_csr_into: " + std::to_string(g.nnz) +
+                                               " entries, capacity " + std::to_string(cap));
+    }
+    rc = copy_d2h(ctx, R, g.R, sizeof(int32_t) * ((size_t)g.N + 1));
+    if (!rc && g.nnz) rc = copy_d2h(ctx, C, g.C, sizeof(int32_t) * (size_t)g.nnz);
+    if (!rc && g.nnz && g.V) rc = copy_d2h(ctx, V, g.V, sizeof(float) * (size_t)g.nnz);
+    const int rr = bna_csr_release(ctx, &g);
+    return rc ? rc : rr;
+}
+
 int bna_corr_packed(bna_ctx *ctx, bna_series *s, int precision, float *packed)
 {
     if (!ctx || !s || !packed) return fail(ctx, BNA_ERR_ARG, "bna_corr_packed: NULL argument");

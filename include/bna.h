@@ -150,6 +150,14 @@ int bna_series_destroy(bna_ctx *ctx, bna_series *s);
 int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision);
 /* Correlate + threshold + CSR on a normalised series; out->on_device = 1. */
 int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *opts, bna_csr *out);
+/* Caller-owned buffers, the reference's calling convention (CorMat.cpp:250,
+ * caller-allocated output): builds as bna_build_csr_series and copies the CSR
+ * into R (N + 1 ints), C and V (cap entries each; V may be NULL for a binary
+ * graph); *nnz gets the entry count.  R == NULL is a size query (nothing is
+ * copied; a later call builds again).  cap < nnz: BNA_ERR_OVERFLOW, nothing
+ * copied.  A density's nnz is known beforehand: 2 * bna_density_to_k(N, s). */
+int bna_build_csr_into(bna_ctx *ctx, bna_series *s, const bna_opts *opts, int32_t *R, int32_t *C, float *V,
+                       int64_t cap, int64_t *nnz);
 /* Several subjects in one pass of batched launches (the batched form of the
  * reference's per-subject loop, CorMat.cpp:195-384): every subject's tiles go
  * through one histogram launch (density subjects) and one emission launch of

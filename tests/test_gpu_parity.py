@@ -284,3 +284,24 @@ def test_refine_general_forms(ctx, N, L, layout):
     err = compare(g, og)
     # refined values are the double sums rounded to float: half a float ulp of r
     assert err <= 6e-8, f"max |V - r64| = {err}"
+
+
+def test_build_into_caller_buffers(ctx):
+    # the caller-allocated form: a size query, a too-small buffer, then the copy
+    N, L = 1500, 120
+    x = bna.synth_factor_model(N, L, 8, 0.7, 0xB17A0000 + 7100)
+    with ctx.series(x) as s:
+        ref = s.build(thr=0.4)
+        nnz = s.build_into(None, None, None, thr=0.4)
+        assert nnz == ref.nnz
+        R = np.empty(N + 1, np.int32)
+        small = np.empty(max(nnz - 1, 0), np.int32)
+        with pytest.raises(bna.BnaError):
+            s.build_into(R, small, np.empty(small.size, np.float32), thr=0.4)
+        Cc = np.empty(nnz, np.int32)
+        V = np.empty(nnz, np.float32)
+        assert s.build_into(R, Cc, V, thr=0.4) == nnz
+        assert np.array_equal(R.astype(np.int64), ref.R) and np.array_equal(Cc, ref.C) and np.array_equal(V, ref.V)
+        k = bna.density_to_k(N, 2.0)
+        Cb = np.empty(2 * k, np.int32)
+        assert s.build_into(R, Cb, None, density=2.0, weighted=False) == 2 * k
