@@ -301,10 +301,11 @@ __global__ __launch_bounds__(256) void place_kernel(const int32_t *__restrict__ 
 // Value refinement of a weighted CSR: every kept pair's value is recomputed as
 // V = (float) sum_l z_i[l] z_j[l] with z = (x - mean) * (1 / norm) in double,
 // so |V - r64| stays within half a float ulp of r plus ~1e-15, whatever the MFMA
-// accumulation did.  One wave per row i takes the entries j > i, four at a time:
-// each 16-lane group owns one entry, reads row j in 16-byte pieces (row i's
-// pieces come from cache), sums in double and writes the value to entry (i, j)
-// and to its mirror (j, i), which the group finds in row j's sorted columns.
+// accumulation did.  In the general form below one wave per row i takes the
+// entries j > i, four at a time: each 16-lane group owns one entry, reads row j
+// in 16-byte pieces (row i's pieces come from cache), sums in double and writes
+// the value to entry (i, j) and to its mirror (j, i), which the group finds in
+// row j's sorted columns.
 constexpr int RG = 16;  // lanes per entry
 
 __device__ __forceinline__ int64_t group_find_col(const int32_t *__restrict__ C, int64_t lo, int64_t hi, int key,

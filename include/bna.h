@@ -343,9 +343,10 @@ void bna_result_free(bna_result *r);
  * lists and CSR (edges: the density target, or 2 % of pairs for an r threshold). */
 int64_t bna_request_footprint(int N, int L, const bna_opts *o);
 /* Long kernels of later builds (tile passes, value refinement) are issued in
- * pieces of about chunk_ms with at most two in flight, and gate(gate_arg) is
- * called before each piece (it may block, e.g. while a small request runs on
- * another context).  chunk_ms = 0 restores single launches. */
+ * pieces of about chunk_ms; with a gate, each piece waits for the previous one
+ * to finish and then for gate(gate_arg) (which may block, e.g. while a small
+ * request runs on another context); without one, two pieces stay in flight.
+ * chunk_ms = 0 restores single launches. */
 int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *gate_arg);
 /* Caps the workgroups of later correlation launches (the persistent tile kernel
  * runs one per CU) at max_workgroups, rounded down to a multiple of 8; 0 lifts
