@@ -58,6 +58,14 @@ struct bna_ctx {
     // pinned double buffer for host <-> device copies of pageable memory
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stev[2] = {};
+    // pinned buffers of the build path, so none of its copies goes through the
+    // runtime's staging of pageable memory (shared by every thread of the
+    // process): [0] subject descriptors and [1] batch tile tables on the way in,
+    // each guarded by the event of its last copy; [2] counters and histograms on
+    // the way out, used between two stream synchronisations
+    void *pin[3] = {nullptr, nullptr, nullptr};
+    size_t pin_bytes[3] = {0, 0, 0};
+    hipEvent_t pin_ev[2] = {};
 };
 
 struct bna_series {
@@ -177,15 +185,46 @@ static int stage_ready(bna_ctx *ctx)
     return BNA_OK;
 }
 
+// Grows pinned buffer `slot` to at least `bytes` (the caller has made sure no
+// copy from or to it is in flight).
+static int pin_grow(bna_ctx *ctx, int slot, size_t bytes)
+{
+    if (ctx->pin_bytes[slot] >= bytes) return BNA_OK;
+    if (ctx->pin[slot]) {
+        HIPCHK(ctx, hipHostFree(ctx->pin[slot]));
+        ctx->pin[slot] = nullptr;
+        ctx->pin_bytes[slot] = 0;
+    }
+    const size_t want = std::max(bytes, (size_t)64 << 10);
+    HIPCHK(ctx, hipHostMalloc(&ctx->pin[slot], want, hipHostMallocDefault));
+    ctx->pin_bytes[slot] = want;
+    return BNA_OK;
+}
+
+// Host -> device copy through pinned buffer `slot` (0 or 1), queued on the
+// context's stream; the buffer is reused only after its previous copy is done.
+static int h2d_pinned(bna_ctx *ctx, int slot, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx->pin_ev[slot]) HIPCHK(ctx, hipEventCreateWithFlags(&ctx->pin_ev[slot], hipEventDisableTiming));
+    else HIPCHK(ctx, hipEventSynchronize(ctx->pin_ev[slot]));
+    int rc = pin_grow(ctx, slot, bytes);
+    if (rc) return rc;
+    memcpy(ctx->pin[slot], src, bytes);
+    HIPCHK(ctx, hipMemcpyAsync(dst, ctx->pin[slot], bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->pin_ev[slot], ctx->stream));
+    return BNA_OK;
+}
+
 static int copy_h2d(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
-    if (bytes < ((size_t)1 << 20)) {
-        HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    int rc = stage_ready(ctx);
+    if (rc) return rc;
+    if (bytes < ((size_t)1 << 20)) {  // one piece through the pinned stage
+        memcpy(ctx->stage[0], src, bytes);
+        HIPCHK(ctx, hipMemcpyAsync(dst, ctx->stage[0], bytes, hipMemcpyHostToDevice, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         return BNA_OK;
     }
-    int rc = stage_ready(ctx);
-    if (rc) return rc;
     size_t k = 0;
     for (size_t off = 0; off < bytes; off += STAGE_BYTES, k++) {
         const int b = (int)(k & 1);
@@ -203,13 +242,14 @@ static int copy_h2d(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
 
 static int copy_d2h(bna_ctx *ctx, void *dst, const void *src, size_t bytes)
 {
-    if (bytes < ((size_t)1 << 20)) {
-        HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        return BNA_OK;
-    }
     int rc = stage_ready(ctx);
     if (rc) return rc;
+    if (bytes < ((size_t)1 << 20)) {  // one piece through the pinned stage
+        HIPCHK(ctx, hipMemcpyAsync(ctx->stage[0], src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy(dst, ctx->stage[0], bytes);
+        return BNA_OK;
+    }
     const size_t nch = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
     auto issue = [&](size_t k) -> hipError_t {
         const size_t off = k * STAGE_BYTES, n = std::min(STAGE_BYTES, bytes - off);
@@ -379,7 +419,10 @@ int bna_destroy(bna_ctx *ctx)
         if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
         if (ctx->stev[b]) (void)hipEventDestroy(ctx->stev[b]);
         if (ctx->chev[b]) (void)hipEventDestroy(ctx->chev[b]);
+        if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
     }
+    for (int b = 0; b < 3; b++)
+        if (ctx->pin[b]) (void)hipHostFree(ctx->pin[b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BNA_OK;
@@ -611,8 +654,7 @@ static int get_tile_table(bna_ctx *ctx, int nb, const void *panel, int64_t kstri
     TileItem *d = nullptr;
     int rc = ws_get_t(ctx, "tiles", t.size(), &d);
     if (rc) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(d, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = copy_h2d(ctx, d, t.data(), t.size() * sizeof(TileItem)))) return rc;
     ctx->tt_nb = nb;
     ctx->tt_panel = panel;
     ctx->tt_kstride = kstride;
@@ -822,7 +864,7 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
     for (Job *j : js) hd.push_back(j->h);
     int rc;
     if ((rc = ws_get_t(ctx, "batch_subjdesc", hd.size(), dsd))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(*dsd, hd.data(), hd.size() * sizeof(SubjDesc), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = h2d_pinned(ctx, 0, *dsd, hd.data(), hd.size() * sizeof(SubjDesc)))) return rc;
     auto kstride = [](const Job *j) { return (int64_t)j->s->N_pad * panel_unit(j->o.precision); };
     auto band = [&](const Job *j) { return ctx->band[j->o.precision][l_bucket(j->s->L_pad)]; };
     if (js.size() == 1)  // cached
@@ -833,7 +875,7 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
         make_tile_table(js[q]->s->N_pad / TILE, (int)q, js[q]->s->panel, kstride(js[q]), js[q]->s->L_pad / 16,
                         band(js[q]), t);
     if ((rc = ws_get_t(ctx, "batch_tiles", t.size(), tiles))) return rc;
-    HIPCHK(ctx, hipMemcpyAsync(*tiles, t.data(), t.size() * sizeof(TileItem), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = h2d_pinned(ctx, 1, *tiles, t.data(), t.size() * sizeof(TileItem)))) return rc;
     *ntiles = (int64_t)t.size();
     return BNA_OK;
 }
@@ -893,10 +935,13 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         launches++;
         HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
         std::vector<unsigned long long> hh(HIST_BINS * dj.size());
+        if ((rc = pin_grow(ctx, 2, hh.size() * sizeof(unsigned long long)))) return rc;
+        unsigned long long *hp = (unsigned long long *)ctx->pin[2];
         for (size_t q = 0; q < dj.size(); q++)
-            HIPCHK(ctx, hipMemcpyAsync(hh.data() + q * HIST_BINS, dj[q]->hist, HIST_BINS * sizeof(unsigned long long),
+            HIPCHK(ctx, hipMemcpyAsync(hp + q * HIST_BINS, dj[q]->hist, HIST_BINS * sizeof(unsigned long long),
                                        hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy(hh.data(), hp, hh.size() * sizeof(unsigned long long));
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ms_corr += ms;
@@ -957,22 +1002,37 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
                                        j->r64c, ctx->stream));
         }
         HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-        for (Job *j : act)
+        // per job, one pinned readback record: selection state, counters, nnz
+        constexpr size_t RB = (sizeof(SelState) + sizeof(act[0]->hc) + sizeof(int64_t) + 63) / 64 * 64;
+        if ((rc = pin_grow(ctx, 2, RB * act.size()))) return rc;
+        unsigned char *rb = (unsigned char *)ctx->pin[2];
+        for (size_t q = 0; q < act.size(); q++) {
+            Job *j = act[q];
             if (j->k > 0) {
                 HIPCHK(ctx, launch_select(j->sel, j->ctr, j->h.cands, j->r64c, j->h.cand_cap, j->k, j->s->N,
                                           j->h.frag_live, j->h.entries, j->h.entry_cap, ctx->stream));
-                HIPCHK(ctx, hipMemcpyAsync(&j->hs, j->sel, sizeof(SelState), hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(ctx, hipMemcpyAsync(rb + q * RB, j->sel, sizeof(SelState), hipMemcpyDeviceToHost, ctx->stream));
             }
+        }
         HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        for (Job *j : act) {
+        for (size_t q = 0; q < act.size(); q++) {
+            Job *j = act[q];
             const int N = j->s->N, nb = j->s->N_pad / TILE;
             HIPCHK(ctx, launch_frag_rows(j->h.frag_orig, j->h.frag_live, j->h.frag_off, N, nb, j->orig_rm, j->live_rm,
                                          j->off_rm, j->rowlen, ctx->stream));
             HIPCHK(ctx, launch_scan(j->rowlen, N, j->R64, j->R32, j->bsum, ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(j->hc, j->ctr, sizeof(j->hc), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(&j->nnz, j->R64 + N, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState), j->ctr, sizeof(j->hc), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState) + sizeof(j->hc), j->R64 + N, sizeof(int64_t),
+                                       hipMemcpyDeviceToHost, ctx->stream));
         }
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t q = 0; q < act.size(); q++) {
+            Job *j = act[q];
+            if (j->k > 0) memcpy(&j->hs, rb + q * RB, sizeof(SelState));
+            memcpy(j->hc, rb + q * RB + sizeof(SelState), sizeof(j->hc));
+            memcpy(&j->nnz, rb + q * RB + sizeof(SelState) + sizeof(j->hc), sizeof(int64_t));
+        }
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
         ms_corr += ms;
