@@ -1376,8 +1376,8 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg)
     HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
     HIPCHK(ctx, launch_degree(R, V, g->N, d, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(deg, d, sizeof(float) * (size_t)g->N, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc2 = copy_d2h(ctx, deg, d, sizeof(float) * (size_t)g->N);  // through pinned memory; synchronises
+    if (rc2) return rc2;
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
     ctx->tm.degree_ms = ms;
@@ -1404,12 +1404,16 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     if ((rc = ws_get_t(ctx, "ec_part", (size_t)3 * ec_nparts(N), &part))) return rc;
     if ((rc = ws_get(ctx, "ec_state", ec_state_bytes(), &stt))) return rc;
     int it = 0;
-    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
-    HIPCHK(ctx, run_eigcent(R, C, V, N, tol, max_iter, output, xa, xb, part, stt, outd, lambda, &it, ctx->graphs,
-                            ctx->stream));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(ec, outd, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost, ctx->stream));
+    // the iteration's state crosses through pinned memory: make sure no earlier
+    // copy through that buffer is still in flight before it is reused
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = pin_grow(ctx, 2, ec_state_bytes()))) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    HIPCHK(ctx, run_eigcent(R, C, V, N, tol, max_iter, output, xa, xb, part, stt, ctx->pin[2], outd, lambda, &it,
+                            ctx->graphs, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
+    int rc2 = copy_d2h(ctx, ec, outd, sizeof(double) * (size_t)N);  // through pinned memory; synchronises
+    if (rc2) return rc2;
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
     ctx->tm.ec_ms = ms;

@@ -168,7 +168,7 @@ hipError_t launch_degree(const int32_t *R, const float *V, int N, float *deg, hi
 // Whole power iteration (see k_graph.hip).  output: 0 last A*y, 1 last y.
 // use_graph: replay batches of 16 iterations from one captured HIP graph.
 hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, double tol, int max_iter,
-                       int output, double *xa, double *xb, double *part, void *state_dev, double *out,
+                       int output, double *xa, double *xb, double *part, void *state_dev, void *state_host, double *out,
                        double *lambda, int *iters, bool use_graph, hipStream_t st);
 size_t ec_state_bytes();
 int ec_nparts(int N);

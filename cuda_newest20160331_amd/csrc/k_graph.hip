@@ -202,14 +202,17 @@ __global__ __launch_bounds__(256) void ec_scale_kernel(const double *__restrict_
 namespace bna {
 
 // Runs the whole power iteration on stream st.  Buffers: x0/x1 (N doubles),
-// part (3 * nparts doubles), state (one EcState), out (N doubles, device).
+// part (3 * nparts doubles), state (one EcState), out (N doubles, device);
+// state_host: pinned host memory for one EcState (the state goes in and the
+// stopping flag comes back through it, not through pageable staging).
 // Returns the iteration count through *iters and the eigenvalue through *lambda.
 hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, double tol, int max_iter,
-                       int output, double *xa, double *xb, double *part, void *state_dev, double *out,
-                       double *lambda, int *iters, bool use_graph, hipStream_t st)
+                       int output, double *xa, double *xb, double *part, void *state_dev, void *state_host,
+                       double *out, double *lambda, int *iters, bool use_graph, hipStream_t st)
 {
     EcState *sd = (EcState *)state_dev;
-    EcState h{};
+    EcState &h = *(EcState *)state_host;
+    h = EcState{};
     h.tol = tol;
     h.max_iter = max_iter;
     hipError_t e = hipMemcpyAsync(sd, &h, sizeof(EcState), hipMemcpyHostToDevice, st);
