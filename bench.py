@@ -216,6 +216,13 @@ def density_and_maps(ctx, steps: int) -> dict:
             t = ctx.timings()
             if ec_ms is None or t["ec_ms"] < ec_ms:
                 ec_ms, iters = t["ec_ms"], t["ec_iters"]
+        cp_ms = None
+        for _ in range(2):
+            _, cp_mean = ctx.cp_device(g)
+            t = ctx.timings()["cp_ms"]
+            cp_ms = t if cp_ms is None else min(cp_ms, t)
+        rl = np.diff(s.to_host(g).R)
+        wedges = int(np.sum(rl.astype(np.int64) ** 2))  # (i, j, k) walks the row kernel tests
         s.free_device(g)
     deg_bytes = 4 * (N + 1) + 4 * nnz + 4 * N
     spmv_bytes = 4 * (N + 1) + 8 * nnz + 16 * N
@@ -231,6 +238,9 @@ def density_and_maps(ctx, steps: int) -> dict:
                       "roofline_frac": spmv_gbs / 8000.0, "bytes_per_iteration": spmv_bytes,
                       "note": "whole power iteration on the device (SpMV, FP64 norms, stopping test); "
                               "GB/s = algorithmic bytes x iterations / time; best of 2 runs"}
+    out["cp"] = {"ms": cp_ms, "mean_cp": cp_mean, "wedges": wedges, "wedges_per_s": wedges / (cp_ms / 1e3),
+                 "note": "weighted clustering coefficient (weight pass + one workgroup per row); "
+                         "wedges = sum of squared row lengths, the (i, j, k) walks tested; best of 2"}
     return out
 
 

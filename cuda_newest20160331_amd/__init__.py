@@ -27,7 +27,7 @@ EXPORTS = [
     "bna_opts_default", "bna_csr_free", "bna_create", "bna_destroy", "bna_synchronize",
     "bna_status_string", "bna_last_error", "bna_get_timings", "bna_workspace_bytes",
     "bna_build_csr", "bna_series_create", "bna_series_destroy", "bna_series_normalize",
-    "bna_build_csr_series", "bna_csr_to_host", "bna_series_download", "bna_degree",
+    "bna_build_csr_series", "bna_csr_to_host", "bna_series_download", "bna_degree", "bna_cp",
     "bna_eigcent", "bna_density_to_k", "bna_synth_factor_model", "bna_write_csr",
     "bna_read_csr", "bna_write_nm", "bna_nifti_mask", "bna_nifti_bold", "bna_free",
     "bna_build_csr_batch", "bna_trim", "bna_server_config_default", "bna_server_create",
@@ -64,7 +64,7 @@ class Timings(C.Structure):
         ("csr_ms", C.c_double), ("refine_ms", C.c_double), ("select_ms", C.c_double),
         ("total_ms", C.c_double),
         ("corr_launches", C.c_int64),
-        ("degree_ms", C.c_double), ("ec_ms", C.c_double), ("ec_iters", C.c_int64),
+        ("degree_ms", C.c_double), ("ec_ms", C.c_double), ("ec_iters", C.c_int64), ("cp_ms", C.c_double),
     ]
 
 
@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         "bna_csr_to_host": (i32, [vp, C.POINTER(CSR), C.POINTER(CSR)]),
         "bna_series_download": (i32, [vp, vp, vp, vp, vp]),
         "bna_degree": (i32, [vp, C.POINTER(CSR), fp]),
+        "bna_cp": (i32, [vp, C.POINTER(CSR), fp, C.POINTER(C.c_double)]),
         "bna_eigcent": (i32, [vp, C.POINTER(CSR), dbl, i32, i32, C.POINTER(C.c_double),
                               C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "bna_density_to_k": (i64, [i32, C.c_float]),
@@ -315,6 +316,21 @@ class Context:
         deg = np.empty(g.N, np.float32)
         self._check(self._L.bna_degree(self._h, C.byref(h[0]), _f32p(deg)), "bna_degree")
         return deg
+
+    def cp(self, g: Graph):
+        """Clustering coefficient per vertex and the mean |Cp| (bna_cp)."""
+        h = _host_csr(g)
+        out = np.empty(g.N, np.float32)
+        mean = C.c_double()
+        self._check(self._L.bna_cp(self._h, C.byref(h[0]), _f32p(out), C.byref(mean)), "bna_cp")
+        return out, mean.value
+
+    def cp_device(self, g: CSR):
+        """Clustering coefficient of a device-resident CSR."""
+        out = np.empty(g.N, np.float32)
+        mean = C.c_double()
+        self._check(self._L.bna_cp(self._h, C.byref(g), _f32p(out), C.byref(mean)), "bna_cp")
+        return out, mean.value
 
     def autotune(self, precision: str, L: int, N: int = 16384, cache: str | None = None):
         """Tunes the tile traversal band height for (precision, L's bucket);

@@ -1384,6 +1384,38 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg)
     return BNA_OK;
 }
 
+int bna_cp(bna_ctx *ctx, const bna_csr *g, float *cp, double *mean)
+{
+    if (!ctx || !g || !cp) return fail(ctx, BNA_ERR_ARG, "bna_cp: NULL argument");
+    if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_cp: empty graph");
+    if (g->N > CP_MAX_N) return fail(ctx, BNA_ERR_ARG, "bna_cp: N above 524288 (the row kernel's LDS bitmap)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    float *d, *w = nullptr;
+    unsigned long long *key = nullptr;
+    if ((rc = ws_get_t(ctx, "cp", (size_t)g->N, &d))) return rc;
+    if (V && g->nnz > 0) {
+        if ((rc = ws_get_t(ctx, "cp_w", (size_t)g->nnz, &w))) return rc;
+        if ((rc = ws_get_t(ctx, "cp_key", (size_t)1, &key))) return rc;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    HIPCHK(ctx, launch_cp(R, C, V, g->N, g->nnz, w, key, d, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
+    if ((rc = copy_d2h(ctx, cp, d, sizeof(float) * (size_t)g->N))) return rc;  // synchronises
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.cp_ms = ms;
+    if (mean) {
+        double s = 0.0;
+        for (int i = 0; i < g->N; ++i) s += std::fabs((double)cp[i]);
+        *mean = (double)((float)s / (float)g->N);
+    }
+    return BNA_OK;
+}
+
 int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec, double *lambda,
                 int *iters)
 {

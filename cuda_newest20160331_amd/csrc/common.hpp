@@ -173,4 +173,12 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
 size_t ec_state_bytes();
 int ec_nparts(int N);
 
+// Weighted clustering coefficient (k_cp.hip).  Columns must ascend within each
+// row.  w: nnz floats of scratch (the cube-rooted weights); key: one u64 of
+// scratch; cp: N floats.  V == NULL: binary graph.  N at most CP_MAX_N (the
+// row kernel's LDS bitmap takes 8 bytes per 32 voxels).
+constexpr int CP_MAX_N = 524288;
+hipError_t launch_cp(const int32_t *R, const int32_t *C, const float *V, int N, int64_t nnz, float *w,
+                     unsigned long long *key, float *cp, hipStream_t st);
+
 }  // namespace bna

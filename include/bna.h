@@ -113,6 +113,7 @@ typedef struct bna_timings {
     double degree_ms;  /* last bna_degree: the degree kernel */
     double ec_ms;      /* last bna_eigcent: the whole power iteration on the device */
     int64_t ec_iters;  /* last bna_eigcent: iterations */
+    double cp_ms;      /* last bna_cp: weight pass and row kernel */
 } bna_timings;
 
 /* ----------------------------------------------------------------- context */
@@ -231,6 +232,15 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg);
  * (V == NULL) uses weight 1 for every entry. */
 int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec,
                 double *lambda, int *iters);
+
+/* Weighted clustering coefficient, the CUCP tool's Cp_2 (W/Cp_GPU/CUCP/Cp.cu:111-471):
+ * w = cube root of V / V[m] (m the first entry of largest |V|), cci_i = (W^3)_ii,
+ * Cp_i = cci_i / (t (t - 1)) with the reference's float t = (float)R[i+1] -
+ * (float)R[i] (0 when t < 2).  A binary graph (V == NULL) uses w = 1.  cci is
+ * summed in double.  cp is host, N floats; mean (may be NULL) gets the average
+ * |Cp| as the float the tool prints to <stem>_cp.txt (Cp_main.cpp, sasum / N).
+ * Columns must ascend within each row; N at most 524288. */
+int bna_cp(bna_ctx *ctx, const bna_csr *g, float *cp, double *mean);
 
 /* -------------------------------------------------------- host utilities */
 /* Number of undirected edges for a density percentage, with the reference's

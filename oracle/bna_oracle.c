@@ -35,6 +35,10 @@
  *                       order); binary B/Degree/Degree/Degree.cpp:84-91.
  *   orc_eigcent         W/CUEC/weighted-net.cu:129-190 (weighted, output A*y) and
  *                       B/CUEC/CUEC/CUEC_unweighted.cu:156-195 (binary, output y).
+ *   orc_cp              Cp_2 (W/Cp_GPU/CUCP/Cp.cu:111-471): cube-rooted weights
+ *                       scaled by the largest |V|, (W^3)_ii over t (t - 1) with
+ *                       the reference's float t = (float)R[i+1] - (float)R[i];
+ *                       cci summed in double over a dense scatter of row i.
  *   orc_fisher,         fishertrans / inv_fishertrans (CorMat.cpp:36-50) in the
  *   orc_inv_fisher      reference's float arithmetic (float quotient, logf/expf).
  *   orc_group_add       post_block (CorMat.cpp:824-887): each pair's r, as float,
@@ -541,6 +545,62 @@ void orc_degree(const int64_t *R, const float *V, int N, float *deg)
             deg[k] = (float)(R[k + 1] - R[k]);
         }
     }
+}
+
+/* -------------------------------------------------- clustering coefficient */
+
+/* The CUCP tool's Cp_2 (W/Cp_GPU/CUCP/Cp.cu:111-471), restated on the host:
+ *   alpha = (float)(1.0 / V[m]), m the first entry of largest |V| (cublasIsamax
+ *   and the host division, :189-205); w = (float)pow((double)(V * alpha), 1.0/3)
+ *   (cublasSscal, pow3_vector :33-46); cci_i = (W^3)_ii (the csrmm over dense
+ *   row blocks and calc_cci_block, :48-92, :317-341), here over a dense scatter
+ *   of row i, summed in double; t = (float)R[i+1] - (float)R[i], inv = 0 if
+ *   t < 2 else (float)(1.0 / (t * (t - 1))) (cal_k_kernel :15-31); Cp_i =
+ *   (float)cci_i * inv (dot_vv :96-109).  V == NULL: every w is 1.  Returns the
+ *   float mean of |Cp| (cublasSasum / N, :455-457) as a double. */
+double orc_cp(const int64_t *R, const int *C, const float *V, int N, float *cp)
+{
+    const int64_t nnz = R[N];
+    float *w = (float *)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+    double *row = (double *)calloc((size_t)N, sizeof(double));
+    unsigned char *mark = (unsigned char *)calloc((size_t)N, 1);
+    if (V && nnz > 0) {
+        int64_t m = -1;
+        float best = -1.0f;
+        for (int64_t q = 0; q < nnz; q++) {
+            const float a = fabsf(V[q]);
+            if (a == a && a > best) { best = a; m = q; }
+        }
+        const float alpha = m >= 0 ? (float)(1.0 / (double)V[m]) : NAN;
+        for (int64_t q = 0; q < nnz; q++) {
+            const float s = V[q] * alpha;
+            w[q] = (float)pow((double)s, 1.0 / 3);
+        }
+    } else {
+        for (int64_t q = 0; q < nnz; q++) w[q] = 1.0f;
+    }
+    double sum = 0.0;
+    for (int i = 0; i < N; i++) {
+        for (int64_t q = R[i]; q < R[i + 1]; q++) { row[C[q]] = w[q]; mark[C[q]] = 1; }
+        double cci = 0.0;
+        for (int64_t q = R[i]; q < R[i + 1]; q++) {
+            const int j = C[q];
+            for (int64_t p = R[j]; p < R[j + 1]; p++) {
+                const int k = C[p];
+                if (mark[k]) cci += (double)w[q] * ((double)w[p] * row[k]);
+            }
+        }
+        for (int64_t q = R[i]; q < R[i + 1]; q++) { row[C[q]] = 0.0; mark[C[q]] = 0; }
+        float t = (float)R[i + 1];
+        t -= (float)R[i];
+        const float inv = t < 2.0f ? 0.0f : (float)(1.0 / (double)(t * (t - 1.0f)));
+        cp[i] = (float)cci * inv;
+        sum += fabs((double)cp[i]);
+    }
+    free(w);
+    free(row);
+    free(mark);
+    return (double)((float)sum / (float)N);
 }
 
 /* ----------------------------------------------------- eigenvector centrality */

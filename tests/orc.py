@@ -42,6 +42,7 @@ def olib() -> C.CDLL:
         "orc_graph_copy": (None, [vp, vp, vp, vp, vp]),
         "orc_graph_free": (None, [vp]),
         "orc_degree": (None, [vp, vp, i32, vp]),
+        "orc_cp": (dbl, [vp, vp, vp, i32, vp]),
         "orc_eigcent": (dbl, [vp, vp, vp, i32, dbl, i32, i32, vp, C.POINTER(C.c_int)]),
         "orc_ref_cpu_rows": (i64, [vp, i32, i32, i32, i32, C.c_float, i32, C.POINTER(C.c_double)]),
         "orc_fisher": (C.c_float, [C.c_float]),
@@ -142,6 +143,16 @@ def degree(R, V, N):
     deg = np.empty(N, np.float32)
     olib().orc_degree(_p(R), _p(V), N, _p(deg))
     return deg
+
+
+def cp(R, Cc, V, N):
+    """Clustering coefficient per vertex and the float mean of |Cp| (orc_cp)."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    V = None if V is None else np.ascontiguousarray(V, np.float32)
+    out = np.empty(N, np.float32)
+    mean = olib().orc_cp(_p(R), _p(Cc), _p(V), N, _p(out))
+    return out, mean
 
 
 def eigcent(R, Cc, V, N, tol=1e-6, max_iter=10000, out_mode=0):

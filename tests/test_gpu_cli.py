@@ -186,3 +186,22 @@ def test_cormat_group_binary(subjects):
     assert len(files) == 1, sorted(p.name for p in d.iterdir())
     g = bna.read_csr(files[0])
     assert g.V is None and g.nnz == 2 * orc.density_k(N, np.float32(3.0))
+
+
+def test_cp_tool(subjects):
+    d, mask, Ls = subjects
+    say("cli: clustering coefficient over the weighted and unweighted outputs")
+    for sub in ("weighted", "unweighted"):
+        if not (d / sub).exists() or not list((d / sub).glob("*.csr")):
+            pytest.skip("construction outputs missing")
+        run("bna_cp", d / sub, 0)
+        for f in sorted((d / sub).glob("*.csr")):
+            g = bna.read_csr(f)
+            ocp, omean = orc.cp(g.R.astype(np.int64), g.C, g.V, g.N)
+            cp = bna.read_nm(str(f)[:-4] + ".cp")
+            if g.V is None:
+                assert np.array_equal(cp, ocp)
+            else:
+                assert np.allclose(cp, ocp, rtol=1e-6, atol=0)
+            mean = float(Path(str(f)[:-4] + "_cp.txt").read_text())
+            assert abs(mean - omean) <= 1e-5 * abs(omean)  # six significant digits
