@@ -458,6 +458,7 @@ def main() -> int:
         line["density_form"] = maps["build"]
         line["degree"] = maps["degree"]
         line["eigcent"] = maps["eigcent"]
+        line["cp"] = maps["cp"]
     if not a.no_batch:
         log("bench: batched form")
         line["batch"] = batch_form(a.batch_subjects, 0xB17A0000 + 4000)
