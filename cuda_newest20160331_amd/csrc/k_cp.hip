@@ -105,27 +105,51 @@ __global__ __launch_bounds__(CP_THREADS) void cp_row_kernel(const int32_t *__res
         base += __popc(bits[q]);
     }
     __syncthreads();
-    // sum over j in row i, k in row j with (i, k) an edge: w_ij w_jk w_ik
+    // sum over j in row i, k in row j with (i, k) an edge: w_ij w_jk w_ik.
+    // Each wave takes every fourth j; the next j and its row bounds are loaded
+    // while the current row is walked, and a lane keeps four of row j's
+    // columns in flight (q, q + 64, q + 128, q + 192).
     double acc = 0.0;
-    for (int jj = b + wv; jj < e; jj += CP_THREADS / 64) {
-        const int j = C[jj];
-        if ((unsigned)j >= (unsigned)N) continue;
+    int jj = b + wv;
+    int jn = jj < e ? C[jj] : -1;
+    int bn = 0, en = 0;
+    if ((unsigned)jn < (unsigned)N) {
+        bn = R[jn];
+        en = R[jn + 1];
+    }
+    while (jj < e) {
+        const int j = jn, bj = bn, ej = en;
         const double wij = WEIGHTED ? (double)w[jj] : 1.0;
-        const int bj = R[j], ej = R[j + 1];
-        for (int q = bj + lane; q < ej; q += 64) {
-            const int k = C[q];
-            if ((unsigned)k >= (unsigned)N) continue;
-            const uint32_t word = bits[k >> 5];
-            const uint32_t m = 1u << (k & 31);
-            if (word & m) {
-                if (WEIGHTED) {
-                    const int rank = (int)(pre[k >> 5] + __popc(word & (m - 1u)));
-                    acc += wij * ((double)w[q] * (double)w[b + rank]);
-                } else {
-                    acc += 1.0;
+        const int jj2 = jj + CP_THREADS / 64;
+        jn = jj2 < e ? C[jj2] : -1;
+        bn = en = 0;
+        if ((unsigned)jn < (unsigned)N) {
+            bn = R[jn];
+            en = R[jn + 1];
+        }
+        if ((unsigned)j < (unsigned)N) {
+            for (int q = bj + lane; q < ej; q += 256) {
+                int kk[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) kk[u] = q + 64 * u < ej ? C[q + 64 * u] : -1;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = kk[u];
+                    if ((unsigned)k >= (unsigned)N) continue;
+                    const uint32_t word = bits[k >> 5];
+                    const uint32_t m = 1u << (k & 31);
+                    if (word & m) {
+                        if (WEIGHTED) {
+                            const int rank = (int)(pre[k >> 5] + __popc(word & (m - 1u)));
+                            acc += wij * ((double)w[q + 64 * u] * (double)w[b + rank]);
+                        } else {
+                            acc += 1.0;
+                        }
+                    }
                 }
             }
         }
+        jj = jj2;
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
     if (lane == 0) red[wv] = acc;
