@@ -68,10 +68,6 @@ def build_lib(jobs: int = 8) -> Path:
 def build_tools() -> list[Path]:
     BINDIR.mkdir(exist_ok=True)
     outs = []
-    # device-ceiling probe for the correlation kernel's roofline (stand-alone HIP program)
-    o = BINDIR / "bna_mfma_ceiling"
-    _run([HIPCC, *COMMON, *DEVICE, str(CSRC / "tool_mfma_ceiling.hip"), "-o", str(o)])
-    outs.append(o)
     for name, src in TOOLS.items():
         s = CSRC / src
         if not s.exists():

@@ -134,6 +134,20 @@ __device__ int64_t claim_tile(unsigned int *claim, int64_t n_items, int64_t per,
     return -1;
 }
 
+// One side of a stage: NQ wave instructions of 1 KB each from one global
+// address and one LDS address; the immediate offset q * 1024 advances both.
+template <int NQ>
+__device__ __forceinline__ void dma_side(const unsigned char *g, unsigned char *l)
+{
+    static_assert(NQ == 2 || NQ == 4, "stage side of 2 or 4 KB per wave");
+    __builtin_amdgcn_global_load_lds((gvoid *)g, (lvoid *)l, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((gvoid *)g, (lvoid *)l, 16, 1024, 0);
+    if constexpr (NQ == 4) {
+        __builtin_amdgcn_global_load_lds((gvoid *)g, (lvoid *)l, 16, 2048, 0);
+        __builtin_amdgcn_global_load_lds((gvoid *)g, (lvoid *)l, 16, 3072, 0);
+    }
+}
+
 // Block-wide exclusive scan of one int per thread (256 threads).
 __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total, int lane, int w) {
     int inc = v;
@@ -415,6 +429,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const int swave = __builtin_amdgcn_readfirstlane(wave);  // the same value, held in a scalar register
     const int wr0 = (wave >> 1) * 128;
     const int wc0 = (wave & 1) * 128;
     const int lrow = lane & 31;
@@ -443,7 +458,10 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
     // [w * 64, w * 64 + 64) of each side.
     // Instruction q covers rows w * 64 + q * (1024 / UNIT) + lane / CH; those row
     // offsets are multiples of 16, so the swizzle does not depend on q and the
-    // byte offset is dsrc + q * 1024.
+    // byte offset is dsrc + q * 1024.  The LDS destination advances by the same
+    // 1024 bytes, so the four instructions of a side share one global address and
+    // one M0 value and differ only in the immediate offset, which the instruction
+    // applies to both addresses.
     const uint32_t dsrc = (uint32_t)((wave * (TILE / 4) + lane / C::CH) * C::UNIT +
                                      (((lane % C::CH) ^ swz<C::CH>(lane / C::CH)) << 4));
     int g = 0;  // stream position of the current tile's stage 0; stage s lives in slot (g + s) % NSLOT
@@ -459,12 +477,11 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         const int64_t o_ = own_ ? s_ * ic.kstride : (s_ - nk) * xK;                                \
         const unsigned char *a_ = (own_ ? cA : xA) + o_;                                           \
         const unsigned char *b_ = (own_ ? cB : xB) + o_;                                           \
-        unsigned char *slot_ = smem + ((g + s_) & (NSLOT - 1)) * C::SLOT;                          \
-        _Pragma("unroll") for (int q = 0; q < C::DMA_PER_SIDE; q++) {                              \
-            unsigned char *dA = slot_ + (wave * C::DMA_PER_SIDE + q) * 1024;                       \
-            __builtin_amdgcn_global_load_lds((gvoid *)(a_ + dsrc + q * 1024), (lvoid *)dA, 16, 0, 0); \
-            __builtin_amdgcn_global_load_lds((gvoid *)(b_ + dsrc + q * 1024), (lvoid *)(dA + C::PANEL), 16, 0, 0); \
-        }                                                                                          \
+        unsigned char *dA_ = smem + ((g + s_) & (NSLOT - 1)) * C::SLOT + swave * C::DMA_PER_SIDE * 1024; \
+        const unsigned char *ga_ = a_ + dsrc;                                                      \
+        const unsigned char *gb_ = b_ + dsrc;                                                      \
+        dma_side<C::DMA_PER_SIDE>(ga_, dA_);                                                       \
+        dma_side<C::DMA_PER_SIDE>(gb_, dA_ + C::PANEL);                                            \
     } while (0)
 
     // Fragment registers: bf16 hi and lo 8-element vectors per block; fp32 two
@@ -528,6 +545,25 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         }                                                                                          \
     } while (0)
 
+    // Issue order of one stage after its barrier: an MFMA first (the fragments
+    // are already in registers), then each stage load and each LDS read placed
+    // behind an MFMA of its own, so their issue cycles fall in the shadow of
+    // MFMAs already running instead of leaving the matrix pipe idle.
+    constexpr int NMF = (PREC == 0) ? 128 : (PREC == 1 ? 16 : 48);  // MFMAs per stage
+    constexpr int NRD = (PREC == 1) ? 8 : 16;                        // LDS fragment reads per stage
+#define BNA_INTERLEAVE()                                                                           \
+    do {                                                                                           \
+        _Pragma("unroll") for (int i_ = 0; i_ < C::DPS; i_++) {                                    \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                     \
+            __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);                                     \
+        }                                                                                          \
+        _Pragma("unroll") for (int i_ = 0; i_ < NRD; i_++) {                                       \
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                                     \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                     \
+        }                                                                                          \
+        __builtin_amdgcn_sched_group_barrier(0x008, NMF - C::DPS - NRD, 0);                        \
+    } while (0)
+
     // Prologue of the first tile: stream positions 0 .. AHEAD - 1.
     {
         const int nk = ic.nk;
@@ -582,12 +618,17 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
                 BNA_DMA(kt + 4);
                 BNA_LDFRAG(fa1, fb1, kt + 1);
                 BNA_MFMA(fa0, fb0);
+                BNA_INTERLEAVE();
                 __builtin_amdgcn_sched_barrier(0);
                 BNA_WAIT_VMCNT(2 * C::DPS);
                 __builtin_amdgcn_s_barrier();
                 BNA_DMA(kt + 5);
-                if (kt + 2 < nk) BNA_LDFRAG(fa0, fb0, kt + 2);
+                // read unconditionally (a slot is always inside LDS; past the
+                // tile's last stage the values are not used), so the block has no
+                // branch and the reads can sit between the MFMAs
+                BNA_LDFRAG(fa0, fb0, kt + 2);
                 BNA_MFMA(fa1, fb1);
+                BNA_INTERLEAVE();
                 __builtin_amdgcn_sched_barrier(0);
             }
             if (nk & 1) {  // the last stage, nk - 1, is in set 0
@@ -723,6 +764,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
 #undef BNA_DMA
 #undef BNA_LDFRAG
 #undef BNA_MFMA
+#undef BNA_INTERLEAVE
 }
 
 // Clears the per-range claim counters of one launch.
