@@ -58,6 +58,19 @@ typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) void gvoid;
+// Device-memory pointers of the subject descriptor, used as global-address-space
+// pointers: a generic (flat) access may touch LDS, so the compiler would make it
+// wait for the stage loads in flight to the LDS ring first.
+typedef __attribute__((address_space(1))) int32_t g_i32;
+typedef __attribute__((address_space(1))) int64_t g_i64;
+typedef __attribute__((address_space(1))) unsigned long long g_u64;
+typedef __attribute__((address_space(1))) float g_f32;
+static_assert(sizeof(Entry) == 8 && sizeof(Candidate) == 32, "entry and candidate records are stored as 64-bit words");
+__device__ __forceinline__ uint64_t pack2(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
+__device__ __forceinline__ unsigned long long gatomic_add(unsigned long long *p, unsigned long long v)
+{
+    return __hip_atomic_fetch_add((g_u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 typedef __attribute__((address_space(3))) void lvoid;
 
 constexpr int CT_THREADS = 256;
@@ -222,13 +235,13 @@ __device__ __forceinline__ void epi_accum(const f32x16_t &a, const EpiCtx &c, fl
     const bool inner = !c.diag && c.gi0 + TILE <= c.N && c.gj0 + TILE <= c.N;
     if (inner) {
         float add[16];
-        float *slot[16];
+        g_f32 *slot[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int64_t i = c.gi0 + c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
             const float v = a[e];
             add[e] = (v >= -1.0f && v <= 1.0f) ? (fisher ? fisher_z(v) : v) : 0.0f;
-            slot[e] = dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
+            slot[e] = (g_f32 *)dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
         }
         float old[16];
 #pragma unroll
@@ -243,7 +256,7 @@ __device__ __forceinline__ void epi_accum(const f32x16_t &a, const EpiCtx &c, fl
         const float v = a[e];
         if (epi_in(c, lr, lc) && v >= -1.0f && v <= 1.0f) {
             const int64_t i = c.gi0 + lr;
-            float *slot = dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
+            g_f32 *slot = (g_f32 *)dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
             *slot += fisher ? fisher_z(v) : v;
         }
     }
@@ -294,7 +307,8 @@ __device__ __forceinline__ void epi_mark_inner(const f32x16_t &a, const EpiCtx &
 #pragma unroll
     for (int g = 0; g < 4; g++) {
         const float mx = fmaxf(fmaxf(a[4 * g], a[4 * g + 1]), fmaxf(a[4 * g + 2], a[4 * g + 3]));
-        if (__ballot(mx > c.keep) == 0) continue;
+        if (__builtin_expect(__ballot(mx > c.keep) == 0, 1)) continue;  // kept pairs are rare: the
+                                                                         // recording code is laid out cold
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int e = 4 * g + q;
@@ -320,8 +334,10 @@ __device__ __forceinline__ void epi_mark_edge(const f32x16_t &a, const EpiCtx &c
         const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
         const bool kp = a[e] > c.keep && epi_in(c, lr, lc);
         const uint64_t m = __ballot(kp);
-        if (kp) epi_record(wl, m, lt, lr, lc, a[e], rowbits, colbits);
-        wl.n += __popcll(m);
+        if (__builtin_expect(m != 0, 0)) {
+            if (kp) epi_record(wl, m, lt, lr, lc, a[e], rowbits, colbits);
+            wl.n += __popcll(m);
+        }
     }
 }
 
@@ -352,8 +368,9 @@ __device__ __forceinline__ void emit_pair(const SubjDesc &sd, const EpiCtx &c, c
     if (act) {
         pair_slots(s, lr, lc, ps, qs);
         if (fits) {
-            sd.entries[ps] = Entry{c.gj0 + lc, v};
-            sd.entries[qs] = Entry{c.gi0 + lr, v};
+            // Entry {int32 col; float r} as one 64-bit word
+            ((g_u64 *)sd.entries)[ps] = pack2((uint32_t)(c.gj0 + lc), __builtin_bit_cast(uint32_t, v));
+            ((g_u64 *)sd.entries)[qs] = pack2((uint32_t)(c.gi0 + lr), __builtin_bit_cast(uint32_t, v));
         }
     }
     const bool cand = act && !(v > sd.sure_thr);
@@ -362,11 +379,17 @@ __device__ __forceinline__ void emit_pair(const SubjDesc &sd, const EpiCtx &c, c
         const int lane = c.lane;
         const int leader = __builtin_ctzll(cm);
         unsigned long long cb = 0;
-        if (lane == leader) cb = atomicAdd(sd.cand_count, (unsigned long long)__popcll(cm));
+        if (lane == leader) cb = gatomic_add(sd.cand_count, (unsigned long long)__popcll(cm));
         cb = __shfl(cb, leader, 64);
         if (cand) {
             const unsigned long long ci = cb + (unsigned long long)__popcll(cm & ((1ull << lane) - 1ull));
-            if ((int64_t)ci < sd.cand_cap) sd.cands[ci] = Candidate{c.gi0 + lr, c.gj0 + lc, ps, qs, v, 0};
+            if ((int64_t)ci < sd.cand_cap) {  // Candidate {i, j, slot_row, slot_col, r, pad} as four words
+                g_u64 *q = (g_u64 *)sd.cands + 4 * (int64_t)ci;
+                q[0] = pack2((uint32_t)(c.gi0 + lr), (uint32_t)(c.gj0 + lc));
+                q[1] = (uint64_t)ps;
+                q[2] = (uint64_t)qs;
+                q[3] = pack2(__builtin_bit_cast(uint32_t, v), 0u);
+            }
         }
     }
 }
@@ -409,7 +432,7 @@ __device__ __forceinline__ void hist_flush(uint32_t *hist, unsigned long long *d
     for (int k = tid; k < HIST_BINS; k += CT_THREADS) {
         const uint32_t h = hist[k];
         if (h) {
-            atomicAdd(&dst[k], (unsigned long long)h);
+            gatomic_add(dst + k, (unsigned long long)h);
             hist[k] = 0;
         }
     }
@@ -601,6 +624,11 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         // counter proves that the stages issued before them have landed.  (Those
         // stages, this tile's first three, had the epilogue's time to arrive.)
         BNA_WAIT_VMCNT(0);
+        // The tile after next is claimed from the home range now; the answer is
+        // needed only at the boundary, so the atomic's round trip overlaps the
+        // stage loop instead of stalling the epilogue.
+        unsigned int pre_t = 0;
+        if (tid == 0 && nxt >= 0) pre_t = atomicAdd(&claim[xcd], 1u);
         if constexpr (AHEAD == 4) {
             // Iteration kt: stage kt's fragments are in set kt & 1.  Wait for stage
             // kt + 1 (positions kt + 2 and kt + 3 may stay in flight; this wave's
@@ -659,9 +687,6 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         int etid = tid;
         asm volatile("" : "+v"(etid));
         const int elane = etid & 63, ewave = etid >> 6;
-        // the tile after next is claimed now; the answer is needed at the boundary
-        int64_t nn_claim = -1;
-        if (etid == 0 && nxt >= 0) nn_claim = claim_tile(claim, n_items, per_xcd, xcd);
         EpiCtx ec;
         ec.N = sd.N;
         ec.gi0 = ic.I * TILE;
@@ -711,7 +736,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
             const int co = block_excl_scan(cc, wsum2, total2, elane, ewave);
             rowoff[etid] = ro;
             coloff[etid] = co;
-            if (etid == 0) *sbase = total ? atomicAdd(sd.entry_count, (unsigned long long)(2 * total)) : 0ull;
+            if (etid == 0) *sbase = total ? gatomic_add(sd.entry_count, (unsigned long long)(2 * total)) : 0ull;
             lds_sync();
             const int64_t base = (int64_t)*sbase;
             const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
@@ -720,15 +745,15 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
             const int N = sd.N;
             if (ec.gi0 + etid < N) {
                 const int64_t f = (int64_t)(ic.J + 1) * N + ec.gi0 + etid;
-                sd.frag_orig[f] = rc;
-                sd.frag_live[f] = rc;
-                sd.frag_off[f] = base + ro;
+                ((g_i32 *)sd.frag_orig)[f] = rc;
+                ((g_i32 *)sd.frag_live)[f] = rc;
+                ((g_i64 *)sd.frag_off)[f] = base + ro;
             }
             if (ec.gj0 + etid < N) {
                 const int64_t f = (int64_t)ic.I * N + ec.gj0 + etid;
-                sd.frag_orig[f] = cc;
-                sd.frag_live[f] = cc;
-                sd.frag_off[f] = base + total + co;
+                ((g_i32 *)sd.frag_orig)[f] = cc;
+                ((g_i32 *)sd.frag_live)[f] = cc;
+                ((g_i64 *)sd.frag_off)[f] = base + total + co;
             }
             const SlotCtx sc{rowbits, colbits, rowoff, coloff, base, total};
             if (wl.n > wl.cap) __builtin_amdgcn_s_waitcnt(0);  // this wave's spilled records are stored
@@ -744,8 +769,17 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         }
 
         // ------------------------------------------------------------ boundary
-        // Publish the claimed tile after next.
-        if (etid == 0) ctl[2] = nn_claim;
+        // Publish the claimed tile after next: the home range's answer, or, once
+        // that range is drained, a claim from the others.
+        if (etid == 0) {
+            int64_t nn_claim = -1;
+            if (nxt >= 0) {
+                const int64_t lo = (int64_t)xcd * per_xcd;
+                const int64_t cnt = n_items - lo < per_xcd ? n_items - lo : per_xcd;
+                nn_claim = (int64_t)pre_t < cnt ? lo + (int64_t)pre_t : claim_tile(claim, n_items, per_xcd, xcd);
+            }
+            ctl[2] = nn_claim;
+        }
         __syncthreads();
         const int64_t nn = uni64(ctl[2]);
         if constexpr (HIST == 1) {
