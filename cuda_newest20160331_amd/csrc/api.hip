@@ -794,6 +794,7 @@ struct Job {
     unsigned long long hc[8] = {};
     SelState hs{};
     int64_t nnz = 0;
+    bool refined = false;  // values refined on the lists before placement
 };
 
 template <typename T>
@@ -973,12 +974,16 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         for (Job *j : act) {
             Entry *entries;
             Candidate *cands;
-            if ((rc = job_ws(ctx, *j, "entries", (size_t)j->ecap, &entries)) ||
+            uint8_t *mirror = nullptr;
+            const bool refine = j->o.weighted && j->o.refine;
+            if ((refine && (rc = job_ws(ctx, *j, "mirror", (size_t)j->ecap, &mirror))) ||
+                (rc = job_ws(ctx, *j, "entries", (size_t)j->ecap, &entries)) ||
                 (rc = job_ws(ctx, *j, "cands", (size_t)j->ccap, &cands)) ||
                 (j->k > 0 && (rc = job_ws(ctx, *j, "cand_r64", (size_t)j->ccap, &j->r64c))))
                 return rc;
             j->h.entries = entries;
             j->h.entry_cap = j->ecap;
+            j->h.mirror = mirror;
             j->h.cands = cands;
             j->h.cand_cap = j->ccap;
             HIPCHK(ctx, hipMemsetAsync(j->ctr, 0, 8 * sizeof(unsigned long long), ctx->stream));
@@ -1052,8 +1057,26 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         }
     }
 
-    // placement (and value refinement) of every finished job
+    // value refinement on the lists, then placement, of every finished job
     HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    for (size_t q = 0; q < jobs.size(); q++) {
+        Job &j = jobs[q];
+        bna_series *s = j.s;
+        if (j.rc != BNA_OK || !j.o.weighted || !j.o.refine || !j.h.mirror) continue;
+        if (j.k > 0 && (j.hs.err || j.nnz != 2 * j.k)) continue;  // fails below
+        const int64_t ld = s->xv ? s->L_pad : s->L;
+        if (!refine_lists_supported(s->xv ? s->xv : s->x, ld, s->L)) continue;  // refined after placement
+        j.refined = true;
+        const double bytes = (double)j.nnz / 2.0 * s->L * 4.0 + (double)s->N * s->L * 4.0;
+        const int64_t per = (int64_t)std::max(64.0, s->N * (ctx->chunk_ms * REFINE_BYTES_PER_MS / bytes));
+        if ((rc = run_chunks(ctx, s->N, per, [&](int64_t r0, int64_t n) {
+                 return launch_refine_lists(j.h.entries, j.h.mirror, j.orig_rm, j.off_rm, j.h.frag_off,
+                                            s->xv ? s->xv : s->x, ld, s->mu, s->sd, s->N, s->L, s->N_pad / TILE,
+                                            (int)r0, (int)(r0 + n), ctx->stream);
+             })))
+            return rc;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
     for (size_t q = 0; q < jobs.size(); q++) {
         Job &j = jobs[q];
         if (j.rc != BNA_OK) continue;
@@ -1075,11 +1098,11 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         HIPCHK(ctx, launch_place(j.orig_rm, j.live_rm, j.off_rm, j.h.entries, j.R64, s->N, s->N_pad / TILE, j.Cd,
                                  j.Vd, ctx->stream));
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
+    // shapes the list form does not take: refinement of the placed CSR
     for (size_t q = 0; q < jobs.size(); q++) {
         Job &j = jobs[q];
         bna_series *s = j.s;
-        if (j.rc == BNA_OK && j.Vd && j.o.refine && j.nnz > 0) {
+        if (j.rc == BNA_OK && j.Vd && j.o.refine && !j.refined && j.nnz > 0) {
             const double bytes = (double)j.nnz / 2.0 * s->L * 4.0 + (double)s->N * s->L * 4.0;
             const int64_t per = (int64_t)std::max(64.0, s->N * (ctx->chunk_ms * REFINE_BYTES_PER_MS / bytes));
             if ((rc = run_chunks(ctx, s->N, per, [&](int64_t r0, int64_t n) {
@@ -1093,8 +1116,8 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 
     float ms_csr = 0.f, ms_ref = 0.f, ms_tot = 0.f;
-    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[5], ctx->ev[6]));
-    HIPCHK(ctx, hipEventElapsedTime(&ms_ref, ctx->ev[6], ctx->ev[7]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms_ref, ctx->ev[5], ctx->ev[6]));
+    HIPCHK(ctx, hipEventElapsedTime(&ms_csr, ctx->ev[6], ctx->ev[7]));
     HIPCHK(ctx, hipEventElapsedTime(&ms_tot, timing_from_norm ? ctx->ev[0] : ctx->ev[1], ctx->ev[7]));
     ctx->tm.corr_ms = ms_corr;
     ctx->tm.recheck_ms = ms_re;

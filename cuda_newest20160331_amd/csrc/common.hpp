@@ -58,6 +58,8 @@ struct SubjDesc {
     unsigned long long *hist; // histogram pass: HIST_BINS counters
     float *dense;        // accumulation pass: packed upper triangle, M1 floats
     int32_t fisher;      // accumulation pass: add Fisher z instead of r
+    uint8_t *mirror;     // per row-view entry slot: position of the pair's other entry within
+                         // its column-view fragment (null: not recorded)
 };
 
 constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
@@ -155,6 +157,10 @@ hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32,
 int scan_blocks(int N);
 // Recomputes every value of a weighted CSR in double from the voxel-major series.
 // Rows [row0, row1) of the CSR; nnz is the whole CSR's entry count (sizes the grid).
+bool refine_lists_supported(const float *xs, int64_t ld, int L);
+hipError_t launch_refine_lists(Entry *entries, const uint8_t *mirror, const int32_t *orig_rm, const int64_t *off_rm,
+                               const int64_t *off_km, const float *xs, int64_t ld, const double *mu, const double *sd,
+                               int N, int L, int nb, int row0, int row1, hipStream_t st);
 hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
                          const double *mu, const double *sd, int N, int L, int row0, int row1, int64_t nnz,
                          hipStream_t st);

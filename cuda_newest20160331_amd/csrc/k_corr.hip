@@ -347,9 +347,10 @@ struct SlotCtx {
     const int *rowoff, *coloff;
     int64_t base, total;
 };
-__device__ __forceinline__ void pair_slots(const SlotCtx &s, int lr, int lc, int64_t &ps, int64_t &qs)
+__device__ __forceinline__ void pair_slots(const SlotCtx &s, int lr, int lc, int64_t &ps, int64_t &qs, int &qw)
 {
     int p = s.rowoff[lr], q = s.coloff[lc];
+    const int q0 = q;
     const int rwd = lc >> 5, cwd = lr >> 5;
     for (int w = 0; w < rwd; w++) p += __popc(s.rowbits[lr * 8 + w]);
     for (int w = 0; w < cwd; w++) q += __popc(s.colbits[lc * 8 + w]);
@@ -357,6 +358,7 @@ __device__ __forceinline__ void pair_slots(const SlotCtx &s, int lr, int lc, int
     q += __popc(s.colbits[lc * 8 + cwd] & ((1u << (lr & 31)) - 1u));
     ps = s.base + p;
     qs = s.base + s.total + q;
+    qw = q - q0;  // position within column lc's list
 }
 
 // Writes one kept pair's two entries and, for band pairs, its candidate record
@@ -366,8 +368,11 @@ __device__ __forceinline__ void emit_pair(const SubjDesc &sd, const EpiCtx &c, c
 {
     int64_t ps = 0, qs = 0;
     if (act) {
-        pair_slots(s, lr, lc, ps, qs);
+        int qw = 0;
+        pair_slots(s, lr, lc, ps, qs, qw);
         if (fits) {
+            // where the value refinement finds the pair's column-view entry
+            if (sd.mirror) ((__attribute__((address_space(1))) uint8_t *)sd.mirror)[ps] = (uint8_t)qw;
             // Entry {int32 col; float r} as one 64-bit word
             ((g_u64 *)sd.entries)[ps] = pack2((uint32_t)(c.gj0 + lc), __builtin_bit_cast(uint32_t, v));
             ((g_u64 *)sd.entries)[qs] = pack2((uint32_t)(c.gi0 + lr), __builtin_bit_cast(uint32_t, v));

@@ -530,6 +530,169 @@ __global__ __launch_bounds__(256) void refine_rows_kernel(const int32_t *__restr
     }
 }
 
+// Value refinement on the tile lists, before placement.  One wave per row i
+// keeps z_i in registers as double (T pieces of four per lane, as above) and
+// takes the entries of its row-view fragments (neighbours j > i, tiles
+// (I(i), J >= I(i))), four at a time, one per 16-lane group at the end: row j
+// is read as 64-lane 16-byte pieces and
+//   r = (sum_l z_i[l] x_j[l] - mean_j sum_l z_i[l]) / norm_j.
+// The value goes to the entry and to the pair's column-view entry, whose slot
+// is the column's fragment offset in tile (I(i), J(j)) plus the position the
+// tile kernel recorded for the pair (mirror), so no search is needed.
+// Rejected entries (column -1) are skipped.  A row's entry slots are queued in
+// LDS first, so sparse fragments fill whole four-entry trips.  Workgroups are
+// single waves: rows differ in length, and a wave that finishes frees its
+// slot at once instead of waiting for the longest row of its workgroup.
+constexpr int RQ = 512;  // queued slots per wave (a fragment holds at most TILE)
+static_assert(RQ >= TILE, "a whole fragment fits the queue");
+
+template <int T>
+__global__ __launch_bounds__(64) void refine_lists_kernel(Entry *__restrict__ entries,
+                                                           const uint8_t *__restrict__ mirror,
+                                                           const int32_t *__restrict__ orig_rm,
+                                                           const int64_t *__restrict__ off_rm,
+                                                           const int64_t *__restrict__ off_km,
+                                                           const float *__restrict__ xs, int64_t ld,
+                                                           const double *__restrict__ mu,
+                                                           const double *__restrict__ sd, int N, int L, int nb,
+                                                           int row0, int row1)
+{
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __shared__ int64_t q[RQ];
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, gl = lane & 15;
+    const bool h5 = lane & 32, h4 = lane & 16;
+    const int L4 = L >> 2, Lt = L & 3;
+    const int64_t i = (int64_t)row0 + blockIdx.x;
+    const int nb1 = nb + 1, I = (int)(i / TILE);
+    float *V = (float *)entries;  // entry e's value is float 2e + 1
+
+    const float *xi = xs + i * ld;
+    const double mi = mu[i], ii = 1.0 / sd[i];
+    double zi[T][4], zt = 0.0, s = 0.0;
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const int pq = lane + 64 * t;
+        f4 x4 = {0.f, 0.f, 0.f, 0.f};
+        if (pq < L4) x4 = *(const f4 *)(xi + 4 * pq);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            zi[t][c] = pq < L4 ? ((double)x4[c] - mi) * ii : 0.0;
+            s += zi[t][c];
+        }
+    }
+    if (lane < Lt) {
+        zt = ((double)xi[4 * L4 + lane] - mi) * ii;
+        s += zt;
+    }
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) s += __shfl_xor(s, k, 64);
+
+    // refines the queued entries q[0, nq)
+    auto flush = [&](int nq) {
+        for (int e0 = 0; e0 < nq; e0 += 4) {
+            int j[4];
+            int64_t sl[4];
+            f4 xj[4][T];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; q4++) {
+                sl[q4] = e0 + q4 < nq ? q[e0 + q4] : -1;
+                j[q4] = sl[q4] >= 0 ? entries[sl[q4]].col : -1;
+                const float *xr = xs + (int64_t)max(j[q4], 0) * ld;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const int pq = lane + 64 * t;
+                    xj[q4][t] = (f4){0.f, 0.f, 0.f, 0.f};
+                    if (j[q4] >= 0 && pq < L4) xj[q4][t] = *(const f4 *)(xr + 4 * pq);
+                }
+            }
+            // this group's entry and its mirror slot, loaded beside the rows
+            const int jg = g == 0 ? j[0] : g == 1 ? j[1] : g == 2 ? j[2] : j[3];
+            const int64_t sg = g == 0 ? sl[0] : g == 1 ? sl[1] : g == 2 ? sl[2] : sl[3];
+            int64_t msl = -1;
+            double mj = 0.0, sj = 1.0;
+            if (jg >= 0 && gl == 0) {
+                msl = off_km[(int64_t)I * N + jg] + mirror[sg];
+                mj = mu[jg];
+                sj = sd[jg];
+            }
+            double acc[4];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; q4++) {
+                double av = 0.0;
+                if (lane < Lt && j[q4] >= 0) av = zt * (double)xs[(int64_t)j[q4] * ld + 4 * L4 + lane];
+#pragma unroll
+                for (int t = 0; t < T; t++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) av = __builtin_fma(zi[t][c], (double)xj[q4][t][c], av);
+                acc[q4] = av;
+            }
+            // fold: after xor 32 lane half h5 holds entries {2h5, 2h5+1}; after xor
+            // 16 quarter (h5, h4) holds entry 2h5 + h4; xor 8..1 finish the sum
+            double v0 = (h5 ? acc[2] : acc[0]) + __shfl_xor(h5 ? acc[0] : acc[2], 32, 64);
+            double v1 = (h5 ? acc[3] : acc[1]) + __shfl_xor(h5 ? acc[1] : acc[3], 32, 64);
+            double w = (h4 ? v1 : v0) + __shfl_xor(h4 ? v0 : v1, 16, 64);
+#pragma unroll
+            for (int k = 8; k >= 1; k >>= 1) w += __shfl_xor(w, k, 64);
+            if (jg >= 0 && gl == 0) {
+                const float v = (float)((w - mj * s) / sj);
+                V[2 * sg + 1] = v;
+                V[2 * msl + 1] = v;
+            }
+        }
+    };
+
+    int nq = 0;  // wave-uniform
+    for (int k0 = I + 1; k0 <= nb; k0 += 64) {
+        const int k = k0 + lane;
+        const int c = k <= nb ? orig_rm[i * nb1 + k] : 0;
+        const int64_t off = c ? off_rm[i * nb1 + k] : 0;
+        uint64_t m = __ballot(c > 0);
+        while (m) {
+            const int src = __builtin_ctzll(m);
+            m &= m - 1;
+            const int cnt = __shfl(c, src, 64);
+            const int64_t o = __shfl(off, src, 64);
+            if (nq + cnt > RQ) {
+                flush(nq);
+                nq = 0;
+            }
+            for (int t = lane; t < cnt; t += 64) q[nq + t] = o + t;
+            nq += cnt;
+        }
+    }
+    flush(nq);
+}
+
+// The list form holds row i in at most 8 pieces of four per lane (L <= 2048)
+// and reads 16-byte aligned rows; other shapes refine the placed CSR instead.
+bool refine_lists_supported(const float *xs, int64_t ld, int L)
+{
+    return L >= 1 && (L >> 2) <= 64 * 8 && ld % 4 == 0 && ((uintptr_t)xs & 15) == 0;
+}
+
+hipError_t launch_refine_lists(Entry *entries, const uint8_t *mirror, const int32_t *orig_rm, const int64_t *off_rm,
+                               const int64_t *off_km, const float *xs, int64_t ld, const double *mu, const double *sd,
+                               int N, int L, int nb, int row0, int row1, hipStream_t st)
+{
+    if (row0 < 0 || row1 > N || row0 > row1) return hipErrorInvalidValue;
+    if (row0 == row1) return hipSuccess;
+    if (!refine_lists_supported(xs, ld, L)) return hipErrorNotSupported;
+    const int T = std::max(1, ((L >> 2) + 63) / 64);
+    const dim3 grid((unsigned)(row1 - row0));
+#define BNA_REFINE_LISTS(TT)                                                                       \
+    case TT:                                                                                       \
+        hipLaunchKernelGGL(refine_lists_kernel<TT>, grid, dim3(64), 0, st, entries, mirror, orig_rm, off_rm, off_km, \
+                           xs, ld, mu, sd, N, L, nb, row0, row1);                                  \
+        break;
+    switch (T) {
+        BNA_REFINE_LISTS(1) BNA_REFINE_LISTS(2) BNA_REFINE_LISTS(3) BNA_REFINE_LISTS(4)
+        BNA_REFINE_LISTS(5) BNA_REFINE_LISTS(6) BNA_REFINE_LISTS(7) BNA_REFINE_LISTS(8)
+    }
+#undef BNA_REFINE_LISTS
+    return hipGetLastError();
+}
+
 hipError_t launch_refine(const int32_t *R, const int32_t *C, float *V, const float *xs, int64_t ld,
                          const double *mu, const double *sd, int N, int L, int row0, int row1, int64_t nnz,
                          hipStream_t st)
