@@ -15,6 +15,7 @@ Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -61,39 +62,83 @@ def time_path(ctx, series, N, L, prec, thr, steps, warmup):
     t1 = time.perf_counter()
     keys = ("znorm_ms", "corr_ms", "recheck_ms", "select_ms", "csr_ms", "refine_ms")
     st_ms = {k: round(float(np.mean([d[k] for d in stages])), 4) for k in keys}
-    return (t1 - t0) * 1e3 / steps, float(np.mean(corr)), last, st_ms
+    # one more build, untimed, copied to the host: digests of R and C and the
+    # values, to compare the precisions' graphs
+    series.normalize(bna.PRECISIONS[prec])
+    g = series.build_device(thr=thr, precision=prec)
+    h = series.to_host(g)
+    series.free_device(g)
+    sha = hashlib.sha256(h.R.astype(np.int64).tobytes() + h.C.astype(np.int32).tobytes()).hexdigest()
+    return (t1 - t0) * 1e3 / steps, float(np.mean(corr)), last, st_ms, sha, h.V
 
 
-def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float):
-    """The reference CPU path (CorMat.cpp:954-965 arithmetic, CorMat_CPU.cpp
-    j-striping) from the oracle restatement, on a bounded sample of rows."""
+def cpu_info() -> dict:
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count() or 1, "affinity": aff, "model": model}
+
+
+def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float, c2: bool = True):
+    """The reference CPU path (normalise, CorMat_cpu correlation with
+    CorMat_cpu_MT's j-striding, FormFullGraph(_s), FormCSRGraph, degree, power
+    iteration; orc_ref_pipeline in the oracle) on the host cores: C1 and C2 in
+    full, and the headline workload (C3) as a bounded row sample.  Threads: the
+    process's CPU share, at most 16 (the box's share for one GPU; the reference
+    uses one thread per logical processor)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import orc
 
-    threads = min(16, os.cpu_count() or 1)
-    _, z32, _, _, _ = orc.znorm(x, 0)
-    rows = 64
-    t0 = time.perf_counter()
-    orc.ref_cpu_rows(z32, 0, rows, thr, threads)
-    dt = time.perf_counter() - t0
-    pairs = sum(N - 1 - i for i in range(rows))
-    rate = pairs / dt
-    # grow the sample to about target_s seconds of CPU work
+    info = cpu_info()
+    threads = max(1, min(16, info["affinity"]))
+    M1 = N * (N - 1) // 2
+
+    def summary(t, M1_):
+        tot = t["t_norm"] + t["t_corr"] + t["t_thr"] + t["t_csr"] + t["t_deg"] + t["t_ec"]
+        return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in t.items()} | {
+            "total_s": round(tot, 3), "voxel_pairs_per_s": M1_ / tot}
+
+    out = {}
+    # C1: N = 5000, L = 200, r > 0.5, whole path
+    x1 = bna.synth_factor_model(5000, 200, 8, 0.7, 0xB17A0000 + 1000)
+    out["c1"] = summary(orc.ref_pipeline(x1, thr=0.5, nthreads=threads), 5000 * 4999 // 2)
+    log(f"cpu: C1 {out['c1']}")
+    if c2:  # C2: N = 45000, L = 240, 1 % density (exact k), whole path
+        x2 = bna.synth_factor_model(45000, 240, 20, 0.6, 0xB17A0000 + 2000)
+        k2 = orc.density_k(45000, 1.0)
+        out["c2"] = summary(orc.ref_pipeline(x2, k=k2, nthreads=threads), 45000 * 44999 // 2)
+        del x2
+        log(f"cpu: C2 {out['c2']}")
+    # C3: rows [0, n) of the headline subject, n sized to about target_s
+    probe = orc.ref_pipeline(x, row0=0, row1=16, thr=thr, nthreads=threads)
+    rate = probe["pairs"] / max(probe["t_corr"], 1e-6)
     want = rate * target_s
-    rows2, acc = 0, 0
-    while acc < want and rows2 < N - 1:
-        acc += N - 1 - (rows + rows2)
-        rows2 += 1
-    t0 = time.perf_counter()
-    orc.ref_cpu_rows(z32, rows, rows + rows2, thr, threads)
-    dt = time.perf_counter() - t0
+    n, acc = 0, 0
+    while acc < want and n < N - 1:
+        acc += N - 1 - n
+        n += 1
+    t = orc.ref_pipeline(x, row0=0, row1=n, thr=thr, nthreads=threads)
+    # the normalisation covers the whole subject; the sample carries its share
+    sec = t["t_corr"] + t["t_thr"] + t["t_csr"] + t["t_norm"] * t["pairs"] / M1
+    out["c3_sample"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in t.items()}
     return {
-        "value": acc / dt,
+        "value": t["pairs"] / sec,
         "unit": "voxel-pairs/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"rows {rows}..{rows + rows2 - 1} of N={N}, L={L}: {acc} pairs, "
-                  f"r>{thr} count, {dt:.1f} s on {threads} threads",
+        "sample": f"rows 0..{n - 1} of N={N}, L={L} (all pairs j > i: {t['pairs']} pairs, "
+                  f"{t['pairs'] / M1:.2%} of the subject): normalise (whole subject, its share counted), "
+                  f"correlate, r>{thr} into the bool matrix, CSR scan; {sec:.1f} s on {threads} threads",
+        "cpu": info,
+        "c1_full": out["c1"],
+        "c2_full": out.get("c2"),
+        "c3_sample": out["c3_sample"],
     }
 
 
@@ -366,6 +411,7 @@ def main() -> int:
     ap.add_argument("--thr", type=float, default=0.4)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-c2", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--serve-subjects", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2.0)
@@ -408,8 +454,8 @@ def main() -> int:
         with ctx.series(x) as s:
             for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"])):
                 log(f"bench: {prec} warmup={a.warmup} steps={a.steps}")
-                ms, corr_ms, st, st_ms = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
-                res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st, "stages_ms": st_ms}
+                ms, corr_ms, st, st_ms, sha, V = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
+                res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st, "stages_ms": st_ms, "csr_sha": sha, "V": V}
                 log(f"bench: {prec} {ms:.3f} ms/step, corr kernel {corr_ms:.3f} ms, nnz={st[0]} "
                     f"cand={st[1]} flips={st[2]} band={st[3]:.3g} max_err={st[4]:.3g} stages={st_ms}")
         if not a.no_maps:
@@ -452,7 +498,8 @@ def main() -> int:
         line["fp32_value"] = fval
         line["speedup_vs_fp32"] = value / fval
         line["fp32_roofline_frac"] = flops / (f["corr_ms"] / 1e3) / 1e12 / PEAK["fp32"]
-        line["same_csr_as_fp32"] = f["stats"][0] == b["stats"][0]
+        line["same_csr_as_fp32"] = f["csr_sha"] == b["csr_sha"]
+        line["max_abs_dV_vs_fp32"] = float(np.max(np.abs(f["V"].astype(np.float64) - b["V"]), initial=0.0))
         line["fp32_stages_ms"] = f["stages_ms"]
     if maps is not None:
         line["density_form"] = maps["build"]
@@ -475,7 +522,7 @@ def main() -> int:
         line["p99_subject_latency_ms"] = sv["p99_ms"]
     if not a.no_cpu:
         log("bench: CPU baseline")
-        line["cpu_baseline"] = cpu_baseline(x, N, L, thr, a.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(x, N, L, thr, a.cpu_seconds, c2=not a.no_cpu_c2)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -37,7 +37,7 @@ EXPORTS = [
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
-    "bna_build_csr_into",
+    "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats",
 ]
 
 
@@ -133,6 +133,8 @@ def lib() -> C.CDLL:
         "bna_corr_packed": (i32, [vp, vp, i32, fp]),
         "bna_build_csr_into": (i32, [vp, vp, C.POINTER(Opts), vp, vp, vp, i64, C.POINTER(C.c_int64)]),
         "bna_select_density": (i32, [vp, vp, i32, dbl, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "bna_set_graphs": (i32, [vp, i32]),
+        "bna_batch_graph_stats": (i32, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -305,6 +307,15 @@ class Context:
         return {k: getattr(t, k) for k, _ in Timings._fields_}
 
     # device-resident series
+    def set_graphs(self, on: bool):
+        self._check(self._L.bna_set_graphs(self._h, int(on)), "bna_set_graphs")
+
+    def batch_graph_stats(self) -> tuple[int, int]:
+        """(captures, replays) of the batched build's graph."""
+        c, r = C.c_int64(), C.c_int64()
+        self._check(self._L.bna_batch_graph_stats(self._h, C.byref(c), C.byref(r)), "bna_batch_graph_stats")
+        return c.value, r.value
+
     def series(self, x: np.ndarray, layout: int = TIME_MAJOR) -> "Series":
         return Series(self, x, layout)
 

@@ -66,6 +66,12 @@ struct bna_ctx {
     void *pin[3] = {nullptr, nullptr, nullptr};
     size_t pin_bytes[3] = {0, 0, 0};
     hipEvent_t pin_ev[2] = {};
+    // batched builds: the emission launch and every subject's tails captured
+    // as one HIP graph, replayed while the launch parameters stay the same
+    std::string bg_key;
+    hipGraph_t bg_graph = nullptr;
+    hipGraphExec_t bg_exec = nullptr;
+    int64_t bg_replays = 0, bg_captures = 0;
 };
 
 struct bna_series {
@@ -423,6 +429,8 @@ int bna_destroy(bna_ctx *ctx)
     }
     for (int b = 0; b < 3; b++)
         if (ctx->pin[b]) (void)hipHostFree(ctx->pin[b]);
+    if (ctx->bg_exec) (void)hipGraphExecDestroy(ctx->bg_exec);
+    if (ctx->bg_graph) (void)hipGraphDestroy(ctx->bg_graph);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return BNA_OK;
@@ -461,6 +469,21 @@ int bna_set_grid_limit(bna_ctx *ctx, int max_workgroups)
 }
 
 int bna_grid_size(bna_ctx *ctx) { return ctx ? ctx->grid : -1; }
+
+int bna_set_graphs(bna_ctx *ctx, int on)
+{
+    if (!ctx) return fail(ctx, BNA_ERR_ARG, "bna_set_graphs: ctx is NULL");
+    ctx->graphs = on != 0;
+    return BNA_OK;
+}
+
+int bna_batch_graph_stats(bna_ctx *ctx, int64_t *captures, int64_t *replays)
+{
+    if (!ctx) return fail(ctx, BNA_ERR_ARG, "bna_batch_graph_stats: ctx is NULL");
+    if (captures) *captures = ctx->bg_captures;
+    if (replays) *replays = ctx->bg_replays;
+    return BNA_OK;
+}
 
 int bna_reserve(bna_ctx *ctx, size_t bytes)
 {
@@ -898,6 +921,8 @@ static int corr_chunks(bna_ctx *ctx, int prec, int hist, SubjDesc *dsd, TileItem
     });
 }
 
+static void put_key(std::string &key, const void *p, size_t n) { key.append((const char *)p, n); }
+
 static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_norm)
 {
     const int prec = jobs[0].o.precision;
@@ -925,7 +950,7 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
             j->h.hist = j->hist;
             j->h.keep_thr = INFINITY;
             j->h.sure_thr = INFINITY;
-            HIPCHK(ctx, hipMemsetAsync(j->hist, 0, HIST_BINS * sizeof(unsigned long long), ctx->stream));
+            HIPCHK(ctx, launch_zero_u64(j->hist, HIST_BINS, ctx->stream));
         }
         SubjDesc *dsd;
         TileItem *tiles;
@@ -986,51 +1011,134 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
             j->h.mirror = mirror;
             j->h.cands = cands;
             j->h.cand_cap = j->ccap;
-            HIPCHK(ctx, hipMemsetAsync(j->ctr, 0, 8 * sizeof(unsigned long long), ctx->stream));
         }
         SubjDesc *dsd;
         TileItem *tiles;
         int64_t nt;
         if ((rc = upload_batch(ctx, act, &dsd, &tiles, &nt))) return rc;
-        HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        if ((rc = corr_chunks(ctx, prec, 0, dsd, tiles, nt, act))) return rc;
-        launches++;
-        HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        for (Job *j : act) {
-            bna_series *s = j->s;
-            const int N = s->N, nb = s->N_pad / TILE;
-            const float *xs = s->xv ? s->xv : s->x;
-            const int64_t ld = s->xv ? s->L_pad : s->L;
-            const int mode = j->k > 0 ? (j->o.recheck ? 1 : 2) : 0;
-            HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, j->h.cands, j->ctr + 1, j->h.cand_cap,
-                                       j->thr_eff, j->h.frag_live, j->h.entries, j->h.entry_cap, j->ctr + 2, mode,
-                                       j->r64c, ctx->stream));
-        }
-        HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
         // per job, one pinned readback record: selection state, counters, nnz
         constexpr size_t RB = (sizeof(SelState) + sizeof(act[0]->hc) + sizeof(int64_t) + 63) / 64 * 64;
         if ((rc = pin_grow(ctx, 2, RB * act.size()))) return rc;
         unsigned char *rb = (unsigned char *)ctx->pin[2];
-        for (size_t q = 0; q < act.size(); q++) {
-            Job *j = act[q];
-            if (j->k > 0) {
-                HIPCHK(ctx, launch_select(j->sel, j->ctr, j->h.cands, j->r64c, j->h.cand_cap, j->k, j->s->N,
-                                          j->h.frag_live, j->h.entries, j->h.entry_cap, ctx->stream));
-                HIPCHK(ctx, hipMemcpyAsync(rb + q * RB, j->sel, sizeof(SelState), hipMemcpyDeviceToHost, ctx->stream));
+        // The segment from the emission launch to the counter readbacks.  In a
+        // batch it is captured as one HIP graph (stage events are not recorded
+        // inside; the tile kernel's time then covers the tails too) and
+        // replayed while its launch parameters, the key, stay the same.
+        const bool graph = ctx->graphs && jobs.size() > 1 && ctx->chunk_ms <= 0.0;
+        auto segment = [&](bool events) -> int {
+            for (Job *j : act) HIPCHK(ctx, launch_zero_u64(j->ctr, 8, ctx->stream));
+            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+            int rc2;
+            if ((rc2 = corr_chunks(ctx, prec, 0, dsd, tiles, nt, act))) return rc2;
+            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+            for (Job *j : act) {
+                bna_series *s = j->s;
+                const int N = s->N, nb = s->N_pad / TILE;
+                const float *xs = s->xv ? s->xv : s->x;
+                const int64_t ld = s->xv ? s->L_pad : s->L;
+                const int mode = j->k > 0 ? (j->o.recheck ? 1 : 2) : 0;
+                HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, j->h.cands, j->ctr + 1, j->h.cand_cap,
+                                           j->thr_eff, j->h.frag_live, j->h.entries, j->h.entry_cap, j->ctr + 2, mode,
+                                           j->r64c, ctx->stream));
             }
+            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+            for (size_t q = 0; q < act.size(); q++) {
+                Job *j = act[q];
+                if (j->k > 0) {
+                    HIPCHK(ctx, launch_select(j->sel, j->ctr, j->h.cands, j->r64c, j->h.cand_cap, j->k, j->s->N,
+                                              j->h.frag_live, j->h.entries, j->h.entry_cap, ctx->stream));
+                    HIPCHK(ctx, hipMemcpyAsync(rb + q * RB, j->sel, sizeof(SelState), hipMemcpyDeviceToHost,
+                                               ctx->stream));
+                }
+            }
+            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+            for (size_t q = 0; q < act.size(); q++) {
+                Job *j = act[q];
+                const int N = j->s->N, nb = j->s->N_pad / TILE;
+                HIPCHK(ctx, launch_frag_rows(j->h.frag_orig, j->h.frag_live, j->h.frag_off, N, nb, j->orig_rm,
+                                             j->live_rm, j->off_rm, j->rowlen, ctx->stream));
+                HIPCHK(ctx, launch_scan(j->rowlen, N, j->R64, nullptr, j->bsum, ctx->stream));
+                HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState), j->ctr, sizeof(j->hc),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState) + sizeof(j->hc), j->R64 + N,
+                                           sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+            }
+            return BNA_OK;
+        };
+        bool replayed = false;
+        if (graph) {
+            // the tile kernel's workspaces exist before the capture, so nothing
+            // is allocated inside it
+            unsigned int *claim_ws = nullptr;
+            uint64_t *spill_ws = nullptr;
+            if ((rc = ws_get_t(ctx, "tile_claim", 8, &claim_ws)) ||
+                (rc = ws_get_t(ctx, "tile_spill", (size_t)corr_spill_records(prec, ctx->grid), &spill_ws)))
+                return rc;
+            // the key: every value the segment's launches take
+            std::string key;
+            put_key(key, &claim_ws, sizeof claim_ws);
+            put_key(key, &spill_ws, sizeof spill_ws);
+            auto put = [&](const void *p, size_t n) { put_key(key, p, n); };
+            const int grid = ctx->grid_limit > 0 ? std::min(ctx->grid, ctx->grid_limit) : ctx->grid;
+            put(&prec, sizeof prec);
+            put(&grid, sizeof grid);
+            put(&dsd, sizeof dsd);
+            put(&tiles, sizeof tiles);
+            put(&nt, sizeof nt);
+            put(&rb, sizeof rb);
+            for (Job *j : act) {
+                put(&j->h, sizeof(SubjDesc));
+                const void *ptrs[] = {j->ctr, j->sel, j->r64c, j->rowlen, j->R64, j->bsum, j->orig_rm,
+                                      j->live_rm, j->off_rm, j->s->x, j->s->xv, j->s->mu, j->s->sd};
+                put(ptrs, sizeof ptrs);
+                const int ints[] = {j->s->N, j->s->L, j->s->L_pad, j->s->N_pad, j->o.recheck};
+                put(ints, sizeof ints);
+                put(&j->thr_eff, sizeof j->thr_eff);
+                put(&j->k, sizeof j->k);
+            }
+            if (ctx->bg_exec && key == ctx->bg_key) {
+                HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+                HIPCHK(ctx, hipGraphLaunch(ctx->bg_exec, ctx->stream));
+                ctx->bg_replays++;
+                replayed = true;
+            } else {
+                if (ctx->bg_exec) (void)hipGraphExecDestroy(ctx->bg_exec);
+                if (ctx->bg_graph) (void)hipGraphDestroy(ctx->bg_graph);
+                ctx->bg_exec = nullptr;
+                ctx->bg_graph = nullptr;
+                ctx->bg_key.clear();
+                HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                const int rc2 = segment(false);
+                hipGraph_t g = nullptr;
+                const hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+                if (rc2) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc2;
+                }
+                HIPCHK(ctx, ee);
+                hipGraphExec_t ex = nullptr;
+                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                if (ei != hipSuccess) {
+                    (void)hipGraphDestroy(g);
+                    HIPCHK(ctx, ei);
+                }
+                ctx->bg_graph = g;
+                ctx->bg_exec = ex;
+                ctx->bg_key = key;
+                ctx->bg_captures++;
+                HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+                HIPCHK(ctx, hipGraphLaunch(ex, ctx->stream));
+                replayed = true;
+            }
+            // the whole segment is timed as the tile kernel's
+            HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+            HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+            HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+        } else if ((rc = segment(true))) {
+            return rc;
         }
-        HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-        for (size_t q = 0; q < act.size(); q++) {
-            Job *j = act[q];
-            const int N = j->s->N, nb = j->s->N_pad / TILE;
-            HIPCHK(ctx, launch_frag_rows(j->h.frag_orig, j->h.frag_live, j->h.frag_off, N, nb, j->orig_rm, j->live_rm,
-                                         j->off_rm, j->rowlen, ctx->stream));
-            HIPCHK(ctx, launch_scan(j->rowlen, N, j->R64, j->R32, j->bsum, ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState), j->ctr, sizeof(j->hc), hipMemcpyDeviceToHost,
-                                       ctx->stream));
-            HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState) + sizeof(j->hc), j->R64 + N, sizeof(int64_t),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-        }
+        (void)replayed;
+        launches++;
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         for (size_t q = 0; q < act.size(); q++) {
             Job *j = act[q];
@@ -1095,6 +1203,7 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
             (j.rc = out_alloc(ctx, (void **)&j.Vd, sizeof(float) * (size_t)std::max<int64_t>(j.nnz, 1))))
             continue;
         bna_series *s = j.s;
+        HIPCHK(ctx, launch_narrow_rows(j.R64, (int64_t)s->N + 1, j.R32, ctx->stream));
         HIPCHK(ctx, launch_place(j.orig_rm, j.live_rm, j.off_rm, j.h.entries, j.R64, s->N, s->N_pad / TILE, j.Cd,
                                  j.Vd, ctx->stream));
     }

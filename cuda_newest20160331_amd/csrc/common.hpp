@@ -157,6 +157,8 @@ hipError_t launch_scan(const int64_t *rowlen, int N, int64_t *R64, int32_t *R32,
 int scan_blocks(int N);
 // Recomputes every value of a weighted CSR in double from the voxel-major series.
 // Rows [row0, row1) of the CSR; nnz is the whole CSR's entry count (sizes the grid).
+hipError_t launch_zero_u64(void *p, int64_t n, hipStream_t st);
+hipError_t launch_narrow_rows(const int64_t *R64, int64_t n, int32_t *R32, hipStream_t st);
 bool refine_lists_supported(const float *xs, int64_t ld, int L);
 hipError_t launch_refine_lists(Entry *entries, const uint8_t *mirror, const int32_t *orig_rm, const int64_t *off_rm,
                                const int64_t *off_km, const float *xs, int64_t ld, const double *mu, const double *sd,

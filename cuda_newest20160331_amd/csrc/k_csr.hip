@@ -735,10 +735,42 @@ hipError_t launch_recheck(const float *xs, int64_t ld, const double *mu, const d
     return hipGetLastError();
 }
 
+// Zeroes n 64-bit words.  A kernel rather than hipMemsetAsync: the build's
+// launches may be captured into a HIP graph, where memset nodes are avoided.
+__global__ __launch_bounds__(256) void zero_u64_kernel(unsigned long long *__restrict__ p, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) p[k] = 0ull;
+}
+
+hipError_t launch_zero_u64(void *p, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (unsigned long long *)p, n);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void narrow_rows_kernel(const int64_t *__restrict__ R64, int64_t n,
+                                                           int32_t *__restrict__ R32)
+{
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256)
+        R32[k] = (int32_t)R64[k];
+}
+
+// R as int32 (the caller's CSR) from the scan's int64 offsets; nnz < 2^31 is
+// checked by the caller.
+hipError_t launch_narrow_rows(const int64_t *R64, int64_t n, int32_t *R32, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(narrow_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, st, R64, n, R32);
+    return hipGetLastError();
+}
+
 hipError_t launch_frag_rows(const int32_t *orig_km, const int32_t *live_km, const int64_t *off_km, int N, int nb,
                             int32_t *orig_rm, int32_t *live_rm, int64_t *off_rm, int64_t *rowlen, hipStream_t st)
 {
-    hipError_t e = hipMemsetAsync(rowlen, 0, sizeof(int64_t) * (size_t)N, st);
+    hipError_t e = launch_zero_u64(rowlen, N, st);
     if (e != hipSuccess) return e;
     const int nb1 = nb + 1;
     hipLaunchKernelGGL(frag_rows_kernel, dim3((unsigned)((nb1 + TR - 1) / TR), (unsigned)((N + TR - 1) / TR)),

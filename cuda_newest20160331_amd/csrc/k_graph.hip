@@ -3,7 +3,7 @@
 //
 // Degree follows W/Degree/Degree/Degree.cpp:83-87: a float running sum of V in
 // CSR order, so the order (and every rounding) is the reference's.  One thread
-// per row walks the row in order; binary graphs take the row length
+// per row walks the row in order, in 16-byte pieces; binary graphs take the row length
 // (B/Degree/Degree/Degree.cpp:84-91).
 //
 // Eigenvector centrality follows Lead_Vector_GPU (W/CUEC/weighted-net.cu:55-221):
@@ -24,10 +24,13 @@ namespace bna {
 // One thread per row: the row's values are added one after another in CSR
 // order, the reference's float loop (W/Degree/Degree/Degree.cpp:83-87), so the
 // sum is the same float bit for bit.  A wave's 64 rows run side by side; each
-// lane walks its own row, four loads ahead of its additions.
+// lane walks its own row in 16-byte pieces (after a scalar head up to the
+// first 16-byte boundary), two pieces ahead of its additions, so a load
+// instruction brings four values per lane instead of one.
 __global__ __launch_bounds__(256) void degree_kernel(const int32_t *__restrict__ R, const float *__restrict__ V, int N,
                                                      float *__restrict__ deg)
 {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int b = R[i], e = R[i + 1];
@@ -37,12 +40,26 @@ __global__ __launch_bounds__(256) void degree_kernel(const int32_t *__restrict__
     }
     float s = 0.0f;
     int k = b;
-    for (; k + 4 <= e; k += 4) {
-        const float v0 = V[k], v1 = V[k + 1], v2 = V[k + 2], v3 = V[k + 3];
-        s += v0;
-        s += v1;
-        s += v2;
-        s += v3;
+    const int head = min(e, (b + 3) & ~3);  // V is 16-byte aligned
+    for (; k < head; k++) s += V[k];
+    for (; k + 8 <= e; k += 8) {
+        const f4 p = *(const f4 *)(V + k), q = *(const f4 *)(V + k + 4);
+        s += p[0];
+        s += p[1];
+        s += p[2];
+        s += p[3];
+        s += q[0];
+        s += q[1];
+        s += q[2];
+        s += q[3];
+    }
+    if (k + 4 <= e) {
+        const f4 p = *(const f4 *)(V + k);
+        s += p[0];
+        s += p[1];
+        s += p[2];
+        s += p[3];
+        k += 4;
     }
     for (; k < e; k++) s += V[k];
     deg[i] = s;
@@ -109,11 +126,23 @@ __global__ __launch_bounds__(256) void spmv_ec_kernel(const int32_t *__restrict_
     double a = 0.0, b = 0.0, c = 0.0;
     for (int row = blockIdx.x * EC_ROWS_PER_BLOCK + w; row < N; row += gridDim.x * EC_ROWS_PER_BLOCK) {
         const int rb = R[row], re = R[row + 1];
+        // four entries per lane in flight: their column and value loads are
+        // issued together, then the four gathers of x, then the products
         double s = 0.0;
-        if (V) {
-            for (int t = rb + lane; t < re; t += 64) s += (double)V[t] * (x[C[t]] * inv);
-        } else {
-            for (int t = rb + lane; t < re; t += 64) s += x[C[t]] * inv;
+        for (int t0 = rb + lane; t0 < re; t0 += 4 * 64) {
+            int cc[4];
+            float vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = t0 + 64 * u;
+                cc[u] = t < re ? C[t] : -1;
+                vv[u] = (t < re && V) ? V[t] : 1.0f;
+            }
+            double xx[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) s += (double)vv[u] * (xx[u] * inv);
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);

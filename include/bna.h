@@ -364,6 +364,14 @@ int bna_set_chunking(bna_ctx *ctx, double chunk_ms, void (*gate)(void *), void *
  * requests on other contexts find CUs free.  No counterpart in the reference,
  * which runs one subject at a time. */
 int bna_set_grid_limit(bna_ctx *ctx, int max_workgroups);
+/* HIP graphs on (default) or off for later calls: the centrality's iteration
+ * batches, and a batched build's emission launch with every subject's tails
+ * (recheck, selection, row counts, scan, counter readbacks) captured as one
+ * graph and replayed while its launch parameters stay the same.  No
+ * counterpart in the reference, which launches kernels one by one. */
+int bna_set_graphs(bna_ctx *ctx, int on);
+/* How often the batched build captured its graph and replayed it. */
+int bna_batch_graph_stats(bna_ctx *ctx, int64_t *captures, int64_t *replays);
 /* ---------------------------------------------------------------- autotuner
  * No counterpart in the reference (its cuBLAS tiles are fixed at B = 2048).
  * The tile kernel walks its upper-triangle tile table in bands of G tile-rows

@@ -47,6 +47,10 @@
  *                       then inv_fishertrans for Fisher sums.
  *   orc_threshold_packed FormFullGraph over packed values (strict v > thr) and
  *                       FormCSRGraph, for the averaged matrix.
+ *   orc_ref_pipeline    The reference's CPU construction path end to end
+ *                       (normalise, CorMat_cpu correlation, FormFullGraph(_s),
+ *                       FormCSRGraph, degree, power iteration), timed by stage:
+ *                       the CPU baseline, not a parity function.
  *
  * Pairs involving a voxel whose z has a non-finite entry (a constant series gives
  * 0/0) are never edges and take no part in density selection.
@@ -60,6 +64,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <pthread.h>
+#include <time.h>
 
 int orc_cpu_has_fma(void) { return __builtin_cpu_supports("fma") && __builtin_cpu_supports("avx2"); }
 
@@ -733,4 +738,223 @@ int64_t orc_ref_cpu_rows(const float *z32, int N, int L, int i0, int i1, float t
     free(args);
     free(th);
     return e;
+}
+
+/* ------------------------------------------ reference CPU path, end to end */
+
+/* The reference's CPU construction path, timed stage by stage, for the CPU
+ * baseline (not a parity function: the GPU is checked against the FP64
+ * functions above):
+ *   normalise   CorMat.cu:33-52 (the host transpose and z in float, double sums)
+ *   correlate   CorMat_cpu's arithmetic (CorMat.cpp:954-965: float z products
+ *               summed in double, stored as float) into the packed upper
+ *               triangle, j striped over threads as CorMat_cpu_MT does
+ *               (B/CorMat/CorMat_CPU/CorMat/CorMat_CPU.cpp:283-383)
+ *   threshold   FormFullGraph (CorMat.cpp:683-703, r > thr) or FormFullGraph_s
+ *               (CorMat.cpp:743-822: the k-th largest packed value by
+ *               selection, then r >= r* with the ties trimmed to exactly k in
+ *               row-major order) into the N x N bool matrix
+ *   csr         FormCSRGraph (CorMat.cpp:889-918): a row-major scan of the bool
+ *               matrix, V from the packed triangle
+ *   degree      Degree.cpp:83-87 (float sums in CSR order)
+ *   ec          the power iteration of weighted-net.cu:154-189 on the CPU
+ * Rows [row0, row1) < N: the correlation, threshold and CSR of those rows'
+ * upper-triangle pairs only (a bounded sample of a large subject); degree and
+ * centrality then do not run.  Returns the undirected edge count, or -1. */
+typedef struct {
+    double t_norm, t_corr, t_thr, t_csr, t_deg, t_ec;
+    int64_t pairs, edges, nnz;
+    int ec_iters;
+    double r_star, lambda;
+} orc_ref_times;
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+typedef struct {
+    const float *z;
+    int N, L, row0, row1, tid, nthreads;
+    float *packed;  /* rows [row0, row1), pairs j > i, row-major */
+    const int64_t *rowbase;
+} rp_arg;
+
+static void *rp_worker(void *p)
+{
+    rp_arg *a = (rp_arg *)p;
+    const int L = a->L;
+    for (int i = a->row0; i < a->row1; i++) {
+        const float *zi = a->z + (int64_t)i * L;
+        float *out = a->packed + a->rowbase[i - a->row0] - (i + 1);
+        int j = i + 1 + a->tid;
+        const int st = a->nthreads;
+        for (; j + 7 * st < a->N; j += 8 * st) {
+            double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            const float *zj[8];
+            for (int u = 0; u < 8; u++) zj[u] = a->z + (int64_t)(j + u * st) * L;
+            for (int l = 0; l < L; l++) {
+                float zl = zi[l];
+                for (int u = 0; u < 8; u++) {
+                    float pr = zl * zj[u][l];
+                    s[u] += (double)pr;
+                }
+            }
+            for (int u = 0; u < 8; u++) out[j + u * st] = (float)s[u];
+        }
+        for (; j < a->N; j += st) out[j] = ref_pair(zi, a->z + (int64_t)j * L, L);
+    }
+    return NULL;
+}
+
+static void swapf(float *a, float *b) { float t = *a; *a = *b; *b = t; }
+
+/* k-th largest (1-based) of v[0, n) by quickselect (the reference's select). */
+static float kth_largest(float *v, int64_t n, int64_t k)
+{
+    int64_t lo = 0, hi = n - 1, want = k - 1;
+    while (lo < hi) {
+        float piv = v[lo + (hi - lo) / 2];
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (v[i] > piv) i++;
+            while (v[j] < piv) j--;
+            if (i <= j) { swapf(&v[i], &v[j]); i++; j--; }
+        }
+        if (want <= j) hi = j;
+        else if (want >= i) lo = i;
+        else break;
+    }
+    return v[want];
+}
+
+int64_t orc_ref_pipeline(const float *x, int N, int L, int layout, int row0, int row1, double thr,
+                         int64_t k, int nthreads, orc_ref_times *t)
+{
+    memset(t, 0, sizeof(*t));
+    if (nthreads < 1) nthreads = 1;
+    if (row0 < 0 || row1 > N || row0 >= row1) return -1;
+    const int full = (row0 == 0 && row1 == N);
+    double t0 = now_s();
+    /* normalise: z in float, mean and norm in double, in time order */
+    float *z = (float *)malloc(sizeof(float) * (size_t)N * L);
+    for (int i = 0; i < N; i++) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int l = 0; l < L; l++) s1 += layout ? x[(int64_t)i * L + l] : x[(int64_t)l * N + i];
+        s1 /= L;
+        for (int l = 0; l < L; l++) {
+            double d = (layout ? x[(int64_t)i * L + l] : x[(int64_t)l * N + i]) - s1;
+            double p = d * d;
+            s2 += p;
+        }
+        s2 = sqrt(s2);
+        for (int l = 0; l < L; l++)
+            z[(int64_t)i * L + l] = (float)(((layout ? x[(int64_t)i * L + l] : x[(int64_t)l * N + i]) - s1) / s2);
+    }
+    double t1 = now_s();
+    t->t_norm = t1 - t0;
+    /* correlate into the packed rows */
+    const int nr = row1 - row0;
+    int64_t *rowbase = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nr + 1));
+    rowbase[0] = 0;
+    for (int q = 0; q < nr; q++) rowbase[q + 1] = rowbase[q] + (N - 1 - (row0 + q));
+    const int64_t np = rowbase[nr];
+    t->pairs = np;
+    float *packed = (float *)malloc(sizeof(float) * (size_t)(np ? np : 1));
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    rp_arg *args = (rp_arg *)calloc((size_t)nthreads, sizeof(rp_arg));
+    for (int q = 0; q < nthreads; q++) {
+        args[q] = (rp_arg){z, N, L, row0, row1, q, nthreads, packed, rowbase};
+        pthread_create(&th[q], NULL, rp_worker, &args[q]);
+    }
+    for (int q = 0; q < nthreads; q++) pthread_join(th[q], NULL);
+    free(th);
+    free(args);
+    double t2 = now_s();
+    t->t_corr = t2 - t1;
+    /* threshold into the bool matrix (rows of the sample; full: N x N) */
+    unsigned char *adj = (unsigned char *)calloc((size_t)nr * N, 1);
+    int64_t edges = 0;
+    if (k > 0) {
+        if (!full) { free(z); free(rowbase); free(packed); free(adj); return -1; }
+        float *cp = (float *)malloc(sizeof(float) * (size_t)np);
+        memcpy(cp, packed, sizeof(float) * (size_t)np);
+        const float rs = kth_largest(cp, np, k);
+        free(cp);
+        int64_t above = 0;
+        for (int64_t q = 0; q < np; q++) above += packed[q] > rs;
+        int64_t m = k - above, idx = 0;
+        for (int i = 0; i < N; i++)
+            for (int j = i + 1; j < N; j++, idx++) {
+                const float r = packed[idx];
+                if (r > rs || (r == rs && m-- > 0)) {
+                    adj[(int64_t)i * N + j] = 1;
+                    adj[(int64_t)j * N + i] = 1;
+                    edges++;
+                }
+            }
+        t->r_star = rs;
+    } else {
+        for (int q = 0; q < nr; q++) {
+            const int i = row0 + q;
+            const float *pr = packed + rowbase[q] - (i + 1);
+            for (int j = i + 1; j < N; j++)
+                if (pr[j] > (float)thr) {
+                    adj[(int64_t)q * N + j] = 1;
+                    if (full) adj[(int64_t)j * N + i] = 1;
+                    edges++;
+                }
+        }
+        t->r_star = thr;
+    }
+    double t3 = now_s();
+    t->t_thr = t3 - t2;
+    /* CSR: row-major scan, V from the packed triangle */
+    int64_t *R = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nr + 1));
+    int64_t nnz = 0;
+    R[0] = 0;
+    for (int q = 0; q < nr; q++) {
+        const unsigned char *ar = adj + (int64_t)q * N;
+        for (int j = 0; j < N; j++) nnz += ar[j];
+        R[q + 1] = nnz;
+    }
+    int *Cc = (int *)malloc(sizeof(int) * (size_t)(nnz ? nnz : 1));
+    float *V = (float *)malloc(sizeof(float) * (size_t)(nnz ? nnz : 1));
+    int64_t e = 0;
+    for (int q = 0; q < nr; q++) {
+        const int i = row0 + q;
+        const unsigned char *ar = adj + (int64_t)q * N;
+        for (int j = 0; j < N; j++)
+            if (ar[j]) {
+                const int a = i < j ? i : j, b = i < j ? j : i;
+                Cc[e] = j;
+                V[e] = packed[rowbase[a - row0] + (b - a - 1)];
+                e++;
+            }
+    }
+    free(adj);
+    double t4 = now_s();
+    t->t_csr = t4 - t3;
+    t->edges = edges;
+    t->nnz = nnz;
+    if (full) {
+        float *deg = (float *)malloc(sizeof(float) * (size_t)N);
+        orc_degree(R, V, N, deg);
+        double t5 = now_s();
+        t->t_deg = t5 - t4;
+        double *ec = (double *)malloc(sizeof(double) * (size_t)N);
+        t->lambda = orc_eigcent(R, Cc, V, N, 1e-6, 10000, 0, ec, &t->ec_iters);
+        t->t_ec = now_s() - t5;
+        free(deg);
+        free(ec);
+    }
+    free(R);
+    free(Cc);
+    free(V);
+    free(z);
+    free(rowbase);
+    free(packed);
+    return edges;
 }

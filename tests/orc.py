@@ -205,3 +205,52 @@ def group_average(packed_list, N: int, fisher: bool) -> np.ndarray:
 def threshold_packed(packed: np.ndarray, N: int, thr: float) -> OGraph:
     packed = np.ascontiguousarray(packed, dtype=np.float64)
     return OGraph(olib().orc_threshold_packed(packed.ctypes.data, N, thr)).load(N)
+
+
+def rows_above(z64, rows, thr, band=1e-9):
+    """For each row i of `rows`: (the columns j != i with r64(i, j) > thr, the
+    columns with r64(i, j) == thr).  r comes from a float64 product of z64 (its
+    order differs from the oracle's by ~1e-13 at L = 1200); every pair within
+    `band` of thr is recomputed in the oracle's order, so the decisions are the
+    oracle's."""
+    z64 = np.ascontiguousarray(z64, np.float64)
+    N, L = z64.shape
+    rows = np.asarray(rows, np.int64)
+    r = z64[rows] @ z64.T
+    out = []
+    for a, i in enumerate(rows):
+        ri = r[a]
+        near = np.nonzero(np.abs(ri - thr) <= band)[0].astype(np.int32)
+        if near.size:
+            ii = np.full(near.size, i, np.int32)
+            ex = np.empty(near.size)
+            olib().orc_corr_pairs(_p(z64), L, _p(ii), _p(near), near.size, _p(ex))
+            ri[near] = ex
+        ri[i] = -np.inf
+        out.append((np.nonzero(ri > thr)[0], np.nonzero(ri == thr)[0], ri))
+    return out
+
+
+class RefTimes(C.Structure):
+    _fields_ = [("t_norm", C.c_double), ("t_corr", C.c_double), ("t_thr", C.c_double),
+                ("t_csr", C.c_double), ("t_deg", C.c_double), ("t_ec", C.c_double),
+                ("pairs", C.c_int64), ("edges", C.c_int64), ("nnz", C.c_int64),
+                ("ec_iters", C.c_int), ("r_star", C.c_double), ("lam", C.c_double)]
+
+
+def ref_pipeline(x, layout=0, row0=0, row1=None, thr=0.0, k=0, nthreads=NTHREADS) -> dict:
+    """The reference CPU path end to end (orc_ref_pipeline): stage seconds,
+    pairs, edges, nnz, r*, and for a whole subject the centrality's iterations
+    and eigenvalue.  k > 0: density selection of k edges; else r > thr."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    L, N = x.shape if layout == 0 else x.shape[::-1]
+    row1 = N if row1 is None else row1
+    t = RefTimes()
+    L_ = olib()
+    L_.orc_ref_pipeline.restype = C.c_int64
+    L_.orc_ref_pipeline.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_int64, C.c_int, C.POINTER(RefTimes)]
+    e = L_.orc_ref_pipeline(_p(x), N, L, layout, row0, row1, thr, k, nthreads, C.byref(t))
+    if e < 0:
+        raise ValueError("orc_ref_pipeline: bad arguments")
+    return {f: getattr(t, f) for f, _ in RefTimes._fields_}

@@ -85,3 +85,28 @@ def test_batch_isolates_a_failing_subject(ctx):
             s.close()
     assert ctx._L.bna_trim(ctx._h) == 0
     assert ctx._L.bna_workspace_bytes(ctx._h) == 0
+
+
+def test_batch_graph_replay_equals_direct_launches(ctx):
+    # The batched build's emission launch and every subject's tails are one
+    # captured HIP graph, replayed while the launch parameters stay the same:
+    # each replay gives the bits of the direct launches.
+    xs = [bna.synth_factor_model(N, L, 8, 0.7, seed) for N, L, seed, _ in SUBJECTS]
+    series = [ctx.series(x) for x in xs]
+    try:
+        opts = [bna.make_opts(precision="bf16x3", **kw) for *_, kw in SUBJECTS]
+        ctx.set_graphs(False)
+        ref = bna.build_batch(ctx, series, opts)
+        c0, r0 = ctx.batch_graph_stats()
+        ctx.set_graphs(True)
+        for rep in range(3):
+            got = bna.build_batch(ctx, series, opts)
+            for g, h in zip(got, ref):
+                same(g, h)
+        c1, r1 = ctx.batch_graph_stats()
+        say(f"  batch graph: {c1 - c0} captures, {r1 - r0} replays in 3 builds")
+        assert c1 > c0 and r1 - r0 >= 1
+    finally:
+        ctx.set_graphs(True)
+        for s in series:
+            s.close()

@@ -230,47 +230,6 @@ def test_ec_known_answer_4x4(ctx):
     assert it == 32
 
 
-def test_c3_full_size_properties(ctx):
-    # C3 shape: N = 45000, L = 1200, r > 0.4.  BF16X3 and FP32 must give the
-    # same CSR (both equal the FP64 oracle by construction); 40 sampled rows
-    # are compared with the oracle's r64 directly.
-    say("C3 full size")
-    N, L = 45000, 1200
-    x = bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 3000)
-    with ctx.series(x) as s:
-        a = s.build(thr=0.4, precision="bf16x3")
-        t = ctx.timings()
-        b = s.build(thr=0.4, precision="fp32")
-        z, _, _ = s.download()
-    say(f"  bf16x3 nnz={a.nnz} cand={a.n_candidates} flips={a.n_flips} max_err={a.max_err:.3g} "
-        f"corr_ms={t['corr_ms']:.2f} total_ms={t['total_ms']:.2f}")
-    assert np.array_equal(a.R, b.R) and np.array_equal(a.C, b.C)
-    rng = np.random.default_rng(0)
-    rows = rng.choice(N, 40, replace=False)
-    z64, _, _, _, valid = orc.znorm(x, 0)
-    assert np.array_equal(z, z64)
-    for i in rows:
-        ii = np.full(N, i, np.int32)
-        jj = np.arange(N, dtype=np.int32)
-        r = np.empty(N)
-        orc.olib().orc_corr_pairs(z64.ctypes.data, L, ii.ctypes.data, jj.ctypes.data, N, r.ctypes.data)
-        r[i] = -np.inf
-        exp = np.nonzero(r > 0.4)[0]
-        assert np.array_equal(a.C[a.R[i]:a.R[i + 1]], exp), f"row {i}"
-    # every weighted value against the oracle's r64 (upper-triangle entries)
-    rows = np.repeat(np.arange(N, dtype=np.int32), np.diff(a.R))
-    up = a.C > rows
-    ii = np.ascontiguousarray(rows[up])
-    jj = np.ascontiguousarray(a.C[up])
-    r = np.empty(len(ii))
-    orc.olib().orc_corr_pairs(z64.ctypes.data, L, ii.ctypes.data, jj.ctypes.data, len(ii), r.ctypes.data)
-    for name, g in (("bf16x3", a), ("fp32", b)):
-        d = np.abs(g.V[up].astype(np.float64) - r)
-        say(f"  {name}: max |V - r64| over all {len(ii)} edges = {d.max():.3g} "
-            f"(> 1e-6: {int(np.sum(d > 1e-6))}), band {g.band:.3g}")
-        assert d.max() <= V_TOL
-
-
 @pytest.mark.parametrize("N,L,layout", [(900, 97, bna.VOXEL_MAJOR), (600, 2100, bna.TIME_MAJOR)])
 def test_refine_general_forms(ctx, N, L, layout):
     # rows that are not 16-byte aligned (voxel-major input, L % 4 != 0) and rows
