@@ -37,7 +37,7 @@ EXPORTS = [
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
-    "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats",
+    "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
 ]
 
 
@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
         "bna_cp": (i32, [vp, C.POINTER(CSR), fp, C.POINTER(C.c_double)]),
         "bna_eigcent": (i32, [vp, C.POINTER(CSR), dbl, i32, i32, C.POINTER(C.c_double),
                               C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "bna_eigcent_noalloc": (i32, [vp, C.POINTER(CSR), dbl, i32, i32, vp,
+                                      C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "bna_density_to_k": (i64, [i32, C.c_float]),
         "bna_synth_factor_model": (i32, [fp, i32, i32, i32, dbl, C.c_uint64, i32]),
         "bna_write_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
@@ -301,6 +303,10 @@ class Context:
         finally:
             self._L.bna_csr_free(C.byref(g))
 
+    def workspace_bytes(self) -> int:
+        """Device bytes the context's workspace holds (bna_workspace_bytes)."""
+        return int(self._L.bna_workspace_bytes(self._h))
+
     def timings(self) -> dict:
         t = Timings()
         self._check(self._L.bna_get_timings(self._h, C.byref(t)), "bna_get_timings")
@@ -380,6 +386,17 @@ class Context:
                                         ec.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam),
                                         C.byref(it)), "bna_eigcent")
         return ec, lam.value, it.value
+
+    def eigcent_noalloc(self, g: CSR, ec_dev: int, tol: float = 1e-6, max_iter: int = 10000,
+                        output: int = EC_WEIGHTED_REF):
+        """bna_eigcent_noalloc: device CSR in, N doubles written at device
+        address ec_dev; returns (lambda, iterations)."""
+        lam = C.c_double()
+        it = C.c_int()
+        self._check(self._L.bna_eigcent_noalloc(self._h, C.byref(g), tol, max_iter, output,
+                                                C.c_void_p(ec_dev), C.byref(lam), C.byref(it)),
+                    "bna_eigcent_noalloc")
+        return lam.value, it.value
 
     def eigcent(self, g: Graph, tol: float = 1e-6, max_iter: int = 10000,
                 output: int = EC_WEIGHTED_REF, binary: bool = False):

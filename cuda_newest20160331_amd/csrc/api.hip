@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,40 @@ struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
 };
+
+// ------------------------------------------------ no-allocation scratch
+// bna_eigcent_noalloc keeps the power iteration's working memory here instead
+// of in the context's workspace: two iterate vectors of BNA_NOALLOC_EC_MAX_N
+// doubles, the per-workgroup partial sums and the iteration state.  Each GPU
+// the library runs on holds its own copy (about 16.4 MiB) until the process
+// ends.  In host code `scratch` names a host-side stand-in: the device address
+// comes from hipGetSymbolAddress, once per device (scratch_setup), and only
+// that address is handed to kernels.
+constexpr size_t NA_X_BYTES = sizeof(double) * (size_t)BNA_NOALLOC_EC_MAX_N;
+constexpr size_t NA_PART_BYTES = sizeof(double) * 3 * (size_t)EC_NPARTS_MAX;
+constexpr size_t NA_SCRATCH_BYTES = 2 * NA_X_BYTES + NA_PART_BYTES + EC_STATE_MAX_BYTES;
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex na_setup_mu;
+static std::mutex na_call_mu[NA_MAX_DEVICES];  // one call at a time per GPU: they share the array
+static bool na_looked_up[NA_MAX_DEVICES];
+static void *na_ptr[NA_MAX_DEVICES];
+static hipError_t na_err[NA_MAX_DEVICES];
+
+// Resolves the device address of `scratch` on `device`, which must be the
+// current device; later calls for the same device do nothing.
+static void scratch_setup(int device)
+{
+    if (device < 0 || device >= NA_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(na_setup_mu);
+    if (na_looked_up[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    na_err[device] = e;
+    na_ptr[device] = (e == hipSuccess) ? p : nullptr;
+    na_looked_up[device] = true;
+}
 
 struct bna_ctx {
     int device = 0;
@@ -352,6 +387,7 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
         delete ctx;
         return fail(nullptr, BNA_ERR_HIP, m);
     }
+    scratch_setup(device);  // a failure is reported by bna_eigcent_noalloc
     for (auto &ev : ctx->chev) {
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
             std::string m = std::string("bna_create: ") + hipGetErrorString(e);
@@ -1578,6 +1614,50 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     int rc2 = copy_d2h(ctx, ec, outd, sizeof(double) * (size_t)N);  // through pinned memory; synchronises
     if (rc2) return rc2;
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.ec_ms = ms;
+    ctx->tm.ec_iters = it;
+    if (iters) *iters = it;
+    return BNA_OK;
+}
+
+int bna_eigcent_noalloc(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec,
+                        double *lambda, int *iters)
+{
+    if (!ctx || !g || !ec || !lambda) return fail(ctx, BNA_ERR_ARG, "bna_eigcent_noalloc: NULL argument");
+    if (!g->on_device)
+        return fail(ctx, BNA_ERR_ARG, "bna_eigcent_noalloc: g must be a device CSR (a host CSR needs an upload buffer)");
+    if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_eigcent_noalloc: empty graph");
+    if (g->N > BNA_NOALLOC_EC_MAX_N)
+        return fail(ctx, BNA_ERR_ARG, "bna_eigcent_noalloc: N = " + std::to_string(g->N) +
+                                          " exceeds the scratch maximum of " + std::to_string(BNA_NOALLOC_EC_MAX_N) +
+                                          " voxels (use bna_eigcent)");
+    if (output != BNA_EC_WEIGHTED_REF && output != BNA_EC_BINARY_REF) return fail(ctx, BNA_ERR_ARG, "bad output mode");
+    const int dev = ctx->device;
+    if (dev >= NA_MAX_DEVICES) return fail(ctx, BNA_ERR_ARG, "bna_eigcent_noalloc: device ordinal too large");
+    const int N = g->N;
+    if ((size_t)3 * ec_nparts(N) * sizeof(double) > NA_PART_BYTES || ec_state_bytes() > EC_STATE_MAX_BYTES)
+        return fail(ctx, BNA_ERR_ARG, "bna_eigcent_noalloc: scratch layout does not fit N");
+    HIPCHK(ctx, hipSetDevice(dev));
+    scratch_setup(dev);  // bna_create did the lookup already; this covers nothing else
+    if (!na_ptr[dev])
+        return fail(ctx, BNA_ERR_HIP, std::string("bna_eigcent_noalloc: hipGetSymbolAddress(scratch): ") +
+                                          hipGetErrorString(na_err[dev]));
+    unsigned char *base = (unsigned char *)na_ptr[dev];
+    double *xa = (double *)base;
+    double *xb = (double *)(base + NA_X_BYTES);
+    double *part = (double *)(base + 2 * NA_X_BYTES);
+    void *stt = base + 2 * NA_X_BYTES + NA_PART_BYTES;
+    // the host side of the stopping state: pageable, read after a stream sync
+    alignas(16) unsigned char hstate[EC_STATE_MAX_BYTES];
+    std::lock_guard<std::mutex> lk(na_call_mu[dev]);
+    int it = 0;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    HIPCHK(ctx, run_eigcent(g->R, g->C, g->V, N, tol, max_iter, output, xa, xb, part, stt, hstate, ec, lambda, &it,
+                            false, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // ec written, scratch free for the next call
     float ms = 0.f;
     HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
     ctx->tm.ec_ms = ms;

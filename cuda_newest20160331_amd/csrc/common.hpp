@@ -180,6 +180,10 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
                        double *lambda, int *iters, bool use_graph, hipStream_t st);
 size_t ec_state_bytes();
 int ec_nparts(int N);
+// Upper bounds of the iteration's scratch, for callers that size it up front:
+// the state struct's bytes, and ec_nparts(N) for every N up to 4194304.
+constexpr size_t EC_STATE_MAX_BYTES = 256;
+constexpr int EC_NPARTS_MAX = 16384;
 
 // Weighted clustering coefficient (k_cp.hip).  Columns must ascend within each
 // row.  w: nnz floats of scratch (the cube-rooted weights); key: one u64 of

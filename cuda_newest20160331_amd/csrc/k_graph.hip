@@ -87,6 +87,8 @@ struct EcState {
 
 constexpr int EC_ROWS_PER_BLOCK = 4;  // waves per workgroup, one row at a time each
 constexpr int EC_SPMV_BLOCKS = 16384;  // fixed grid, so the partial sums keep one order
+static_assert(sizeof(EcState) <= EC_STATE_MAX_BYTES, "EcState outgrew EC_STATE_MAX_BYTES");
+static_assert(EC_SPMV_BLOCKS <= EC_NPARTS_MAX && 4194304 / 256 <= EC_NPARTS_MAX, "ec_nparts bound");
 
 __global__ __launch_bounds__(256) void ec_init_kernel(double *x, int N)
 {

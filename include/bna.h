@@ -233,6 +233,22 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg);
 int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec,
                 double *lambda, int *iters);
 
+/* No counterpart in the reference: bna_eigcent for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * g must be a device CSR (on_device = 1, e.g. from bna_build_csr_series); ec is
+ * a DEVICE array of N doubles; lambda and iters as bna_eigcent, with the same
+ * results bit for bit.  The iterate vectors, partial sums and stopping state
+ * live in a static device array of the library (about 16.4 MiB per GPU, held
+ * until the process ends), so N is at most BNA_NOALLOC_EC_MAX_N (BNA_ERR_ARG
+ * beyond it) and no device or pinned memory is allocated.  The array keeps the
+ * last call's contents; every call initialises what it reads.  Calls on one GPU
+ * share the array and would race: the library serialises them (a lock per GPU),
+ * so overlapping calls wait.  Returns once ec is written (it synchronises the
+ * context's stream). */
+#define BNA_NOALLOC_EC_MAX_N 1048576
+int bna_eigcent_noalloc(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec,
+                        double *lambda, int *iters);
+
 /* Weighted clustering coefficient, the CUCP tool's Cp_2 (W/Cp_GPU/CUCP/Cp.cu:111-471):
  * w = cube root of V / V[m] (m the first entry of largest |V|), cci_i = (W^3)_ii,
  * Cp_i = cci_i / (t (t - 1)) with the reference's float t = (float)R[i+1] -
