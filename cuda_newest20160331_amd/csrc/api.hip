@@ -71,6 +71,7 @@ struct bna_ctx {
     hipEvent_t ev[10] = {};  // [0..7] builds, [8..9] graph maps
     std::map<std::string, DevBuf> ws;
     int grid = 256;  // persistent correlation workgroups: CUs rounded up to a multiple of 8
+    int ncu = 256;   // compute units of the device
     int grid_limit = 0;  // > 0: launches use at most this many (a multiple of 8)
     bool graphs = true;  // replay captured HIP graphs where a path has them (BNA_GRAPHS=0: off)
     // tile traversal band height per (precision, L bucket), from the autotuner
@@ -428,8 +429,10 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
         (void)hipMemPoolSetAttribute(ctx->pool, hipMemPoolReuseAllowInternalDependencies, &no);
     }
     int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0) {
         ctx->grid = (ncu + 7) / 8 * 8;
+        ctx->ncu = ncu;
+    }
     {
         const char *g = std::getenv("BNA_GRAPHS");
         ctx->graphs = !(g && g[0] == '0');
@@ -1542,7 +1545,7 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg)
     float *d;
     if ((rc = ws_get_t(ctx, "deg", (size_t)g->N, &d))) return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
-    HIPCHK(ctx, launch_degree(R, V, g->N, d, ctx->stream));
+    HIPCHK(ctx, launch_degree(R, V, g->N, g->nnz, d, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     int rc2 = copy_d2h(ctx, deg, d, sizeof(float) * (size_t)g->N);  // through pinned memory; synchronises
     if (rc2) return rc2;
@@ -1596,6 +1599,14 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     int rc = device_view(ctx, g, &R, &C, &V);
     if (rc) return rc;
     const int N = g->N;
+    const EcPlan plan = ec_plan(N, g->nnz, ctx->ncu);
+    EcBlocked bl;
+    if (plan.blocked) {
+        if ((rc = ws_get_t(ctx, "ec_qc", (size_t)N * plan.S, &bl.qc))) return rc;
+        if ((rc = ws_get_t(ctx, "ec_qoff", (size_t)plan.items + 1, &bl.qoff))) return rc;
+        if ((rc = ws_get_t(ctx, "ec_cq", (size_t)plan.quads * 4, &bl.cq))) return rc;
+        if (V && (rc = ws_get_t(ctx, "ec_vq", (size_t)plan.quads * 4, &bl.vq))) return rc;
+    }
     double *xa, *xb, *part, *outd;
     void *stt;
     if ((rc = ws_get_t(ctx, "ec_xa", (size_t)N, &xa))) return rc;
@@ -1609,8 +1620,8 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if ((rc = pin_grow(ctx, 2, ec_state_bytes()))) return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
-    HIPCHK(ctx, run_eigcent(R, C, V, N, tol, max_iter, output, xa, xb, part, stt, ctx->pin[2], outd, lambda, &it,
-                            ctx->graphs, ctx->stream));
+    HIPCHK(ctx, run_eigcent(R, C, V, N, plan, bl, tol, max_iter, output, xa, xb, part, stt, ctx->pin[2], outd,
+                            lambda, &it, ctx->graphs, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     int rc2 = copy_d2h(ctx, ec, outd, sizeof(double) * (size_t)N);  // through pinned memory; synchronises
     if (rc2) return rc2;
@@ -1649,13 +1660,14 @@ int bna_eigcent_noalloc(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter
     double *xb = (double *)(base + NA_X_BYTES);
     double *part = (double *)(base + 2 * NA_X_BYTES);
     void *stt = base + 2 * NA_X_BYTES + NA_PART_BYTES;
+    const EcPlan plan;  // the plain SpMV: the blocked layout needs a copy of the graph
     // the host side of the stopping state: pageable, read after a stream sync
     alignas(16) unsigned char hstate[EC_STATE_MAX_BYTES];
     std::lock_guard<std::mutex> lk(na_call_mu[dev]);
     int it = 0;
     HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
-    HIPCHK(ctx, run_eigcent(g->R, g->C, g->V, N, tol, max_iter, output, xa, xb, part, stt, hstate, ec, lambda, &it,
-                            false, ctx->stream));
+    HIPCHK(ctx, run_eigcent(g->R, g->C, g->V, N, plan, EcBlocked{}, tol, max_iter, output, xa, xb, part, stt, hstate,
+                            ec, lambda, &it, false, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // ec written, scratch free for the next call
     float ms = 0.f;

@@ -172,16 +172,37 @@ hipError_t launch_place(const int32_t *frag_orig, const int32_t *frag_live, cons
                         float *V, hipStream_t st);
 
 // Degree and eigenvector centrality (kernel d).
-hipError_t launch_degree(const int32_t *R, const float *V, int N, float *deg, hipStream_t st);
+hipError_t launch_degree(const int32_t *R, const float *V, int N, int64_t nnz, float *deg, hipStream_t st);
+// How the power iteration's SpMV runs: segment-blocked with x staged in LDS
+// (blocked), or the plain L2-gather kernel.  blocked needs N >= 4096, 16 or
+// more entries per row on average and a layout that fits LDS: G workgroups of
+// RB rows, S segments of SEG columns, lds bytes of dynamic LDS; items = G S RB
+// (block, segment, row) slices and at most quads 4-entry quads.
+struct EcPlan {
+    bool blocked = false;
+    int G = 0, RB = 0, S = 0, SEG = 0;
+    size_t lds = 0;
+    int64_t items = 0, quads = 0;
+};
+EcPlan ec_plan(int N, int64_t nnz, int ncu);
+// Workspace of the blocked plan: qc N S ints, qoff items + 1 ints, cq 4 quads
+// uint16, vq 4 quads floats (weighted graphs only).
+struct EcBlocked {
+    int32_t *qc = nullptr, *qoff = nullptr;
+    uint16_t *cq = nullptr;
+    float *vq = nullptr;
+};
 // Whole power iteration (see k_graph.hip).  output: 0 last A*y, 1 last y.
 // use_graph: replay batches of 16 iterations from one captured HIP graph.
-hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, double tol, int max_iter,
-                       int output, double *xa, double *xb, double *part, void *state_dev, void *state_host, double *out,
-                       double *lambda, int *iters, bool use_graph, hipStream_t st);
+hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, const EcPlan &plan,
+                       const EcBlocked &bl, double tol, int max_iter, int output, double *xa, double *xb, double *part,
+                       void *state_dev, void *state_host, double *out, double *lambda, int *iters, bool use_graph,
+                       hipStream_t st);
 size_t ec_state_bytes();
 int ec_nparts(int N);
 // Upper bounds of the iteration's scratch, for callers that size it up front:
-// the state struct's bytes, and ec_nparts(N) for every N up to 4194304.
+// the state struct's bytes, and ec_nparts(N) for every N up to 4194304 (the
+// blocked plan uses G <= CU count partials, fewer than either).
 constexpr size_t EC_STATE_MAX_BYTES = 256;
 constexpr int EC_NPARTS_MAX = 16384;
 

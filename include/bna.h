@@ -236,10 +236,13 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
 /* No counterpart in the reference: bna_eigcent for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc).
  * g must be a device CSR (on_device = 1, e.g. from bna_build_csr_series); ec is
- * a DEVICE array of N doubles; lambda and iters as bna_eigcent, with the same
- * results bit for bit.  The iterate vectors, partial sums and stopping state
- * live in a static device array of the library (about 16.4 MiB per GPU, held
- * until the process ends), so N is at most BNA_NOALLOC_EC_MAX_N (BNA_ERR_ARG
+ * a DEVICE array of N doubles; lambda and iters as bna_eigcent.  Its SpMV is
+ * the plain L2-gather kernel (bna_eigcent copies a graph with 16 or more
+ * entries per row into a blocked layout first, which needs memory), so the
+ * results agree with bna_eigcent's to FP64 rounding, with the same stopping
+ * rule.  The iterate vectors, partial sums and stopping state live in a static
+ * device array of the library (about 16.4 MiB per GPU, held until the process
+ * ends), so N is at most BNA_NOALLOC_EC_MAX_N (BNA_ERR_ARG
  * beyond it) and no device or pinned memory is allocated.  The array keeps the
  * last call's contents; every call initialises what it reads.  Calls on one GPU
  * share the array and would race: the library serialises them (a lock per GPU),

@@ -199,6 +199,54 @@ def test_degree_and_ec_parity(ctx):
         assert abs(it - oit) <= 1
 
 
+def ragged_graph(N, m, seed, hub_deg):
+    """Symmetric graph with positive weights: m random edges, one hub joined to
+    hub_deg voxels spread over every column range, a run of empty rows, and
+    the columns of every row shuffled (a host CSR may list them in any order)."""
+    rng = np.random.default_rng(seed)
+    i = rng.integers(0, N, m)
+    j = rng.integers(0, N, m)
+    hub = N // 3
+    hj = rng.choice(N, hub_deg, replace=False)
+    i = np.concatenate([i, np.full(hub_deg, hub)])
+    j = np.concatenate([j, hj])
+    empty = (i >= 100) & (i < 140) | (j >= 100) & (j < 140)
+    keep = (i != j) & ~empty
+    a, b = np.minimum(i[keep], j[keep]), np.maximum(i[keep], j[keep])
+    key = np.unique(a.astype(np.int64) * N + b)
+    a, b = key // N, key % N
+    w = rng.uniform(0.05, 1.0, len(a)).astype(np.float32)
+    rows = np.concatenate([a, b])
+    cols = np.concatenate([b, a]).astype(np.int32)
+    vals = np.concatenate([w, w])
+    order = np.lexsort((rng.random(len(rows)), rows))  # by row, columns shuffled
+    g = bna.Graph.__new__(bna.Graph)
+    g.N, g.nnz = N, len(rows)
+    g.R = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=N))]).astype(np.int64)
+    g.C = cols[order]
+    g.V = vals[order]
+    return g
+
+
+@pytest.mark.parametrize("N,m", [(6000, 60000), (24000, 260000)])
+def test_ec_blocked_layout_ragged(ctx, N, m):
+    # 20 or more entries per row on average: bna_eigcent copies the graph
+    # into its segment-blocked layout (one column segment at N = 6000, two at
+    # N = 24000); empty rows, a hub row across all segments and unsorted
+    # columns must give the oracle's vector in both output modes
+    g = ragged_graph(N, m, 7 + N, hub_deg=N // 2)
+    assert g.nnz >= 16 * N
+    say(f"EC blocked layout: N={N} nnz={g.nnz} max row {np.diff(g.R).max()}")
+    for out_mode, binary in [(bna.EC_WEIGHTED_REF, False), (bna.EC_BINARY_REF, True)]:
+        ec, lam, it = ctx.eigcent(g, output=out_mode, binary=binary)
+        oec, olam, oit = orc.eigcent(g.R, g.C, None if binary else g.V, N, out_mode=out_mode)
+        rel = np.linalg.norm(ec - oec) / np.linalg.norm(oec)
+        say(f"  mode={out_mode} iters={it}/{oit} lambda={lam:.12g}/{olam:.12g} relL2={rel:.3g}")
+        assert rel < 1e-9
+        assert lam == pytest.approx(olam, rel=1e-12)
+        assert it == oit
+
+
 def test_ec_graph_replay_matches_launches(ctx, monkeypatch):
     """The power iteration replays captured 16-iteration graphs; with graphs
     off (BNA_GRAPHS=0 at context creation) every kernel is launched directly.

@@ -2,10 +2,14 @@
 bna_eigcent on the same device CSRs, through the C ABI.
 
 Two calls in a row on a context that never built anything (so it owns no
-workspace), the second on a smaller graph: eigenvector, lambda and iteration
-count must equal bna_eigcent's bit for bit (the same kernels in the same order),
-and the context's workspace must still be empty afterwards.  Sizes beyond the
-static scratch and host CSRs are refused on the host before any launch.
+workspace), the second on a smaller graph.  The no-alloc path runs the plain
+L2-gather SpMV while bna_eigcent copies these graphs (16 or more entries per
+row) into its blocked layout, so the two sum each row in a different order:
+the iteration counts must be equal and the eigenvector and lambda must agree
+within 1e-12 relative (FP64 rounding; the bar for eigenvector centrality is
+1e-5 relative L2).  The context's workspace must still be empty afterwards.
+Sizes beyond the static scratch and host CSRs are refused on the host before
+any launch.
 """
 import ctypes as C
 import sys
@@ -69,9 +73,11 @@ def test_noalloc_eigcent_equals_eigcent_twice_second_smaller():
             finally:
                 h.hipFree(ec_dev)
             say(f"  nnz={g.nnz} iterations {it} (eigcent {it_ref}) lambda {lam!r} ({lam_ref!r})")
+            rel = np.linalg.norm(ec - ec_ref) / np.linalg.norm(ec_ref)
+            say(f"  rel L2 vs eigcent {rel:.2e}")
             assert it == it_ref and it > 0
-            assert lam == lam_ref
-            assert np.array_equal(ec.view(np.uint64), ec_ref.view(np.uint64))
+            assert abs(lam - lam_ref) <= 1e-12 * abs(lam_ref)
+            assert rel <= 1e-12
             assert user.workspace_bytes() == 0
         # refused on the host, before any launch
         s, g = keep[0]
