@@ -91,6 +91,7 @@ struct bna_ctx {
     void (*gate)(void *) = nullptr;
     void *gate_arg = nullptr;
     hipEvent_t chev[2] = {};
+    hipEvent_t ecev[2] = {};  // the power iteration's two in-flight batches
     // pinned double buffer for host <-> device copies of pageable memory
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stev[2] = {};
@@ -389,7 +390,8 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
         return fail(nullptr, BNA_ERR_HIP, m);
     }
     scratch_setup(device);  // a failure is reported by bna_eigcent_noalloc
-    for (auto &ev : ctx->chev) {
+    for (int b = 0; b < 4; b++) {
+        hipEvent_t &ev = b < 2 ? ctx->chev[b] : ctx->ecev[b - 2];
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
             std::string m = std::string("bna_create: ") + hipGetErrorString(e);
             bna_destroy(ctx);
@@ -464,6 +466,7 @@ int bna_destroy(bna_ctx *ctx)
         if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
         if (ctx->stev[b]) (void)hipEventDestroy(ctx->stev[b]);
         if (ctx->chev[b]) (void)hipEventDestroy(ctx->chev[b]);
+        if (ctx->ecev[b]) (void)hipEventDestroy(ctx->ecev[b]);
         if (ctx->pin_ev[b]) (void)hipEventDestroy(ctx->pin_ev[b]);
     }
     for (int b = 0; b < 3; b++)
@@ -1604,6 +1607,7 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     if (plan.blocked) {
         if ((rc = ws_get_t(ctx, "ec_qc", (size_t)N * plan.S, &bl.qc))) return rc;
         if ((rc = ws_get_t(ctx, "ec_qoff", (size_t)plan.items + 1, &bl.qoff))) return rc;
+        if ((rc = ws_get_t(ctx, "ec_btot", (size_t)plan.G, &bl.btot))) return rc;
         if ((rc = ws_get_t(ctx, "ec_cq", (size_t)plan.quads * 4, &bl.cq))) return rc;
         if (V && (rc = ws_get_t(ctx, "ec_vq", (size_t)plan.quads * 4, &bl.vq))) return rc;
     }
@@ -1618,10 +1622,10 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
     // the iteration's state crosses through pinned memory: make sure no earlier
     // copy through that buffer is still in flight before it is reused
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    if ((rc = pin_grow(ctx, 2, ec_state_bytes()))) return rc;
+    if ((rc = pin_grow(ctx, 2, 2 * EC_STATE_MAX_BYTES))) return rc;
     HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
     HIPCHK(ctx, run_eigcent(R, C, V, N, plan, bl, tol, max_iter, output, xa, xb, part, stt, ctx->pin[2], outd,
-                            lambda, &it, ctx->graphs, ctx->stream));
+                            lambda, &it, ctx->graphs, ctx->ecev, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     int rc2 = copy_d2h(ctx, ec, outd, sizeof(double) * (size_t)N);  // through pinned memory; synchronises
     if (rc2) return rc2;
@@ -1661,13 +1665,14 @@ int bna_eigcent_noalloc(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter
     double *part = (double *)(base + 2 * NA_X_BYTES);
     void *stt = base + 2 * NA_X_BYTES + NA_PART_BYTES;
     const EcPlan plan;  // the plain SpMV: the blocked layout needs a copy of the graph
-    // the host side of the stopping state: pageable, read after a stream sync
-    alignas(16) unsigned char hstate[EC_STATE_MAX_BYTES];
+    // the host side of the stopping state (two slots): pageable, read after
+    // the copy into it has completed
+    alignas(16) unsigned char hstate[2 * EC_STATE_MAX_BYTES];
     std::lock_guard<std::mutex> lk(na_call_mu[dev]);
     int it = 0;
     HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
     HIPCHK(ctx, run_eigcent(g->R, g->C, g->V, N, plan, EcBlocked{}, tol, max_iter, output, xa, xb, part, stt, hstate,
-                            ec, lambda, &it, false, ctx->stream));
+                            ec, lambda, &it, false, ctx->ecev, ctx->stream));
     HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));  // ec written, scratch free for the next call
     float ms = 0.f;

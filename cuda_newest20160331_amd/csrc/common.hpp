@@ -185,19 +185,22 @@ struct EcPlan {
     int64_t items = 0, quads = 0;
 };
 EcPlan ec_plan(int N, int64_t nnz, int ncu);
-// Workspace of the blocked plan: qc N S ints, qoff items + 1 ints, cq 4 quads
-// uint16, vq 4 quads floats (weighted graphs only).
+// Workspace of the blocked plan: qc N S ints, qoff items + 1 ints, btot G
+// ints, cq 4 quads uint16, vq 4 quads floats (weighted graphs only).
 struct EcBlocked {
-    int32_t *qc = nullptr, *qoff = nullptr;
+    int32_t *qc = nullptr, *qoff = nullptr, *btot = nullptr;
     uint16_t *cq = nullptr;
     float *vq = nullptr;
 };
 // Whole power iteration (see k_graph.hip).  output: 0 last A*y, 1 last y.
-// use_graph: replay batches of 16 iterations from one captured HIP graph.
+// state_host: two EcState slots (2 * EC_STATE_MAX_BYTES bytes), pinned for the
+// batches to overlap; ev: two events for the batches in flight.
+// use_graph: replay batches of 16 iterations from one captured HIP graph (the
+// plain plan only: the blocked plan's launches are long enough on their own).
 hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, const EcPlan &plan,
                        const EcBlocked &bl, double tol, int max_iter, int output, double *xa, double *xb, double *part,
                        void *state_dev, void *state_host, double *out, double *lambda, int *iters, bool use_graph,
-                       hipStream_t st);
+                       const hipEvent_t ev[2], hipStream_t st);
 size_t ec_state_bytes();
 int ec_nparts(int N);
 // Upper bounds of the iteration's scratch, for callers that size it up front:

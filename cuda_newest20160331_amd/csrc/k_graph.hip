@@ -385,38 +385,76 @@ __global__ __launch_bounds__(256) void ec_blk_count_kernel(const int32_t *__rest
 }
 
 // Exclusive scan of the quad counts in (block, segment, row) order into
-// qoff[0 .. items]; rows past N count 0.  One workgroup of 1024 threads, each
-// scanning a contiguous run of items.
-__global__ __launch_bounds__(1024) void ec_blk_scan_kernel(const int32_t *__restrict__ qc, int N, int RB, int S,
-                                                          int64_t items, int32_t *__restrict__ qoff)
+// qoff[0 .. items]; rows past N count 0.  Three steps: each workgroup scans
+// its block's S RB items (local offsets, block total into btot[b]); one
+// workgroup scans the G block totals into btot (and writes qoff[items]); each
+// workgroup adds its block's base.
+__device__ inline int32_t blk_qc(const int32_t *__restrict__ qc, int N, int RB, int S, int b, int k)
 {
-    __shared__ int32_t sums[1024];
+    const int sg = k / RB, i = k - sg * RB;
+    const int64_t r = (int64_t)b * RB + i;
+    return r < N ? qc[r * S + sg] : 0;
+}
+
+// Block-wide exclusive scan of one value per thread (1024 threads); returns
+// the thread's exclusive prefix and the total through *tot.
+__device__ inline int32_t blk_excl_scan(int32_t v, int32_t *sh, int32_t *tot)
+{
     const int t = threadIdx.x;
-    const int64_t per = (items + 1023) / 1024;
-    const int64_t k0 = min(items, t * per), k1 = min(items, k0 + per);
-    auto val = [&](int64_t k) -> int32_t {
-        const int64_t bs = k / RB;
-        const int i = (int)(k - bs * RB);
-        const int b = (int)(bs / S), sg = (int)(bs - (int64_t)b * S);
-        const int64_t r = (int64_t)b * RB + i;
-        return r < N ? qc[r * S + sg] : 0;
-    };
-    int32_t local = 0;
-    for (int64_t k = k0; k < k1; k++) local += val(k);
-    sums[t] = local;
+    sh[t] = v;
     __syncthreads();
     for (int d = 1; d < 1024; d <<= 1) {
-        const int32_t v = t >= d ? sums[t - d] : 0;
+        const int32_t u = t >= d ? sh[t - d] : 0;
         __syncthreads();
-        sums[t] += v;
+        sh[t] += u;
         __syncthreads();
     }
-    int32_t run = sums[t] - local;  // exclusive prefix of this thread's run
-    for (int64_t k = k0; k < k1; k++) {
-        qoff[k] = run;
-        run += val(k);
+    const int32_t inc = sh[t];
+    *tot = sh[1023];
+    __syncthreads();
+    return inc - v;
+}
+
+__global__ __launch_bounds__(1024) void ec_blk_scan_local_kernel(const int32_t *__restrict__ qc, int N, int RB,
+                                                                 int S, int32_t *__restrict__ qoff,
+                                                                 int32_t *__restrict__ btot)
+{
+    __shared__ int32_t sh[1024];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int nq = S * RB;
+    const int per = (nq + 1023) / 1024;
+    const int k0 = min(nq, t * per), k1 = min(nq, k0 + per);
+    int32_t local = 0;
+    for (int k = k0; k < k1; k++) local += blk_qc(qc, N, RB, S, b, k);
+    int32_t tot;
+    int32_t run = blk_excl_scan(local, sh, &tot);
+    int32_t *q = qoff + (int64_t)b * nq;
+    for (int k = k0; k < k1; k++) {
+        q[k] = run;
+        run += blk_qc(qc, N, RB, S, b, k);
     }
-    if (t == 1023) qoff[items] = sums[1023];
+    if (t == 0) btot[b] = tot;
+}
+
+// G <= 1024 block totals -> exclusive bases in place; qoff[items] = total.
+__global__ __launch_bounds__(1024) void ec_blk_scan_base_kernel(int G, int64_t items, int32_t *__restrict__ qoff,
+                                                                int32_t *__restrict__ btot)
+{
+    __shared__ int32_t sh[1024];
+    const int t = threadIdx.x;
+    const int32_t v = t < G ? btot[t] : 0;
+    int32_t tot;
+    const int32_t base = blk_excl_scan(v, sh, &tot);
+    if (t < G) btot[t] = base;
+    if (t == 0) qoff[items] = tot;
+}
+
+__global__ __launch_bounds__(1024) void ec_blk_scan_add_kernel(int nq, int32_t *__restrict__ qoff,
+                                                               const int32_t *__restrict__ btot)
+{
+    const int32_t base = btot[blockIdx.x];
+    int32_t *q = qoff + (int64_t)blockIdx.x * nq;
+    for (int k = threadIdx.x; k < nq; k += 1024) q[k] += base;
 }
 
 // Copies each row's entries to their quads, in CSR order within a segment,
@@ -630,7 +668,7 @@ namespace bna {
 EcPlan ec_plan(int N, int64_t nnz, int ncu)
 {
     EcPlan p{};
-    if (N < BLK_MIN_N || nnz < 16 * (int64_t)N || ncu < 1) return p;
+    if (N < BLK_MIN_N || nnz < 16 * (int64_t)N || ncu < 1 || ncu > 1024) return p;  // G <= ncu totals in one scan
     const int RB = (N + ncu - 1) / ncu;
     for (int S = 1; S <= BLK_MAX_S; S++) {
         const int SEG = (((N + S - 1) / S) + 7) / 8 * 8;
@@ -655,10 +693,11 @@ EcPlan ec_plan(int N, int64_t nnz, int ncu)
 hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N, const EcPlan &plan,
                        const EcBlocked &bl, double tol, int max_iter, int output, double *xa, double *xb, double *part,
                        void *state_dev, void *state_host, double *out, double *lambda, int *iters, bool use_graph,
-                       hipStream_t st)
+                       const hipEvent_t ev[2], hipStream_t st)
 {
     EcState *sd = (EcState *)state_dev;
-    EcState &h = *(EcState *)state_host;
+    EcState *hs[2] = {(EcState *)state_host, (EcState *)((unsigned char *)state_host + EC_STATE_MAX_BYTES)};
+    EcState &h = *hs[0];
     h = EcState{};
     h.tol = tol;
     h.max_iter = max_iter;
@@ -675,8 +714,11 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
     if (plan.blocked) {
         const unsigned row_waves = (unsigned)((N + 3) / 4);
         hipLaunchKernelGGL(ec_blk_count_kernel, dim3(row_waves), dim3(256), 0, st, R, C, N, plan.S, plan.SEG, bl.qc);
-        hipLaunchKernelGGL(ec_blk_scan_kernel, dim3(1), dim3(1024), 0, st, bl.qc, N, plan.RB, plan.S, plan.items,
-                           bl.qoff);
+        const int nq = plan.S * plan.RB;
+        hipLaunchKernelGGL(ec_blk_scan_local_kernel, dim3(plan.G), dim3(1024), 0, st, bl.qc, N, plan.RB, plan.S,
+                           bl.qoff, bl.btot);
+        hipLaunchKernelGGL(ec_blk_scan_base_kernel, dim3(1), dim3(1024), 0, st, plan.G, plan.items, bl.qoff, bl.btot);
+        hipLaunchKernelGGL(ec_blk_scan_add_kernel, dim3(plan.G), dim3(1024), 0, st, nq, bl.qoff, bl.btot);
         hipLaunchKernelGGL(ec_blk_scatter_kernel, dim3(row_waves), dim3(256), 0, st, R, C, V, N, plan.RB, plan.S,
                            plan.SEG, bl.qoff, bl.cq, V ? bl.vq : nullptr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -703,7 +745,7 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
     // every batch when graphs are off) is launched directly.
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    if (use_graph && max_iter >= batch) {
+    if (use_graph && !plan.blocked && max_iter >= batch) {
         if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
         issue(0, batch);
         e = hipStreamEndCapture(st, &graph);
@@ -713,24 +755,53 @@ hipError_t run_eigcent(const int32_t *R, const int32_t *C, const float *V, int N
             return e;
         }
     }
-    int launched = 0;
-    while (launched < max_iter) {
+    // Two batches in flight: batch b + 1 is queued before the host waits for
+    // batch b's copy of the state, so the device never idles on the host.
+    // Once the stopping rule fires, the kernels of a batch already queued
+    // return at once; the state copied after batch b is final if it says done.
+    int launched = 0, queued = 0, oldest = 0;
+    int slot_of[2] = {0, 1};
+    auto enqueue = [&](int slot) -> hipError_t {
         const int upto = (launched + batch < max_iter) ? launched + batch : max_iter;
+        hipError_t r;
         if (exec && upto - launched == batch)
-            e = hipGraphLaunch(exec, st);
+            r = hipGraphLaunch(exec, st);
         else {
             issue(launched, upto);
-            e = hipGetLastError();
+            r = hipGetLastError();
         }
-        if (e != hipSuccess) break;
+        if (r != hipSuccess) return r;
         launched = upto;
-        if ((e = hipMemcpyAsync(&h, sd, sizeof(EcState), hipMemcpyDeviceToHost, st)) != hipSuccess) break;
-        if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
-        if (h.done) break;
+        if ((r = hipMemcpyAsync(hs[slot], sd, sizeof(EcState), hipMemcpyDeviceToHost, st)) != hipSuccess) return r;
+        return hipEventRecord(ev[slot], st);
+    };
+    const EcState *fin = nullptr;
+    if (max_iter > 0) {
+        e = enqueue(0);
+        queued = 1;
+        oldest = 0;
     }
+    while (e == hipSuccess && queued > 0) {
+        const int s_old = slot_of[oldest & 1];
+        if (queued == 1 && launched < max_iter) {
+            e = enqueue(s_old ^ 1);
+            if (e != hipSuccess) break;
+            slot_of[(oldest + 1) & 1] = s_old ^ 1;
+            queued = 2;
+        }
+        if ((e = hipEventSynchronize(ev[s_old])) != hipSuccess) break;
+        queued--;
+        oldest++;
+        if (hs[s_old]->done || queued == 0) {
+            fin = hs[s_old];
+            break;
+        }
+    }
+    if (e == hipSuccess && queued > 0) e = hipStreamSynchronize(st);  // the batch behind the final one
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return e;
+    if (fin) h = *fin;
     const int K = h.iter;  // iterations actually executed
     if (K == 0) {
         // no iteration: the reference returns x as initialised and n = 0
