@@ -492,6 +492,20 @@ def main() -> int:
         "p99_subject_latency_ms": None,
         "stages_ms": b["stages_ms"],
     }
+    # kernel (c), the CSR compaction (fragment rows, scan, placement): its
+    # algorithmic bytes are the CSR it writes, 8 nnz + 4 (N + 1); an edge is
+    # two entries, 16 bytes, so HBM bounds it at 8 TB/s / 16 B = 5e11 edges/s
+    nnz = b["stats"][0]
+    csr_s = b["stages_ms"]["csr_ms"] / 1e3
+    csr_bytes = 8 * nnz + 4 * (N + 1)
+    csr_gbs = csr_bytes / csr_s / 1e9 if csr_s > 0 else None
+    line["csr"] = {"ms": b["stages_ms"]["csr_ms"], "bytes": csr_bytes, "GB_per_s": csr_gbs,
+                   "roofline_frac": csr_gbs / 8000.0 if csr_gbs else None,
+                   "edges_per_s": (nnz // 2) / csr_s if csr_s > 0 else None,
+                   "edges_roofline_frac": ((nnz // 2) / csr_s) / (8e12 / 16) if csr_s > 0 else None,
+                   "note": "fragment rows + scan + placement (kernel c), HIP events; bytes = 8 nnz + 4 (N + 1)"}
+    line["csr_roofline_frac"] = line["csr"]["roofline_frac"]
+    line["edges_roofline_frac"] = line["edges_per_s"] / (8e12 / 16)
     if "fp32" in res:
         f = res["fp32"]
         fval = M1 / (f["ms"] / 1e3)
