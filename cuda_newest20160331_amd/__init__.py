@@ -38,6 +38,7 @@ EXPORTS = [
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
     "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
+    "bna_modularity",
 ]
 
 
@@ -65,6 +66,7 @@ class Timings(C.Structure):
         ("total_ms", C.c_double),
         ("corr_launches", C.c_int64),
         ("degree_ms", C.c_double), ("ec_ms", C.c_double), ("ec_iters", C.c_int64), ("cp_ms", C.c_double),
+        ("modularity_ms", C.c_double), ("modularity_iters", C.c_int64),
     ]
 
 
@@ -105,6 +107,8 @@ def lib() -> C.CDLL:
                               C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "bna_eigcent_noalloc": (i32, [vp, C.POINTER(CSR), dbl, i32, i32, vp,
                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "bna_modularity": (i32, [vp, C.POINTER(CSR), C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int)]),
         "bna_density_to_k": (i64, [i32, C.c_float]),
         "bna_synth_factor_model": (i32, [fp, i32, i32, i32, dbl, C.c_uint64, i32]),
         "bna_write_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
@@ -386,6 +390,21 @@ class Context:
                                         ec.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam),
                                         C.byref(it)), "bna_eigcent")
         return ec, lam.value, it.value
+
+    def modularity(self, g: "Graph | CSR"):
+        """bna_modularity: (labels numbered from 1, Q, module count) of a host
+        Graph or a device-resident bna_csr (taken as binary)."""
+        if isinstance(g, CSR):
+            ref, N = g, g.N
+        else:
+            h = _host_csr(g, drop_values=True)
+            ref, N = h[0], g.N
+        lab = np.empty(N, np.int32)
+        q = C.c_double()
+        mods = C.c_int()
+        self._check(self._L.bna_modularity(self._h, C.byref(ref), lab.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           C.byref(q), C.byref(mods)), "bna_modularity")
+        return lab, q.value, mods.value
 
     def eigcent_noalloc(self, g: CSR, ec_dev: int, tol: float = 1e-6, max_iter: int = 10000,
                         output: int = EC_WEIGHTED_REF):

@@ -1558,6 +1558,56 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg)
     return BNA_OK;
 }
 
+int bna_modularity(bna_ctx *ctx, const bna_csr *g, int32_t *labels, double *Q, int *n_modules)
+{
+    if (!ctx || !g || !labels || !Q) return fail(ctx, BNA_ERR_ARG, "bna_modularity: NULL argument");
+    if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_modularity: empty graph");
+    if (g->nnz < 2) return fail(ctx, BNA_ERR_ARG, "bna_modularity: the graph has no edge");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    const int N = g->N;
+    const size_t nz = (size_t)g->nnz;
+    NmBuffers b;
+    if ((rc = ws_get_t(ctx, "nm_AD", (size_t)N, &b.AD))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_cnt", (size_t)N, &b.cnt))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_res", (size_t)N, &b.res))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_SR", (size_t)N + 1, &b.SR))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_SC", nz, &b.SC))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_sumBG", (size_t)N, &b.sumBG))) return rc;
+    const char *yn[3] = {"nm_y0", "nm_y1", "nm_y2"}, *ygn[2] = {"nm_yg0", "nm_yg1"};
+    for (int q = 0; q < 3; q++)
+        if ((rc = ws_get_t(ctx, yn[q], (size_t)N, &b.y[q]))) return rc;
+    for (int q = 0; q < 2; q++)
+        if ((rc = ws_get_t(ctx, ygn[q], (size_t)N, &b.yg[q]))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_part", (size_t)4 * NM_BLOCKS_MAX, &b.part))) return rc;
+    if ((rc = ws_get(ctx, "nm_state", nm_state_bytes(), &b.state))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_same", 1, &b.same))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = pin_grow(ctx, 2, 2 * EC_STATE_MAX_BYTES))) return rc;
+    b.state_host = ctx->pin[2];
+    b.ev[0] = ctx->ecev[0];
+    b.ev[1] = ctx->ecev[1];
+    std::vector<int32_t> rh;
+    if (g->on_device) {
+        rh.resize((size_t)N + 1);
+        if ((rc = copy_d2h(ctx, rh.data(), g->R, sizeof(int32_t) * ((size_t)N + 1)))) return rc;
+        b.R_host = rh.data();
+    } else {
+        b.R_host = g->R;
+    }
+    int mods = 0, rounds = 0;
+    int64_t its = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(ctx, run_modularity(R, C, N, g->nnz, 10000, 1e-6, b, labels, Q, &mods, &its, &rounds, ctx->stream));
+    ctx->tm.modularity_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->tm.modularity_iters = its;
+    if (n_modules) *n_modules = mods;
+    return BNA_OK;
+}
+
 int bna_cp(bna_ctx *ctx, const bna_csr *g, float *cp, double *mean)
 {
     if (!ctx || !g || !cp) return fail(ctx, BNA_ERR_ARG, "bna_cp: NULL argument");

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace bna {
 
 // Correlation tile edge (voxels), the K padding of the operand panels and the
@@ -208,6 +210,29 @@ int ec_nparts(int N);
 // blocked plan uses G <= CU count partials, fewer than either).
 constexpr size_t EC_STATE_MAX_BYTES = 256;
 constexpr int EC_NPARTS_MAX = 16384;
+
+// Newman leading-eigenvector modularity (k_modularity.hip).  Device buffers:
+// AD, cnt, res N ints; SR N + 1 ints; SC nnz ints; sumBG, y[0..2], yg[0..1]
+// N doubles; part 4 NM_BLOCKS_MAX doubles; state nm_state_bytes(); same one
+// u64.  state_host: pinned, two states; ev: two events.  R_host: the CSR's row
+// offsets on the host.
+constexpr int NM_BLOCKS_MAX = 8192;
+struct NmBuffers {
+    int32_t *AD = nullptr, *cnt = nullptr, *res = nullptr, *SR = nullptr, *SC = nullptr;
+    double *sumBG = nullptr, *y[3] = {}, *yg[2] = {}, *part = nullptr;
+    void *state = nullptr, *state_host = nullptr;
+    unsigned long long *same = nullptr;
+    hipEvent_t ev[2] = {};
+    std::vector<int32_t> AD_host;
+    const int32_t *R_host = nullptr;
+};
+size_t nm_state_bytes();
+// The whole partition: labels (host, N) numbered from 1, Q, module count,
+// power iterations over all rounds, rounds.  V is not used (the reference
+// tool reads binary graphs).
+hipError_t run_modularity(const int32_t *R, const int32_t *C, int N, int64_t nnz, int max_iter, double eps,
+                          NmBuffers &b, int32_t *labels, double *Q, int *modules, int64_t *iters_total,
+                          int *rounds_out, hipStream_t st);
 
 // Weighted clustering coefficient (k_cp.hip).  Columns must ascend within each
 // row.  w: nnz floats of scratch (the cube-rooted weights); key: one u64 of

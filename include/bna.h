@@ -114,6 +114,8 @@ typedef struct bna_timings {
     double ec_ms;      /* last bna_eigcent: the whole power iteration on the device */
     int64_t ec_iters;  /* last bna_eigcent: iterations */
     double cp_ms;      /* last bna_cp: weight pass and row kernel */
+    double modularity_ms;      /* last bna_modularity: the whole partition */
+    int64_t modularity_iters;  /* last bna_modularity: power iterations over all rounds */
 } bna_timings;
 
 /* ----------------------------------------------------------------- context */
@@ -232,6 +234,22 @@ int bna_degree(bna_ctx *ctx, const bna_csr *g, float *deg);
  * (V == NULL) uses weight 1 for every entry. */
 int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec,
                 double *lambda, int *iters);
+
+/* Newman leading-eigenvector modularity, the CUmodularity tool
+ * (B/Newman_Modularity_GPU/CUmodularity/modularity_GPU.cu:408-528 Partition_GPU,
+ * :332-400 Sub_Partition_GPU, Modularity_CPU.cpp:176-272 Sub_Partition and
+ * Lead_Vector): repeated bisection by the sign of the leading eigenvector of
+ * the group's generalised modularity matrix (power iteration, eps 1e-6, at
+ * most 10000 iterations; a negative leading eigenvalue is shifted out), a
+ * division kept when the eigenvalue exceeds 0.01 and each side has two or more
+ * nodes.  labels (host, N ints) gets each voxel's module, numbered from 1 in
+ * the order the modules are found final (the _modu.nm values); Q the
+ * modularity of the partition; n_modules (may be NULL) the module count.  The
+ * graph is taken as binary (V is not used), as the reference tool reads it.
+ * The power iteration starts from v_i = i (the weighted tree's Lead_Vector_GPU,
+ * W/.../modularity_GPU.cu:243); the binary tool starts from rand() seeded by
+ * the clock, which no run can repeat.  g may be host or device. */
+int bna_modularity(bna_ctx *ctx, const bna_csr *g, int32_t *labels, double *Q, int *n_modules);
 
 /* No counterpart in the reference: bna_eigcent for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc).

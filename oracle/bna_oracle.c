@@ -47,6 +47,15 @@
  *                       then inv_fishertrans for Fisher sums.
  *   orc_threshold_packed FormFullGraph over packed values (strict v > thr) and
  *                       FormCSRGraph, for the averaged matrix.
+ *   orc_modularity      Newman leading-eigenvector modularity: B/Newman_Modularity_GPU/
+ *                       CUmodularity/modularity_GPU.cu:408-528 (Partition_GPU, the
+ *                       round loop and Q), :332-400 (Sub_Partition_GPU) and
+ *                       Modularity_CPU.cpp:176-320 (Sub_Partition, Lead_Vector: the
+ *                       power iteration on the generalised modularity matrix).
+ *                       The start vector is the node's index, as
+ *                       W/Newman_Modularity_GPU/CUmodularity/modularity_GPU.cu:243
+ *                       sets it (the binary tool draws rand() seeded by time(NULL),
+ *                       which no run can reproduce).
  *   orc_ref_pipeline    The reference's CPU construction path end to end
  *                       (normalise, CorMat_cpu correlation, FormFullGraph(_s),
  *                       FormCSRGraph, degree, power iteration), timed by stage:
@@ -660,6 +669,175 @@ double orc_eigcent(const int64_t *R, const int *C, const float *V, int N, double
     free(y);
     free(d);
     return n;
+}
+
+/* ----------------------------------------------------- Newman modularity */
+#define NM_LAMBDA 0.01      /* a division needs an eigenvalue above this (modularity_GPU.cu:30) */
+#define NM_MIN_GROUP 1      /* and more than this many nodes on each side (:31) */
+#define NM_GPU_ROUND 25000  /* groups above this size take the GPU sub-partition (:494) */
+
+/* Lead_Vector (Modularity_CPU.cpp:209-266): power iteration on B(g) - beta I
+ * restricted to the group; v is left in nv[0, n), the previous iterate in nv0.
+ * Returns the signed norm of the last iterate, as the reference does. */
+static double nm_lead_vector(const int64_t *R, int n, const int *AD, const int64_t *SR, const int *SC,
+                             const double *sumBG, double m, double beta, int max_iter, double eps, double *nv,
+                             double *nv0, double *vv, int *iters)
+{
+    for (int i = 0; i < n; i++) nv[i] = (double)AD[i];
+    double err1 = 1, err2 = 1, vnorm = 0;
+    int it = 0;
+    while (err1 > eps && err2 > eps && it < max_iter) {
+        for (int i = 0; i < n; i++) {
+            nv0[i] = nv[i];
+            vv[AD[i]] = nv[i];
+        }
+        double vk = 0;
+        for (int i = 0; i < n; i++) vk += nv0[i] * (double)(R[AD[i] + 1] - R[AD[i]]);
+        for (int i = 0; i < n; i++) {
+            double t = 0;
+            for (int64_t j = SR[i]; j < SR[i + 1]; j++) t += vv[SC[j]];
+            t -= vk / 2 / m * (double)(R[AD[i] + 1] - R[AD[i]]) + (sumBG[i] - beta) * nv0[i];
+            nv[i] = t;
+        }
+        double s = 0;
+        for (int i = 0; i < n; i++) s += nv[i] * nv[i];
+        vnorm = sqrt(s);
+        for (int i = 0; i < n; i++) nv[i] = nv[i] / vnorm;
+        double e1 = 0, e2 = 0;
+        for (int i = 0; i < n; i++) {
+            const double a = nv[i] - nv0[i], b = nv[i] + nv0[i];
+            e1 += a * a;
+            e2 += b * b;
+        }
+        err1 = sqrt(e1);
+        err2 = sqrt(e2);
+        it++;
+    }
+    *iters = it;
+    int mi = 0;
+    for (int i = 0; i < n; i++)
+        if (fabs(nv[i]) > fabs(nv[mi])) mi = i;
+    return (n > 0 && nv[mi] * nv0[mi] > 0) ? vnorm : -vnorm;
+}
+
+/* Round 0 of the partition alone (Sub_Partition on the whole graph): the
+ * leading vector v (n = N entries) and the eigenvalue after the shift. */
+double orc_modularity_first(const int64_t *R, const int *C, int N, int max_iter, double eps, double *v)
+{
+    if (N < 1) return 0.0;
+    const double m = (double)((R[N] - R[0]) / 2);
+    int *AD = malloc(sizeof(int) * (size_t)(N + 1));
+    double *sumBG = malloc(sizeof(double) * (size_t)(N + 1));
+    double *nv0 = malloc(sizeof(double) * (size_t)(N + 1));
+    double *vv = calloc((size_t)N + 1, sizeof(double));
+    for (int i = 0; i < N; i++) {
+        AD[i] = i;
+        sumBG[i] = (double)(R[i + 1] - R[i]) - (double)(R[i + 1] - R[i]) * (double)(R[N] - R[0]) / 2 / m;
+    }
+    int it = 0;
+    double lambda = nm_lead_vector(R, N, AD, R, C, sumBG, m, 0.0, max_iter, eps, v, nv0, vv, &it);
+    const int shift = N > NM_GPU_ROUND ? (lambda < -NM_LAMBDA) : (lambda < 0);
+    if (shift) lambda += nm_lead_vector(R, N, AD, R, C, sumBG, m, -lambda, max_iter, eps, v, nv0, vv, &it);
+    free(AD);
+    free(sumBG);
+    free(nv0);
+    free(vv);
+    return lambda;
+}
+
+/* Partition_GPU (modularity_GPU.cu:408-528) with Sub_Partition(_GPU): repeated
+ * bisection of the group labelled Round until no group divides; labels[i] is
+ * the module of voxel i, numbered from 1 in the order the modules are found
+ * final.  Q = (sum over entries of [same module] - sum over modules of
+ * (sum of degrees)^2 / 2M) / 2M (the reference's double loop over all pairs,
+ * summed per module).  Returns the number of modules; iters_total gets the
+ * power iterations of all rounds.  R is int64 offsets of a binary CSR. */
+int orc_modularity(const int64_t *R, const int *C, int N, int max_iter, double eps, int *labels, double *Q,
+                   int *rounds_out, long long *iters_total)
+{
+    if (N < 1) return 0;
+    const double m = (double)((R[N] - R[0]) / 2);
+    int *res = calloc((size_t)N, sizeof(int));
+    int *adjust = calloc((size_t)2 * N + 4, sizeof(int));
+    int *AD = malloc(sizeof(int) * (size_t)(N + 1));
+    char *G = malloc((size_t)N + 1);
+    int64_t *SR = malloc(sizeof(int64_t) * (size_t)(N + 1));
+    int *SC = malloc(sizeof(int) * (size_t)(R[N] + 1));
+    double *sumBG = malloc(sizeof(double) * (size_t)(N + 1));
+    double *nv = malloc(sizeof(double) * (size_t)(N + 1));
+    double *nv0 = malloc(sizeof(double) * (size_t)(N + 1));
+    double *vv = calloc((size_t)N + 1, sizeof(double));
+    int round = 0, max_result = 0, index = 1, modules = 1;
+    long long its = 0;
+    while (round <= max_result) {
+        long long inner = (long long)(R[N] - R[0]);
+        int n = 0;
+        for (int i = 0; i < N; i++) {
+            G[i] = res[i] == round;
+            if (G[i])
+                AD[n++] = i;
+            else
+                inner -= R[i + 1] - R[i];
+        }
+        int64_t e = 0;
+        SR[0] = 0;
+        for (int a = 0; a < n; a++) {
+            const int i = AD[a];
+            for (int64_t j = R[i]; j < R[i + 1]; j++)
+                if (G[C[j]]) SC[e++] = C[j];
+            SR[a + 1] = e;
+        }
+        if (n) {
+            for (int a = 0; a < n; a++)
+                sumBG[a] = (double)(SR[a + 1] - SR[a]) - (double)(R[AD[a] + 1] - R[AD[a]]) * (double)inner / 2 / m;
+            int it = 0;
+            double lambda = nm_lead_vector(R, n, AD, SR, SC, sumBG, m, 0.0, max_iter, eps, nv, nv0, vv, &it);
+            its += it;
+            const int shift = n > NM_GPU_ROUND ? (lambda < -NM_LAMBDA) : (lambda < 0);
+            if (shift) {
+                lambda += nm_lead_vector(R, n, AD, SR, SC, sumBG, m, -lambda, max_iter, eps, nv, nv0, vv, &it);
+                its += it;
+            }
+            long long sp = 0, sn = 0;
+            for (int a = 0; a < n; a++) {
+                sp += nv[a] > 0;
+                sn += nv[a] <= 0;
+            }
+            const int sub = lambda > NM_LAMBDA && sp > NM_MIN_GROUP && sn > NM_MIN_GROUP;
+            if (sub) {
+                for (int a = 0; a < n; a++) res[AD[a]] = max_result + 1 + (nv[a] * ((double)(sp - sn) + 0.5) <= 0);
+                max_result += 2;
+            } else {
+                adjust[round] = index++;
+            }
+            modules += sub;
+        }
+        round++;
+    }
+    /* Q: entries inside a module, minus the degree products per module */
+    double within = 0;
+    for (int i = 0; i < N; i++)
+        for (int64_t j = R[i]; j < R[i + 1]; j++) within += 1.0 * (res[i] == res[C[j]]);
+    double *ksum = calloc((size_t)2 * N + 4, sizeof(double));
+    for (int i = 0; i < N; i++) ksum[res[i]] += (double)(R[i + 1] - R[i]);
+    double expect = 0;
+    for (int c = 0; c < 2 * N + 4; c++) expect += ksum[c] * ksum[c] / 2 / m;
+    *Q = (within - expect) / (2 * m);
+    for (int i = 0; i < N; i++) labels[i] = adjust[res[i]];
+    if (rounds_out) *rounds_out = round;
+    if (iters_total) *iters_total = its;
+    free(ksum);
+    free(res);
+    free(adjust);
+    free(AD);
+    free(G);
+    free(SR);
+    free(SC);
+    free(sumBG);
+    free(nv);
+    free(nv0);
+    free(vv);
+    return modules;
 }
 
 /* ------------------------------------------------ reference CPU path timing */

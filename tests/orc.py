@@ -50,6 +50,9 @@ def olib() -> C.CDLL:
         "orc_group_add": (None, [vp, vp, i64, i32]),
         "orc_group_finish": (None, [vp, i64, i32, i32]),
         "orc_threshold_packed": (vp, [vp, i32, dbl]),
+        "orc_modularity": (i32, [vp, vp, i32, i32, dbl, vp, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_longlong)]),
+        "orc_modularity_first": (dbl, [vp, vp, i32, i32, dbl, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -163,6 +166,29 @@ def eigcent(R, Cc, V, N, tol=1e-6, max_iter=10000, out_mode=0):
     it = C.c_int()
     lam = olib().orc_eigcent(_p(R), _p(Cc), _p(V), N, tol, max_iter, out_mode, _p(ec), C.byref(it))
     return ec, lam, it.value
+
+
+def modularity(R, Cc, N, max_iter=10000, eps=1e-6):
+    """Newman leading-eigenvector partition of a binary graph: (labels from 1,
+    Q, modules, rounds, power iterations)."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    lab = np.empty(N, np.int32)
+    q = C.c_double()
+    rounds = C.c_int()
+    its = C.c_longlong()
+    mods = olib().orc_modularity(_p(R), _p(Cc), N, max_iter, C.c_double(eps), _p(lab), C.byref(q),
+                                 C.byref(rounds), C.byref(its))
+    return lab, q.value, mods, rounds.value, its.value
+
+
+def modularity_first(R, Cc, N, max_iter=10000, eps=1e-6):
+    """Round 0 alone: (leading vector, eigenvalue after the shift)."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    v = np.empty(N, np.float64)
+    lam = olib().orc_modularity_first(_p(R), _p(Cc), N, max_iter, C.c_double(eps), _p(v))
+    return v, lam
 
 
 def ref_cpu_rows(z32, i0, i1, thr, nthreads=NTHREADS):

@@ -205,3 +205,20 @@ def test_cp_tool(subjects):
                 assert np.allclose(cp, ocp, rtol=1e-6, atol=0)
             mean = float(Path(str(f)[:-4] + "_cp.txt").read_text())
             assert abs(mean - omean) <= 1e-5 * abs(omean)  # six significant digits
+
+
+def test_modularity_tool(subjects):
+    d, mask, Ls = subjects
+    say("cli: modularity over the unweighted outputs")
+    sub = d / "unweighted"
+    if not sub.exists() or not list(sub.glob("*.csr")):
+        pytest.skip("construction outputs missing")
+    run("bna_modularity", sub, 0)
+    for f in sorted(sub.glob("*.csr")):
+        g = bna.read_csr(f)
+        olab, oQ, omods, rounds, its = orc.modularity(g.R.astype(np.int64), g.C, g.N)
+        lab = bna.read_nm(str(f)[:-4] + "_modu.nm")
+        assert np.array_equal(lab, olab.astype(np.float32))
+        txt = Path(str(f)[:-4] + "_modu.txt").read_text()
+        assert txt.startswith(f"Number of Modules: {omods},")
+        assert abs(float(txt.split("Q=")[1]) - oQ) <= 1e-5 * abs(oQ)
