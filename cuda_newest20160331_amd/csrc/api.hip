@@ -1585,6 +1585,9 @@ int bna_modularity(bna_ctx *ctx, const bna_csr *g, int32_t *labels, double *Q, i
     if ((rc = ws_get_t(ctx, "nm_part", (size_t)4 * NM_BLOCKS_MAX, &b.part))) return rc;
     if ((rc = ws_get(ctx, "nm_state", nm_state_bytes(), &b.state))) return rc;
     if ((rc = ws_get_t(ctx, "nm_same", 1, &b.same))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_groups", (size_t)N, &b.groups))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_gout", (size_t)N, &b.gout))) return rc;
+    if ((rc = ws_get_t(ctx, "nm_side", (size_t)N, &b.side))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if ((rc = pin_grow(ctx, 2, 2 * EC_STATE_MAX_BYTES))) return rc;
     b.state_host = ctx->pin[2];

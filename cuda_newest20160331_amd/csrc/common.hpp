@@ -214,14 +214,26 @@ constexpr int EC_NPARTS_MAX = 16384;
 // Newman leading-eigenvector modularity (k_modularity.hip).  Device buffers:
 // AD, cnt, res N ints; SR N + 1 ints; SC nnz ints; sumBG, y[0..2], yg[0..1]
 // N doubles; part 4 NM_BLOCKS_MAX doubles; state nm_state_bytes(); same one
-// u64.  state_host: pinned, two states; ev: two events.  R_host: the CSR's row
+// u64; groups, gout, side N entries each.  state_host: pinned, two states; ev: two events.  R_host: the CSR's row
 // offsets on the host.
 constexpr int NM_BLOCKS_MAX = 8192;
+struct NmGroup {     // a group's rows [a0, a1) of the batch, and its degree sum
+    int32_t a0, a1;
+    double inner;
+};
+struct NmGroupOut {  // a small group's eigenvalue, split counts, iterations
+    double lambda;
+    long long subP, subN;
+    int32_t iters, pad;
+};
 struct NmBuffers {
     int32_t *AD = nullptr, *cnt = nullptr, *res = nullptr, *SR = nullptr, *SC = nullptr;
     double *sumBG = nullptr, *y[3] = {}, *yg[2] = {}, *part = nullptr;
     void *state = nullptr, *state_host = nullptr;
     unsigned long long *same = nullptr;
+    NmGroup *groups = nullptr;     // N entries
+    NmGroupOut *gout = nullptr;    // N entries
+    int8_t *side = nullptr;        // N entries
     hipEvent_t ev[2] = {};
     std::vector<int32_t> AD_host;
     const int32_t *R_host = nullptr;
