@@ -20,6 +20,8 @@ TIME_MAJOR, VOXEL_MAJOR = 0, 1
 FP32, BF16, BF16X3 = 0, 1, 2
 THR_R, THR_DENSITY = 0, 1
 EC_WEIGHTED_REF, EC_BINARY_REF = 0, 1
+# bna_status
+OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_DEVICE_NOMEM, ERR_IO, ERR_FORMAT, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6, -7
 PRECISIONS = {"fp32": FP32, "bf16": BF16, "bf16x3": BF16X3}
 
 # every function declared in include/bna.h
@@ -38,7 +40,7 @@ EXPORTS = [
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
     "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
-    "bna_modularity",
+    "bna_modularity", "bna_check_csr",
 ]
 
 
@@ -109,6 +111,7 @@ def lib() -> C.CDLL:
                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "bna_modularity": (i32, [vp, C.POINTER(CSR), C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int)]),
+        "bna_check_csr": (i32, [C.POINTER(CSR), i32]),
         "bna_density_to_k": (i64, [i32, C.c_float]),
         "bna_synth_factor_model": (i32, [fp, i32, i32, i32, dbl, C.c_uint64, i32]),
         "bna_write_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
@@ -201,6 +204,12 @@ def read_csr(path: str) -> "Graph":
         return Graph(h)
     finally:
         lib().bna_csr_free(C.byref(h))
+
+
+def check_csr(g: "Graph", ascending: bool = True) -> int:
+    """bna_check_csr on a host Graph: 0 (BNA_OK) or the error status."""
+    h = _host_csr(g)
+    return lib().bna_check_csr(C.byref(h[0]), int(ascending))
 
 
 def read_nm(path: str) -> np.ndarray:

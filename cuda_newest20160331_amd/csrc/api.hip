@@ -1616,6 +1616,10 @@ int bna_cp(bna_ctx *ctx, const bna_csr *g, float *cp, double *mean)
     if (!ctx || !g || !cp) return fail(ctx, BNA_ERR_ARG, "bna_cp: NULL argument");
     if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_cp: empty graph");
     if (g->N > CP_MAX_N) return fail(ctx, BNA_ERR_ARG, "bna_cp: N above 524288 (the row kernel's LDS bitmap)");
+    // the row kernel finds w_ik by rank in row i, so a host CSR must list
+    // every row's columns strictly ascending (device CSRs built here do)
+    if (!g->on_device && bna_check_csr(g, 1) != BNA_OK)
+        return fail(ctx, BNA_ERR_FORMAT, "bna_cp: columns must ascend strictly within every row");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int32_t *R, *C;
     const float *V;
@@ -1886,9 +1890,7 @@ static int group_select(bna_ctx *ctx, bna_group *g, int64_t k, float *r_star, in
     for (int shift = 24; shift >= 0; shift -= 8) {
         HIPCHK(ctx, hipMemsetAsync(hist, 0, 256 * sizeof(unsigned long long), ctx->stream));
         HIPCHK(ctx, launch_group_digits(g->acc, g->M1, prefix, mask, shift, hist, ctx->stream));
-        HIPCHK(ctx, hipMemcpyAsync(h.data(), hist, 256 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = copy_d2h(ctx, h.data(), hist, 256 * sizeof(unsigned long long)))) return rc;
         int d = 255;
         for (; d >= 0; d--) {
             if ((int64_t)h[d] >= need) break;
@@ -2091,16 +2093,14 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
             if ((rc = ws_get_t(ctx, "group_ties", (size_t)N, &tr))) return rc;
             HIPCHK(ctx, launch_group_ties(g->acc, N, thr, tr, ctx->stream));
             std::vector<int64_t> rows(N);
-            HIPCHK(ctx, hipMemcpyAsync(rows.data(), tr, sizeof(int64_t) * N, hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            if ((rc = copy_d2h(ctx, rows.data(), tr, sizeof(int64_t) * N))) return rc;
             int64_t before = 0;
             int i = 0;
             for (; i < N && before + rows[i] < m; i++) before += rows[i];
             if (i >= N - 1) return fail(ctx, BNA_ERR_DENSITY, "group density: tie counts are inconsistent");
             std::vector<float> row((size_t)(N - 1 - i));
-            HIPCHK(ctx, hipMemcpyAsync(row.data(), g->acc + packed_pos(N, i, i + 1), sizeof(float) * row.size(),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            if ((rc = copy_d2h(ctx, row.data(), g->acc + packed_pos(N, i, i + 1), sizeof(float) * row.size())))
+                return rc;
             int64_t want = m - before, c = 0;
             size_t q = 0;
             for (; q < row.size(); q++)
@@ -2122,9 +2122,8 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
     HIPCHK(ctx, launch_group_rowcount(keep, N, ld, cnt, ctx->stream));
     std::vector<int64_t> c(N);
     unsigned int bad = 0;
-    HIPCHK(ctx, hipMemcpyAsync(c.data(), cnt, sizeof(int64_t) * N, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = copy_d2h(ctx, c.data(), cnt, sizeof(int64_t) * N))) return rc;
+    if ((rc = copy_d2h(ctx, &bad, flag, sizeof(bad)))) return rc;
     if (bad) return fail(ctx, BNA_ERR_FORMAT, "FormFullGraph: illegal correlation coefficient (an average above 1)");
     std::vector<int32_t> R((size_t)N + 1);
     int64_t nnz = 0;

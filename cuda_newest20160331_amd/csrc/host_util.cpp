@@ -193,6 +193,20 @@ done:
     return BNA_OK;
 }
 
+extern "C" int bna_check_csr(const bna_csr *g, int ascending)
+{
+    if (!g || g->on_device || g->N < 0 || g->nnz < 0 || !g->R || (g->nnz > 0 && !g->C)) return BNA_ERR_ARG;
+    if (g->R[0] != 0 || (int64_t)g->R[g->N] != g->nnz) return BNA_ERR_FORMAT;
+    for (int32_t i = 0; i < g->N; i++) {
+        if (g->R[i + 1] < g->R[i]) return BNA_ERR_FORMAT;
+        for (int64_t k = g->R[i]; k < g->R[i + 1]; k++) {
+            if (g->C[k] < 0 || g->C[k] >= g->N) return BNA_ERR_FORMAT;
+            if (ascending && k > g->R[i] && g->C[k] <= g->C[k - 1]) return BNA_ERR_FORMAT;
+        }
+    }
+    return BNA_OK;
+}
+
 extern "C" int bna_write_nm(const char *path, const float *v, int N)
 {
     if (!path || !v || N < 0) return BNA_ERR_ARG;

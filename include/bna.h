@@ -276,7 +276,8 @@ int bna_eigcent_noalloc(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter
  * (float)R[i] (0 when t < 2).  A binary graph (V == NULL) uses w = 1.  cci is
  * summed in double.  cp is host, N floats; mean (may be NULL) gets the average
  * |Cp| as the float the tool prints to <stem>_cp.txt (Cp_main.cpp, sasum / N).
- * Columns must ascend within each row; N at most 524288. */
+ * Columns must ascend strictly within each row (a host CSR that breaks this is
+ * BNA_ERR_FORMAT, see bna_check_csr); N at most 524288. */
 int bna_cp(bna_ctx *ctx, const bna_csr *g, float *cp, double *mean);
 
 /* -------------------------------------------------------- host utilities */
@@ -296,6 +297,11 @@ int bna_synth_factor_model(float *x, int N, int L, int K, double phi, uint64_t s
 int bna_write_csr(const char *path, const bna_csr *g);
 /* Reads a .csr (weighted when a V section follows); out gets host arrays. */
 int bna_read_csr(const char *path, bna_csr *out);
+/* Structural check of a host CSR: R from 0 to nnz without decreasing, columns
+ * in [0, N), and with ascending != 0 columns strictly ascending within every
+ * row (what FormCSRGraph writes, CorMat.cpp:889-918).  BNA_OK or
+ * BNA_ERR_FORMAT; BNA_ERR_ARG for a device CSR.  Needs no device. */
+int bna_check_csr(const bna_csr *g, int ascending);
 /* .nm node maps (Degree.cpp:95-99, weighted-net.cu:305-322): int32 N, float[N]. */
 int bna_write_nm(const char *path, const float *v, int N);
 /* A vertex map (N floats, e.g. a _deg.nm or _ec.nm payload) as a NIfTI volume

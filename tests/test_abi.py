@@ -137,3 +137,19 @@ def test_argument_errors_without_device():
     x = np.zeros(4, np.float32)
     assert L.bna_synth_factor_model(x.ctypes.data_as(C.POINTER(C.c_float)), 2, 2, 1, 0.5, 1, 0) == -1
     assert L.bna_synth_factor_model(x.ctypes.data_as(C.POINTER(C.c_float)), 2, 2, 2, 1.0, 1, 0) == -1
+
+
+def test_check_csr_rejects_unsorted_and_duplicate_columns(tmp_path):
+    # a .csr whose row 0 lists its columns out of order, as a file written by
+    # another tool might; bna_cp needs ascending columns and refuses it
+    R = [0, 3, 4, 5, 6]
+    ok = [1, 2, 3, 0, 0, 0]
+    for cols, want in ((ok, 0), ([2, 1, 3, 0, 0, 0], bna.ERR_FORMAT), ([1, 1, 3, 0, 0, 0], bna.ERR_FORMAT)):
+        g, keep = _host_graph(R, cols, None)
+        p = tmp_path / "u.csr"
+        assert bna.lib().bna_write_csr(str(p).encode(), C.byref(g)) == 0
+        back = bna.read_csr(p)
+        assert bna.check_csr(back) == want
+        assert bna.check_csr(back, ascending=False) == 0
+    bad_range, keep = _host_graph([0, 1, 1], [5], None)
+    assert bna.lib().bna_check_csr(C.byref(bad_range), 0) == bna.ERR_FORMAT

@@ -115,3 +115,17 @@ def test_too_large_rejected(ctx):
     g = graph(N, np.zeros(N + 1, np.int64), np.zeros(0, np.int32))
     with pytest.raises(bna.BnaError):
         ctx.cp(g)
+
+
+def test_unsorted_host_csr_is_refused(ctx):
+    g = bna.Graph.__new__(bna.Graph)
+    g.N, g.nnz = 4, 6
+    g.R = np.array([0, 3, 4, 5, 6])
+    g.C = np.array([2, 1, 3, 0, 0, 0], np.int32)
+    g.V = None
+    with pytest.raises(bna.BnaError) as ei:
+        ctx.cp(g)
+    assert ei.value.status == bna.ERR_FORMAT
+    g.C = np.array([1, 2, 3, 0, 0, 0], np.int32)
+    cp, mean = ctx.cp(g)
+    assert cp.shape == (4,)

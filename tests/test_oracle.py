@@ -245,3 +245,27 @@ def test_reference_cpu_path_close_to_fp64():
     # the threaded timing path counts the same edges as the packed values
     e, _ = orc.ref_cpu_rows(z32, 0, 200, 0.3, nthreads=3)
     assert e == int(np.sum(r_ref > np.float32(0.3)))
+
+
+def test_reference_cpu_path_bit_exact_numpy_restatement():
+    # CorMat.cpp:954-965: each product a[l] * b[l] rounded to float, summed in
+    # double in l order, the sum stored as float.  numpy: float32 products, a
+    # sequential float64 accumulation (add.accumulate keeps the order), a cast.
+    # A float accumulator, a double product or another order all differ.
+    from cuda_newest20160331_amd import synth_factor_model
+
+    x = synth_factor_model(120, 97, 6, 0.7, 314)
+    _, z32, _, _, _ = orc.znorm(x, 0)
+    N, L = z32.shape
+    got = orc.ref_corr_rows(z32, 3, 40)
+    want = []
+    for i in range(3, 40):
+        p = (z32[i][None, :] * z32[i + 1:]).astype(np.float32)  # float products
+        s = np.add.accumulate(p.astype(np.float64), axis=1)[:, -1]  # double, in l order
+        want.append(s.astype(np.float32))
+    want = np.concatenate(want)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the pin is sharp: a float accumulator differs somewhere
+    wrong = np.concatenate([np.add.accumulate((z32[i][None, :] * z32[i + 1:]).astype(np.float32), axis=1)[:, -1]
+                            for i in range(3, 40)])
+    assert not np.array_equal(got, wrong)
