@@ -77,7 +77,7 @@ constexpr int CT_THREADS = 256;
 constexpr int NSLOT = 4;                   // LDS ring of k16 stages (power of two)
 constexpr int LDS_MAX = 160 * 1024;        // gfx950 LDS per workgroup
 constexpr int CTL_BYTES = 64;              // tile-claim broadcast words at the top of LDS
-constexpr int EPI_FIXED = 16384 + 2048 + 64;  // bitmaps, offsets, scan partials
+constexpr int EPI_FIXED = 16384 + 2048 + 64 + 4096;  // bitmaps, offsets, scan partials, word prefixes
 
 template <int PREC>
 struct CorrCfg {
@@ -341,24 +341,25 @@ __device__ __forceinline__ void epi_mark_edge(const f32x16_t &a, const EpiCtx &c
     }
 }
 
-// Slots of pair (lr, lc) in the tile's row-view and column-view lists.
+// Slots of pair (lr, lc) in the tile's row-view and column-view lists.  Byte w
+// of rowpre[r] (colpre[c]) is the count of kept pairs in words 0 .. w - 1 of
+// row r's (column c's) bitmap, so a slot is three LDS reads and a popcount.
 struct SlotCtx {
     const uint32_t *rowbits, *colbits;
     const int *rowoff, *coloff;
+    const uint64_t *rowpre, *colpre;
     int64_t base, total;
 };
 __device__ __forceinline__ void pair_slots(const SlotCtx &s, int lr, int lc, int64_t &ps, int64_t &qs, int &qw)
 {
-    int p = s.rowoff[lr], q = s.coloff[lc];
-    const int q0 = q;
     const int rwd = lc >> 5, cwd = lr >> 5;
-    for (int w = 0; w < rwd; w++) p += __popc(s.rowbits[lr * 8 + w]);
-    for (int w = 0; w < cwd; w++) q += __popc(s.colbits[lc * 8 + w]);
-    p += __popc(s.rowbits[lr * 8 + rwd] & ((1u << (lc & 31)) - 1u));
-    q += __popc(s.colbits[lc * 8 + cwd] & ((1u << (lr & 31)) - 1u));
-    ps = s.base + p;
-    qs = s.base + s.total + q;
-    qw = q - q0;  // position within column lc's list
+    const int pw = (int)((s.rowpre[lr] >> (8 * rwd)) & 0xff) +
+                   __popc(s.rowbits[lr * 8 + rwd] & ((1u << (lc & 31)) - 1u));
+    const int qp = (int)((s.colpre[lc] >> (8 * cwd)) & 0xff) +
+                   __popc(s.colbits[lc * 8 + cwd] & ((1u << (lr & 31)) - 1u));
+    ps = s.base + s.rowoff[lr] + pw;
+    qs = s.base + s.total + s.coloff[lc] + qp;
+    qw = qp;  // position within column lc's list
 }
 
 // Writes one kept pair's two entries and, for band pairs, its candidate record
@@ -716,8 +717,11 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
             int *wsum = coloff + TILE;                   // [4]
             int *wsum2 = wsum + 4;                       // [4]
             unsigned long long *sbase = (unsigned long long *)(wsum2 + 4);
+            int *wcnt = (int *)(sbase + 1);              // [4] kept pairs per wave
+            uint64_t *rowpre = (uint64_t *)(epi + 16384 + 2048 + 64);  // [256] word prefixes
+            uint64_t *colpre = rowpre + TILE;                           // [256]
             lds_u64 *lists = (lds_u64 *)(epi + EPI_FIXED);  // [4 waves][LIST_LDS]
-            for (int k = etid; k < TILE * 8 * 2; k += CT_THREADS) rowbits[k] = 0;
+            for (int k = etid; k < TILE * 8 * 2 / 4; k += CT_THREADS) ((u32x4 *)rowbits)[k] = (u32x4){0u, 0u, 0u, 0u};
             lds_sync();
 
             WaveList wl{lists + ewave * C::LIST_LDS,
@@ -728,20 +732,34 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
                 BNA_FOR_BLOCKS_T(epi_mark_inner, ec, rowbits, colbits, wl);
             else
                 BNA_FOR_BLOCKS_T(epi_mark_edge, ec, rowbits, colbits, wl);
+            if (elane == 0) wcnt[ewave] = wl.n;
             lds_sync();
 
+            // The tile's entry range is reserved now, from the per-wave counts
+            // (every kept pair is in one wave's list and sets one row bit), so
+            // the atomic's round trip overlaps the popcounts and scans below.
+            unsigned long long sb = 0;
+            if (etid == 0) {
+                const int tp = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+                if (tp) sb = gatomic_add(sd.entry_count, (unsigned long long)(2 * tp));
+            }
             int rc = 0, cc = 0;
+            uint64_t rpre = 0, cpre = 0;
 #pragma unroll
             for (int w = 0; w < 8; w++) {
+                rpre |= (uint64_t)rc << (8 * w);
+                cpre |= (uint64_t)cc << (8 * w);
                 rc += __popc(rowbits[etid * 8 + w]);
                 cc += __popc(colbits[etid * 8 + w]);
             }
+            rowpre[etid] = rpre;
+            colpre[etid] = cpre;
             int total = 0, total2 = 0;
             const int ro = block_excl_scan(rc, wsum, total, elane, ewave);
             const int co = block_excl_scan(cc, wsum2, total2, elane, ewave);
             rowoff[etid] = ro;
             coloff[etid] = co;
-            if (etid == 0) *sbase = total ? gatomic_add(sd.entry_count, (unsigned long long)(2 * total)) : 0ull;
+            if (etid == 0) *sbase = sb;
             lds_sync();
             const int64_t base = (int64_t)*sbase;
             const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
@@ -760,7 +778,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
                 ((g_i32 *)sd.frag_live)[f] = cc;
                 ((g_i64 *)sd.frag_off)[f] = base + total + co;
             }
-            const SlotCtx sc{rowbits, colbits, rowoff, coloff, base, total};
+            const SlotCtx sc{rowbits, colbits, rowoff, coloff, rowpre, colpre, base, total};
             if (wl.n > wl.cap) __builtin_amdgcn_s_waitcnt(0);  // this wave's spilled records are stored
             for (int k0 = 0; k0 < wl.n; k0 += 64) {
                 const int k = k0 + elane;
