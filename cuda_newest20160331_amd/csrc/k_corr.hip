@@ -50,6 +50,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 
 namespace bna {
 
@@ -824,13 +825,385 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
 #undef BNA_INTERLEAVE
 }
 
+// ------------------------------------------------------------ 8-wave form
+// The same tile, stream and epilogue with eight waves, two per SIMD, each owning
+// a 128 x 64 sub-tile (4 x 2 accumulator blocks, 128 AGPRs).  Waves 0-3 (group
+// 0) and 4-7 (group 1) share SIMDs pairwise and alternate: while one group runs
+// the MFMAs of a stage, the other reads its next stage's fragments from LDS and
+// issues its quarter of a stage load, so the matrix pipe of each SIMD is fed by
+// one wave while the other waits on memory and barriers.  Each group's waves
+// run the phases M(s) (fragments of stage s into registers, load of stream
+// position s + 3, waits) and C(s) (the MFMAs of stage s) with a workgroup
+// barrier after every phase; group 1 starts one phase later, so its C(s)
+// overlaps group 0's M(s + 1).  Group 0 loads the A side of every stage, group 1
+// the B side.  Stage s is read by group 0 in half-period 2s and by group 1 in
+// 2s + 1, so its slot is refilled (with position s + 4) from half-period 2s + 2
+// on; each wave waits for its own part of stage s + 1 at the end of its M(s),
+// before the barrier that precedes the first read of stage s + 1.
+constexpr int CT8 = 512;
+constexpr int WAVE_PAIRS8 = 128 * 64;
+
+template <int PREC>
+struct Corr8Cfg {
+    using B = CorrCfg<PREC>;
+    static constexpr int UNIT = B::UNIT, CH = B::CH, PANEL = B::PANEL, SLOT = B::SLOT, RING = B::RING;
+    static constexpr int DPW = PANEL / 1024 / 4;  // 1 KB stage loads per wave and stage (a quarter of one side)
+    static constexpr int LIST_RAW = (LDS_MAX - CTL_BYTES - RING - EPI_FIXED) / (8 * 8);
+    static constexpr int LIST_LDS = (LIST_RAW > 2048 ? 2048 : LIST_RAW) & ~7;  // kept pairs per wave
+};
+static_assert(Corr8Cfg<2>::LIST_LDS >= 128, "8-wave epilogue lists do not fit beside the ring");
+static_assert(HIST_BINS * 4 <= EPI_FIXED + 8 * 8 * Corr8Cfg<2>::LIST_LDS, "8-wave histogram region");
+
+#define BNA_FOR_BLOCKS8_T(F, ...)                                                                  \
+    do {                                                                                           \
+        BNA_BLOCK_T(F, 0, 0, __VA_ARGS__); BNA_BLOCK_T(F, 0, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 1, 0, __VA_ARGS__); BNA_BLOCK_T(F, 1, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 2, 0, __VA_ARGS__); BNA_BLOCK_T(F, 2, 1, __VA_ARGS__);                      \
+        BNA_BLOCK_T(F, 3, 0, __VA_ARGS__); BNA_BLOCK_T(F, 3, 1, __VA_ARGS__);                      \
+    } while (0)
+#define BNA_FOR_BLOCKS8(F, ...)                                                                    \
+    do {                                                                                           \
+        F(acc[0][0], 0, 0, __VA_ARGS__); F(acc[0][1], 0, 1, __VA_ARGS__);                          \
+        F(acc[1][0], 1, 0, __VA_ARGS__); F(acc[1][1], 1, 1, __VA_ARGS__);                          \
+        F(acc[2][0], 2, 0, __VA_ARGS__); F(acc[2][1], 2, 1, __VA_ARGS__);                          \
+        F(acc[3][0], 3, 0, __VA_ARGS__); F(acc[3][1], 3, 1, __VA_ARGS__);                          \
+    } while (0)
+
+// Exclusive scan of one int per thread over 512 threads in two independent
+// halves of four waves (threads 0-255 and 256-511).  lo_total: the first
+// half's total, seen by every thread.
+__device__ __forceinline__ int halves_excl_scan(int v, int *wsum, int &lo_total, int lane, int w) {
+    int inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
+    lds_sync();
+    int pre = 0, t0 = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int s = wsum[k];
+        if (k < 4) t0 += s;
+        if ((k >> 2) == (w >> 2) && k < w) pre += s;
+    }
+    lo_total = t0;
+    return pre + inc - v;
+}
+
+template <int PREC, int HIST>
+__global__ __launch_bounds__(CT8, 1) void corr_tile8_kernel(const SubjDesc *__restrict__ subjs,
+                                                            const TileItem *__restrict__ items, int64_t n_items,
+                                                            unsigned int *__restrict__ claim,
+                                                            uint64_t *__restrict__ spill)
+{
+    static_assert(PREC == 1 || PREC == 2, "the 8-wave form runs the bf16 precisions");
+    static_assert(HIST == 0 || HIST == 1, "the 8-wave form runs the emission and histogram passes");
+    using C = Corr8Cfg<PREC>;
+    extern __shared__ __align__(1024) unsigned char smem[];
+    unsigned char *const epi = smem + C::RING;
+    int64_t *const ctl = (int64_t *)(smem + LDS_MAX - CTL_BYTES);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int swave = __builtin_amdgcn_readfirstlane(wave);
+    const int grp = swave >> 2, quarter = swave & 3;  // group, and quarter of the group's side it loads
+    const int wr0 = (wave & 1) * 128;
+    const int wc0 = (wave >> 1) * 64;
+    const int lrow = lane & 31;
+    const int half = lane >> 5;
+    const int xcd = blockIdx.x & 7;
+    const int64_t per_xcd = (n_items + 7) / 8;
+
+    if (tid == 0) {
+        const int64_t t0 = claim_tile(claim, n_items, per_xcd, xcd);
+        ctl[0] = t0;
+        ctl[1] = t0 >= 0 ? claim_tile(claim, n_items, per_xcd, xcd) : -1;
+    }
+    if constexpr (HIST == 1) {
+        for (int k = tid; k < HIST_BINS; k += CT8) ((uint32_t *)epi)[k] = 0;
+    }
+    __syncthreads();
+    int64_t cur = uni64(ctl[0]);
+    int64_t nxt = uni64(ctl[1]);
+    if (cur < 0) return;
+    TileItem ic = items[cur];
+    TileItem in = nxt >= 0 ? items[nxt] : ic;
+
+    // Stage loads: wave (grp, quarter) fills rows [quarter * 64, quarter * 64 + 64)
+    // of side grp (0: A, 1: B) with DPW 1 KB instructions, addressed as in the
+    // 4-wave form.
+    const uint32_t dsrc = (uint32_t)((quarter * (TILE / 4) + lane / C::CH) * C::UNIT +
+                                     (((lane % C::CH) ^ swz<C::CH>(lane / C::CH)) << 4));
+    int g = 0;  // stream position of the current tile's stage 0; stage s lives in slot (g + s) % NSLOT
+
+// Source and LDS destination of this wave's part of stream position S.
+#define BNA8_DMA_ADDR(S, P, D)                                                                     \
+    do {                                                                                           \
+        const int s_ = (S);                                                                        \
+        const bool own_ = s_ < nk;                                                                 \
+        const int64_t o_ = own_ ? s_ * ic.kstride : (s_ - nk) * xK;                                \
+        P = (own_ ? (grp ? cB : cA) : (grp ? xB : xA)) + o_ + dsrc;                                \
+        D = smem + ((g + s_) & (NSLOT - 1)) * C::SLOT + grp * C::PANEL + quarter * C::DPW * 1024;  \
+    } while (0)
+#define BNA8_DMA(S)                                                                                \
+    do {                                                                                           \
+        const unsigned char *p_;                                                                   \
+        unsigned char *d_;                                                                         \
+        BNA8_DMA_ADDR(S, p_, d_);                                                                  \
+        dma_side<C::DPW>(p_, d_);                                                                  \
+    } while (0)
+
+    u32x4 fa[4][2], fb[2][2];
+    const int c0 = half, c1 = 2 + half;  // logical chunks: hi of lane half h, lo of lane half h
+    const int sw = swz<C::CH>(lrow);
+    const uint32_t fA0 = (uint32_t)((wr0 + lrow) * C::UNIT + ((c0 ^ sw) << 4));
+    const uint32_t fA1 = (uint32_t)((wr0 + lrow) * C::UNIT + ((c1 ^ sw) << 4));
+    const uint32_t fB0 = (uint32_t)(C::PANEL + (wc0 + lrow) * C::UNIT + ((c0 ^ sw) << 4));
+    const uint32_t fB1 = (uint32_t)(C::PANEL + (wc0 + lrow) * C::UNIT + ((c1 ^ sw) << 4));
+#define BNA8_LDFRAG(STAGE_IDX)                                                                     \
+    do {                                                                                           \
+        const unsigned char *slot_ = smem + ((g + (STAGE_IDX)) & (NSLOT - 1)) * C::SLOT;           \
+        _Pragma("unroll") for (int blk = 0; blk < 4; blk++) {                                      \
+            fa[blk][0] = *(const u32x4 *)(slot_ + fA0 + blk * 32 * C::UNIT);                       \
+            if constexpr (PREC == 2) fa[blk][1] = *(const u32x4 *)(slot_ + fA1 + blk * 32 * C::UNIT); \
+        }                                                                                          \
+        _Pragma("unroll") for (int blk = 0; blk < 2; blk++) {                                      \
+            fb[blk][0] = *(const u32x4 *)(slot_ + fB0 + blk * 32 * C::UNIT);                       \
+            if constexpr (PREC == 2) fb[blk][1] = *(const u32x4 *)(slot_ + fB1 + blk * 32 * C::UNIT); \
+        }                                                                                          \
+    } while (0)
+#define BNA8_MFMA()                                                                                \
+    do {                                                                                           \
+        _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                           \
+            _Pragma("unroll") for (int cb = 0; cb < 2; cb++)                                       \
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                             \
+                    __builtin_bit_cast(bf16x8_t, fa[rb][0]), __builtin_bit_cast(bf16x8_t, fb[cb][0]), \
+                    acc[rb][cb], 0, 0, 0);                                                         \
+        if constexpr (PREC == 2) {                                                                 \
+            _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                       \
+                _Pragma("unroll") for (int cb = 0; cb < 2; cb++)                                   \
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                         \
+                        __builtin_bit_cast(bf16x8_t, fa[rb][0]), __builtin_bit_cast(bf16x8_t, fb[cb][1]), \
+                        acc[rb][cb], 0, 0, 0);                                                     \
+            _Pragma("unroll") for (int rb = 0; rb < 4; rb++)                                       \
+                _Pragma("unroll") for (int cb = 0; cb < 2; cb++)                                   \
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(                         \
+                        __builtin_bit_cast(bf16x8_t, fa[rb][1]), __builtin_bit_cast(bf16x8_t, fb[cb][0]), \
+                        acc[rb][cb], 0, 0, 0);                                                     \
+        }                                                                                          \
+    } while (0)
+
+    // Prologue of the first tile: stream positions 0, 1, 2.
+    {
+        const int nk = ic.nk;
+        const unsigned char *cA = (const unsigned char *)ic.panel + (int64_t)ic.I * C::PANEL;
+        const unsigned char *cB = (const unsigned char *)ic.panel + (int64_t)ic.J * C::PANEL;
+        const unsigned char *xA = cA, *xB = cB;
+        const int64_t xK = 0;
+        BNA8_DMA(0);
+        BNA8_DMA(1);
+        BNA8_DMA(2);
+    }
+
+    for (;;) {
+        const int nk = ic.nk;
+        const unsigned char *cA = (const unsigned char *)ic.panel + (int64_t)ic.I * C::PANEL;
+        const unsigned char *cB = (const unsigned char *)ic.panel + (int64_t)ic.J * C::PANEL;
+        const unsigned char *xA = nxt >= 0 ? (const unsigned char *)in.panel + (int64_t)in.I * C::PANEL
+                                           : cA + (int64_t)(nk - 1) * ic.kstride;
+        const unsigned char *xB = nxt >= 0 ? (const unsigned char *)in.panel + (int64_t)in.J * C::PANEL
+                                           : cB + (int64_t)(nk - 1) * ic.kstride;
+        const int64_t xK = nxt >= 0 ? in.kstride : 0;
+        const SubjDesc sd = subjs[ic.subj];
+
+        f32x16_t acc[4][2];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[rb][cb][e] = 0.0f;
+
+        // Everything in flight is waited for first (the previous tile's epilogue
+        // stores share the counter with the stage loads), then a barrier makes
+        // stages 0-2 visible to every wave.
+        BNA_WAIT_VMCNT(0);
+        unsigned int pre_t = 0;
+        if (tid == 0 && nxt >= 0) pre_t = atomicAdd(&claim[xcd], 1u);
+        __builtin_amdgcn_s_barrier();
+        if (grp == 1) __builtin_amdgcn_s_barrier();  // group 1 runs one phase behind
+        __builtin_amdgcn_sched_barrier(0);
+        for (int kt = 0; kt < nk; kt++) {
+            // M(kt): fragments of stage kt; then this wave's part of stage kt + 1
+            // (loaded during its C(kt - 2)) has landed, position kt + 2 may stay
+            // in flight, and its fragment reads are complete.
+            BNA8_LDFRAG(kt);
+            const unsigned char *dp;
+            unsigned char *dd;
+            BNA8_DMA_ADDR(kt + 3, dp, dd);
+            BNA_WAIT_VMCNT(C::DPW);
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+            // C(kt): the MFMAs of stage kt, with this wave's part of stream
+            // position kt + 3 issued between them
+            dma_side<C::DPW>(dp, dd);
+            BNA8_MFMA();
+#pragma unroll
+            for (int i_ = 0; i_ < C::DPW; i_++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, (PREC == 2 ? 24 : 8) - C::DPW, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (grp == 0) __builtin_amdgcn_s_barrier();  // group 1's last C phase
+        __builtin_amdgcn_sched_barrier(0);
+
+        // ------------------------------------------------------------ epilogue
+        int etid = tid;
+        asm volatile("" : "+v"(etid));
+        const int elane = etid & 63, ewave = etid >> 6;
+        EpiCtx ec;
+        ec.N = sd.N;
+        ec.gi0 = ic.I * TILE;
+        ec.gj0 = ic.J * TILE;
+        ec.wr0 = (ewave & 1) * 128;
+        ec.wc0 = (ewave >> 1) * 64;
+        ec.half = elane >> 5;
+        ec.lrow = elane & 31;
+        ec.lane = elane;
+        ec.diag = (ic.I == ic.J);
+        ec.keep = sd.keep_thr;
+
+        if constexpr (HIST == 1) {
+            BNA_FOR_BLOCKS8(epi_hist, ec, (uint32_t *)epi);
+        } else {
+            uint32_t *rowbits = (uint32_t *)epi;         // [256][8] tile-row bitmaps
+            uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps
+            int *rowoff = (int *)(colbits + TILE * 8);   // [256] first slot of each tile row
+            int *coloff = rowoff + TILE;                 // [256] first slot of each tile column
+            int *wsum = coloff + TILE;                   // [8]
+            int *wcnt = wsum + 8;                        // [8] kept pairs per wave
+            uint64_t *rowpre = (uint64_t *)(epi + 16384 + 2048 + 64);  // [256] word prefixes
+            uint64_t *colpre = rowpre + TILE;                           // [256]
+            lds_u64 *lists = (lds_u64 *)(epi + EPI_FIXED);  // [8 waves][LIST_LDS]
+            for (int k = etid; k < TILE * 8 * 2 / 4; k += CT8) ((u32x4 *)rowbits)[k] = (u32x4){0u, 0u, 0u, 0u};
+            lds_sync();
+
+            WaveList wl{lists + ewave * C::LIST_LDS,
+                        (glb_u64 *)(spill + ((int64_t)blockIdx.x * 8 + ewave) * (WAVE_PAIRS8 - C::LIST_LDS)),
+                        C::LIST_LDS, 0};
+            const bool full = !(ec.diag || ec.gi0 + TILE > sd.N || ec.gj0 + TILE > sd.N);
+            if (full)
+                BNA_FOR_BLOCKS8_T(epi_mark_inner, ec, rowbits, colbits, wl);
+            else
+                BNA_FOR_BLOCKS8_T(epi_mark_edge, ec, rowbits, colbits, wl);
+            if (elane == 0) wcnt[ewave] = wl.n;
+            lds_sync();
+
+            unsigned long long sb = 0;
+            if (etid == 0) {
+                int tp = 0;
+#pragma unroll
+                for (int w = 0; w < 8; w++) tp += wcnt[w];
+                if (tp) sb = gatomic_add(sd.entry_count, (unsigned long long)(2 * tp));
+            }
+            // threads 0-255: tile row etid; threads 256-511: tile column etid - 256
+            const bool is_row = etid < TILE;
+            const int rr = etid & (TILE - 1);
+            const uint32_t *bits = is_row ? rowbits : colbits;
+            int cnt = 0;
+            uint64_t pre = 0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) {
+                pre |= (uint64_t)cnt << (8 * w);
+                cnt += __popc(bits[rr * 8 + w]);
+            }
+            (is_row ? rowpre : colpre)[rr] = pre;
+            int total = 0;
+            const int o = halves_excl_scan(cnt, wsum, total, elane, ewave);
+            (is_row ? rowoff : coloff)[rr] = o;
+            if (etid == 0) ctl[3] = (int64_t)sb;
+            lds_sync();
+            const int64_t base = ctl[3];
+            const bool fits = base + 2 * (int64_t)total <= sd.entry_cap;
+
+            const int N = sd.N;
+            if (is_row) {
+                if (ec.gi0 + rr < N) {
+                    const int64_t f = (int64_t)(ic.J + 1) * N + ec.gi0 + rr;
+                    ((g_i32 *)sd.frag_orig)[f] = cnt;
+                    ((g_i32 *)sd.frag_live)[f] = cnt;
+                    ((g_i64 *)sd.frag_off)[f] = base + o;
+                }
+            } else if (ec.gj0 + rr < N) {
+                const int64_t f = (int64_t)ic.I * N + ec.gj0 + rr;
+                ((g_i32 *)sd.frag_orig)[f] = cnt;
+                ((g_i32 *)sd.frag_live)[f] = cnt;
+                ((g_i64 *)sd.frag_off)[f] = base + total + o;
+            }
+            const SlotCtx sc{rowbits, colbits, rowoff, coloff, rowpre, colpre, base, total};
+            if (wl.n > wl.cap) __builtin_amdgcn_s_waitcnt(0);  // this wave's spilled records are stored
+            for (int k0 = 0; k0 < wl.n; k0 += 64) {
+                const int k = k0 + elane;
+                const bool act = k < wl.n;
+                uint64_t raw = 0;
+                if (act) raw = k < wl.cap ? (uint64_t)wl.lds[k] : __builtin_nontemporal_load(wl.spill + (k - wl.cap));
+                const uint32_t lo32 = (uint32_t)raw;
+                emit_pair(sd, ec, sc, fits, act, (int)(lo32 & 0xffff), (int)(lo32 >> 16),
+                          __builtin_bit_cast(float, (uint32_t)(raw >> 32)));
+            }
+        }
+
+        // ------------------------------------------------------------ boundary
+        if (etid == 0) {
+            int64_t nn_claim = -1;
+            if (nxt >= 0) {
+                const int64_t lo = (int64_t)xcd * per_xcd;
+                const int64_t cnt = n_items - lo < per_xcd ? n_items - lo : per_xcd;
+                nn_claim = (int64_t)pre_t < cnt ? lo + (int64_t)pre_t : claim_tile(claim, n_items, per_xcd, xcd);
+            }
+            ctl[2] = nn_claim;
+        }
+        __syncthreads();
+        const int64_t nn = uni64(ctl[2]);
+        if constexpr (HIST == 1) {
+            if (nxt < 0 || in.subj != ic.subj) {
+                for (int k = etid; k < HIST_BINS; k += CT8) {
+                    const uint32_t h = ((uint32_t *)epi)[k];
+                    if (h) {
+                        gatomic_add(sd.hist + k, (unsigned long long)h);
+                        ((uint32_t *)epi)[k] = 0;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (nxt < 0) break;
+        cur = nxt;
+        ic = items[cur];
+        nxt = nn;
+        in = nxt >= 0 ? items[nxt] : ic;
+        g += nk;
+    }
+#undef BNA8_DMA
+#undef BNA8_DMA_ADDR
+#undef BNA8_LDFRAG
+#undef BNA8_MFMA
+}
+
 // Clears the per-range claim counters of one launch.
 __global__ void claim_reset_kernel(unsigned int *claim)
 {
     if (threadIdx.x < 8) claim[threadIdx.x] = 0u;
 }
 
-template <int PREC, int HIST>
+template <int PREC, int HIST, bool EIGHT>
 static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, int64_t n_items, unsigned int *claim,
                              uint64_t *spill, int grid_max, hipStream_t st)
 {
@@ -838,9 +1211,13 @@ static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, in
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
+    const void *fn;
+    if constexpr (EIGHT)
+        fn = (const void *)corr_tile8_kernel<PREC, HIST>;
+    else
+        fn = (const void *)corr_tile_kernel<PREC, HIST>;
     if (attr_dev.load() != dev) {
-        e = hipFuncSetAttribute((const void *)corr_tile_kernel<PREC, HIST>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
         if (e != hipSuccess) return e;
         attr_dev.store(dev);
     }
@@ -848,25 +1225,45 @@ static hipError_t launch_one(const SubjDesc *subj_dev, const TileItem *items, in
     if (grid_max < 8 || grid_max % 8) return hipErrorInvalidValue;
     const int64_t grid = std::min<int64_t>(grid_max, (n_items + 7) / 8 * 8);
     hipLaunchKernelGGL(claim_reset_kernel, dim3(1), dim3(64), 0, st, claim);
-    hipLaunchKernelGGL((corr_tile_kernel<PREC, HIST>), dim3((unsigned)grid), dim3(CT_THREADS), LDS_MAX, st,
-                       subj_dev, items, n_items, claim, spill);
+    if constexpr (EIGHT)
+        hipLaunchKernelGGL((corr_tile8_kernel<PREC, HIST>), dim3((unsigned)grid), dim3(CT8), LDS_MAX, st, subj_dev,
+                           items, n_items, claim, spill);
+    else
+        hipLaunchKernelGGL((corr_tile_kernel<PREC, HIST>), dim3((unsigned)grid), dim3(CT_THREADS), LDS_MAX, st,
+                           subj_dev, items, n_items, claim, spill);
     return hipGetLastError();
+}
+
+// The bf16 precisions' emission and histogram passes run the 8-wave form
+// unless BNA_TILE_WAVES=4 is set in the environment (read once per process).
+static bool eight_waves()
+{
+    static const bool on = [] {
+        const char *v = getenv("BNA_TILE_WAVES");
+        return !(v && atoi(v) == 4);
+    }();
+    return on;
 }
 
 hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const TileItem *items,
                        int64_t n_items, unsigned int *claim, uint64_t *spill, int grid_max, hipStream_t st)
 {
     if (hist_mode < 0 || hist_mode > 2) return hipErrorInvalidValue;
+    const bool e8 = eight_waves() && prec != 0 && hist_mode != 2;
     switch (prec * 3 + hist_mode) {
-    case 0: return launch_one<0, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 1: return launch_one<0, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 2: return launch_one<0, 2>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 3: return launch_one<1, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 4: return launch_one<1, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 5: return launch_one<1, 2>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 6: return launch_one<2, 0>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 7: return launch_one<2, 1>(subj_dev, items, n_items, claim, spill, grid_max, st);
-    case 8: return launch_one<2, 2>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 0: return launch_one<0, 0, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 1: return launch_one<0, 1, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 2: return launch_one<0, 2, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 3: return e8 ? launch_one<1, 0, true>(subj_dev, items, n_items, claim, spill, grid_max, st)
+                      : launch_one<1, 0, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 4: return e8 ? launch_one<1, 1, true>(subj_dev, items, n_items, claim, spill, grid_max, st)
+                      : launch_one<1, 1, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 5: return launch_one<1, 2, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 6: return e8 ? launch_one<2, 0, true>(subj_dev, items, n_items, claim, spill, grid_max, st)
+                      : launch_one<2, 0, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 7: return e8 ? launch_one<2, 1, true>(subj_dev, items, n_items, claim, spill, grid_max, st)
+                      : launch_one<2, 1, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
+    case 8: return launch_one<2, 2, false>(subj_dev, items, n_items, claim, spill, grid_max, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -875,7 +1272,8 @@ hipError_t launch_corr(int prec, int hist_mode, const SubjDesc *subj_dev, const 
 int64_t corr_spill_records(int prec, int grid_max)
 {
     const int cap = prec == 0 ? CorrCfg<0>::LIST_LDS : prec == 1 ? CorrCfg<1>::LIST_LDS : CorrCfg<2>::LIST_LDS;
-    return (int64_t)grid_max * 4 * (WAVE_PAIRS - cap);
+    const int cap8 = prec == 1 ? Corr8Cfg<1>::LIST_LDS : Corr8Cfg<2>::LIST_LDS;
+    return (int64_t)grid_max * std::max<int64_t>(4 * (WAVE_PAIRS - cap), 8 * (WAVE_PAIRS8 - cap8));
 }
 
 }  // namespace bna
