@@ -68,7 +68,8 @@ struct bna_ctx {
     size_t held = 0;  // workspace + series + csr bytes allocated through the context
     std::string err;
     bna_timings tm{};
-    hipEvent_t ev[10] = {};  // [0..7] builds, [8..9] graph maps
+    hipEvent_t ev[12] = {};  // [0..7] builds, [8..9] graph maps, [10..11] bna_series_normalize
+    bool znorm_pending = false;  // ev[10..11] bracket a normalisation not yet read into tm
     std::map<std::string, DevBuf> ws;
     int grid = 256;  // persistent correlation workgroups: CUs rounded up to a multiple of 8
     int ncu = 256;   // compute units of the device
@@ -488,6 +489,13 @@ int bna_synchronize(bna_ctx *ctx)
 int bna_get_timings(bna_ctx *ctx, bna_timings *t)
 {
     if (!ctx || !t) return fail(ctx, BNA_ERR_ARG, "bna_get_timings: NULL argument");
+    if (ctx->znorm_pending) {
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev[11]));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[10], ctx->ev[11]));
+        ctx->tm.znorm_ms = ms;
+        ctx->znorm_pending = false;
+    }
     *t = ctx->tm;
     return BNA_OK;
 }
@@ -648,14 +656,13 @@ int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision)
 {
     if (!ctx || !s) return fail(ctx, BNA_ERR_ARG, "bna_series_normalize: NULL argument");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    // asynchronous: work queued behind it on the context's stream (a build) is
+    // ordered after it; its time is read at the next bna_get_timings
+    HIPCHK(ctx, hipEventRecord(ctx->ev[10], ctx->stream));
     int rc = series_normalize(ctx, s, precision);
     if (rc) return rc;
-    HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
-    float ms = 0;
-    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
-    ctx->tm.znorm_ms = ms;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[11], ctx->stream));
+    ctx->znorm_pending = true;
     return BNA_OK;
 }
 
@@ -1358,6 +1365,7 @@ int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *o, bna_csr
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
         ctx->tm.znorm_ms = ms;
+        ctx->znorm_pending = false;
     }
     return BNA_OK;
 }
@@ -1392,6 +1400,7 @@ int bna_build_csr_batch(bna_ctx *ctx, bna_series *const *series, const bna_opts 
     if (normed && rc == BNA_OK) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->tm.znorm_ms = ms;
+        ctx->znorm_pending = false;
     }
     return rc ? rc : first;
 }

@@ -149,7 +149,9 @@ typedef struct bna_series bna_series;
 int bna_series_create(bna_ctx *ctx, const float *x, int N, int L, int layout, int x_on_device,
                       bna_series **out);
 int bna_series_destroy(bna_ctx *ctx, bna_series *s);
-/* Normalisation only (z-score, kernel a); run again to re-time it. */
+/* Normalisation only (z-score, kernel a).  Asynchronous: it is queued on the
+ * context's stream and a build queued after it waits for it; its time is
+ * read into bna_timings.znorm_ms at the next bna_get_timings. */
 int bna_series_normalize(bna_ctx *ctx, bna_series *s, int precision);
 /* Correlate + threshold + CSR on a normalised series; out->on_device = 1. */
 int bna_build_csr_series(bna_ctx *ctx, bna_series *s, const bna_opts *opts, bna_csr *out);
