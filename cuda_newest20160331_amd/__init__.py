@@ -38,6 +38,7 @@ EXPORTS = [
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
+    "bna_group_set_exact", "bna_group_band_pairs",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
     "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
     "bna_modularity", "bna_check_csr",
@@ -134,6 +135,8 @@ def lib() -> C.CDLL:
         "bna_group_destroy": (i32, [vp, vp]),
         "bna_group_add": (i32, [vp, vp, vp, i32]),
         "bna_group_count": (i32, [vp]),
+        "bna_group_set_exact": (i32, [vp, vp, i32]),
+        "bna_group_band_pairs": (i64, [vp]),
         "bna_group_finish": (i32, [vp, vp]),
         "bna_group_download": (i32, [vp, vp, fp]),
         "bna_group_build_csr": (i32, [vp, vp, C.POINTER(Opts), C.POINTER(CSR)]),
@@ -338,8 +341,8 @@ class Context:
     def series(self, x: np.ndarray, layout: int = TIME_MAJOR) -> "Series":
         return Series(self, x, layout)
 
-    def group(self, N: int, fisher: bool = False) -> "Group":
-        return Group(self, N, fisher)
+    def group(self, N: int, fisher: bool = False, exact: bool = True) -> "Group":
+        return Group(self, N, fisher, exact)
 
     def degree(self, g: Graph) -> np.ndarray:
         h = _host_csr(g)
@@ -551,13 +554,20 @@ class Group:
     """Group averaging of correlation matrices (bna_group): add subjects, finish,
     then threshold the averages into a graph."""
 
-    def __init__(self, ctx: Context, N: int, fisher: bool = False):
+    def __init__(self, ctx: Context, N: int, fisher: bool = False, exact: bool = True):
         self.ctx = ctx
         self.N = N
         self.M1 = N * (N - 1) // 2
         self._h = C.c_void_p()
         ctx._check(ctx._L.bna_group_create(ctx._h, N, 1 if fisher else 0, C.byref(self._h)),
                    "bna_group_create")
+        if not exact:
+            ctx._check(ctx._L.bna_group_set_exact(ctx._h, self._h, 0), "bna_group_set_exact")
+
+    @property
+    def band_pairs(self) -> int:
+        """Pairs the last graph build recomputed at the threshold."""
+        return int(self.ctx._L.bna_group_band_pairs(self._h))
 
     def close(self):
         if self._h:

@@ -1871,12 +1871,31 @@ int bna_autotune(bna_ctx *ctx, int precision, int L, int N, const char *cache_pa
 // The reference's avg = y/b path (CorMat.cpp:418-670): per-pair sums of r or
 // Fisher z over the subjects of a directory, averaged, then thresholded like a
 // single subject's matrix.  The sums live in device memory for the whole group.
+struct GroupCopy {
+    float *xs = nullptr;  // voxel-major series, row stride ld
+    double *mu = nullptr, *sd = nullptr;
+    int64_t ld = 0;
+    int L = 0;
+    size_t xs_bytes = 0;
+};
 struct bna_group {
     int N = 0, fisher = 0, count = 0;
     bool finished = false;
     int64_t M1 = 0;
     float *acc = nullptr;  // packed upper triangle, row-major, i < j
+    // exact averages: each subject's normalised series is kept so that the
+    // pairs at the threshold (and every kept edge's value) can be recomputed
+    // from FP64 r in the reference's float arithmetic
+    bool exact = true;
+    std::vector<GroupCopy> subs;
+    double band_max = 0.0;       // largest MFMA error bound of the subjects' r
+    uint32_t *flags = nullptr;   // Fisher sums: pairs with |r| > GROUP_FLAG_R in some subject
+    int64_t last_fixed = 0;      // pairs the last bna_group_build_csr recomputed at the threshold
 };
+// Fisher sums: a pair whose |r| exceeds this in some subject is always
+// recomputed (the slope of z = atanh r is 1 / (1 - r^2): 5.3 here).
+constexpr float GROUP_FLAG_R = 0.9f;
+static size_t group_flag_bytes(int64_t M1) { return (size_t)((M1 + 63) / 64) * 8; }  // whole 64-bit words
 
 static inline int64_t packed_pos(int64_t N, int64_t i, int64_t j) { return i * N - (i + 1) * i / 2 + j - i - 1; }
 
@@ -1917,6 +1936,65 @@ static int group_select(bna_ctx *ctx, bna_group *g, int64_t k, float *r_star, in
     return BNA_OK;
 }
 
+// Bound on |device average - exact average| for a pair the exact path does not
+// recompute: each subject's term differs by the MFMA error band (through the
+// slope of atanh for Fisher sums, whose pairs with |r| > GROUP_FLAG_R in some
+// subject are always recomputed) plus the float rounding of r; both sides round
+// their S float additions (partial sums at most S, or S z_max), the division by
+// S and, for Fisher sums, the inverse transform.  u = 2^-24 is half an ulp of 1.
+static double group_bound(const bna_group *g)
+{
+    const double u = std::ldexp(1.0, -24);
+    const double S = (double)g->count, b = g->band_max;
+    if (!g->fisher) return 1.5 * (b + u / 2 + 2.0 * S * u + 2.0 * u);
+    const double R = (double)GROUP_FLAG_R + b + u;  // |r64| of an unflagged pair
+    const double slope = 1.0 / (1.0 - R * R);
+    const double zmax = std::atanh(R) + 1e-6;
+    const double term = slope * (b + u / 2) + 16.0 * u * (1.0 + zmax);  // logf and the quotient, both sides
+    return 1.5 * (term + 2.0 * S * zmax * u + 2.0 * zmax * u + 16.0 * u);
+}
+
+// The pairs listed in pairs[0, *count) get their exact averages (the count is
+// read on the device, so the list may come straight from a kernel).
+static int group_exact_pairs(bna_ctx *ctx, bna_group *g, const int2 *pairs, const unsigned long long *count,
+                             int64_t cap)
+{
+    std::vector<GroupSubj> hs(g->subs.size());
+    for (size_t q = 0; q < g->subs.size(); q++) {
+        const GroupCopy &c = g->subs[q];
+        hs[q] = GroupSubj{c.xs, c.ld, c.mu, c.sd, c.L, 0};
+    }
+    GroupSubj *ds = nullptr;
+    int rc = ws_get_t(ctx, "group_subjs", hs.size(), &ds);
+    if (rc) return rc;
+    if ((rc = h2d_pinned(ctx, 0, ds, hs.data(), hs.size() * sizeof(GroupSubj)))) return rc;
+    HIPCHK(ctx, launch_group_exact(pairs, count, cap, ds, (int)hs.size(), g->fisher, g->acc, g->N, ctx->stream));
+    return BNA_OK;
+}
+
+// Recomputes the averages of every pair within w of t (and every flagged pair).
+static int group_fix_band(bna_ctx *ctx, bna_group *g, float t, float w, int64_t *n_fixed)
+{
+    unsigned long long *cnt = nullptr;
+    int rc = ws_get_t(ctx, "group_pair_count", 1, &cnt);
+    if (rc) return rc;
+    int64_t cap = std::max<int64_t>(1 << 20, (int64_t)(ctx->ws["group_pairs"].bytes / sizeof(int2)));
+    for (int attempt = 0; attempt < 2; attempt++) {
+        int2 *pairs = nullptr;
+        if ((rc = ws_get_t(ctx, "group_pairs", (size_t)cap, &pairs))) return rc;
+        HIPCHK(ctx, launch_zero_u64(cnt, 1, ctx->stream));
+        HIPCHK(ctx, launch_group_cands(g->acc, g->N, t, w, g->flags, pairs, cnt, cap, ctx->stream));
+        unsigned long long n = 0;
+        if ((rc = copy_d2h(ctx, &n, cnt, sizeof(n)))) return rc;
+        if ((int64_t)n <= cap) {
+            *n_fixed = (int64_t)n;
+            return n ? group_exact_pairs(ctx, g, pairs, cnt, cap) : BNA_OK;
+        }
+        cap = (int64_t)n + (int64_t)n / 8 + 1024;
+    }
+    return fail(ctx, BNA_ERR_DEVICE_NOMEM, "group: band pair list keeps overflowing");
+}
+
 extern "C" {
 
 int bna_group_create(bna_ctx *ctx, int N, int fisher, bna_group **out)
@@ -1952,7 +2030,22 @@ int bna_group_destroy(bna_ctx *ctx, bna_group *g)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     dev_free(ctx, g->acc, (size_t)g->M1 * 4);
+    for (GroupCopy &c : g->subs) {
+        dev_free(ctx, c.xs, c.xs_bytes);
+        dev_free(ctx, c.mu, (size_t)g->N * 8);
+        dev_free(ctx, c.sd, (size_t)g->N * 8);
+    }
+    if (g->flags) dev_free(ctx, g->flags, group_flag_bytes(g->M1));
+    (void)hipStreamSynchronize(ctx->stream);
     delete g;
+    return BNA_OK;
+}
+
+int bna_group_set_exact(bna_ctx *ctx, bna_group *g, int exact)
+{
+    if (!ctx || !g) return fail(ctx, BNA_ERR_ARG, "bna_group_set_exact: NULL argument");
+    if (g->count > 0) return fail(ctx, BNA_ERR_ARG, "bna_group_set_exact: subjects were already added");
+    g->exact = exact != 0;
     return BNA_OK;
 }
 
@@ -1978,6 +2071,14 @@ int bna_group_add(bna_ctx *ctx, bna_group *g, bna_series *s, int precision)
     h.nb = s->N_pad / TILE;
     h.dense = g->acc;
     h.fisher = g->fisher;
+    if (g->exact && g->fisher) {
+        if (!g->flags) {
+            if ((rc = dev_alloc(ctx, (void **)&g->flags, group_flag_bytes(g->M1)))) return rc;
+            HIPCHK(ctx, launch_zero_u64(g->flags, (int64_t)(group_flag_bytes(g->M1) / 8), ctx->stream));
+        }
+        h.flags = g->flags;
+        h.flag_r = GROUP_FLAG_R;
+    }
     SubjDesc *dsd = nullptr;
     if ((rc = ws_get_t(ctx, "group_subjdesc", 1, &dsd))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(dsd, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
@@ -1991,6 +2092,24 @@ int bna_group_add(bna_ctx *ctx, bna_group *g, bna_series *s, int precision)
     if ((rc = ws_get_t(ctx, "tile_claim", 8, &claim))) return rc;
     const int grid = ctx->grid_limit > 0 ? std::min(ctx->grid, ctx->grid_limit) : ctx->grid;
     HIPCHK(ctx, launch_corr(precision, 2, dsd, tiles, nt, claim, nullptr, grid, ctx->stream));
+    if (g->exact) {
+        // the subject's series as the FP64 recheck reads it, with its mean and norm
+        GroupCopy c;
+        c.ld = s->xv ? s->L_pad : s->L;
+        c.L = s->L;
+        c.xs_bytes = (size_t)s->N * (size_t)c.ld * 4;
+        if ((rc = dev_alloc(ctx, (void **)&c.xs, c.xs_bytes)) || (rc = dev_alloc(ctx, (void **)&c.mu, (size_t)s->N * 8)) ||
+            (rc = dev_alloc(ctx, (void **)&c.sd, (size_t)s->N * 8))) {
+            dev_free(ctx, c.xs, c.xs_bytes);
+            dev_free(ctx, c.mu, (size_t)s->N * 8);
+            return rc;
+        }
+        HIPCHK(ctx, hipMemcpyAsync(c.xs, s->xv ? s->xv : s->x, c.xs_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(c.mu, s->mu, (size_t)s->N * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(c.sd, s->sd, (size_t)s->N * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        g->subs.push_back(c);
+        g->band_max = std::max(g->band_max, band_for(precision, s->L_pad));
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCHK(ctx, hipEventSynchronize(ctx->ev[2]));
     float ms = 0.f;
@@ -2015,6 +2134,8 @@ int bna_group_finish(bna_ctx *ctx, bna_group *g)
 }
 
 int bna_group_count(const bna_group *g) { return g ? g->count : 0; }
+
+int64_t bna_group_band_pairs(const bna_group *g) { return g ? g->last_fixed : -1; }
 
 int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed)
 {
@@ -2052,6 +2173,7 @@ int bna_corr_packed(bna_ctx *ctx, bna_series *s, int precision, float *packed)
     bna_group *g = nullptr;
     int rc = bna_group_create(ctx, s->N, 0, &g);
     if (rc) return rc;
+    g->exact = false;  // one subject's r as the tile kernel gives it
     if (!(rc = bna_group_add(ctx, g, s, precision)) && !(rc = bna_group_finish(ctx, g)))
         rc = bna_group_download(ctx, g, packed);
     bna_group_destroy(ctx, g);
@@ -2089,6 +2211,25 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
     int rc;
     memset(&ctx->tm, 0, sizeof(ctx->tm));
     const auto t0 = std::chrono::steady_clock::now();
+    const bool exact = g->exact && g->count > 0 && (int)g->subs.size() == g->count;
+    g->last_fixed = 0;
+    if (exact) {
+        // Pairs whose device average may sit on the other side of the threshold
+        // than the exact one get the exact value.  For a density, the k-th
+        // largest device average t lies within the bound B of the exact r*, so
+        // recomputing every pair within 2B of t leaves only pairs whose order
+        // against r* is certain; the selection below then runs on those values.
+        const double B = group_bound(g);
+        float t = thr;
+        if (dens) {
+            const int64_t k = bna_density_to_k(N, (float)o->thr);
+            if (k < 1) return fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
+            if (k > g->M1) return fail(ctx, BNA_ERR_DENSITY, "density: more edges than pairs");
+            int64_t a0 = 0, t0_ = 0;
+            if ((rc = group_select(ctx, g, k, &t, &a0, &t0_))) return rc;
+        }
+        if ((rc = group_fix_band(ctx, g, t, (float)((dens ? 2.0 : 1.0) * B), &g->last_fixed))) return rc;
+    }
     if (dens) {
         const int64_t k = bna_density_to_k(N, (float)o->thr);
         if (k < 1) return fail(ctx, BNA_ERR_DENSITY, "density: the percentage gives no edge (k = 0)");
@@ -2159,6 +2300,24 @@ int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *o, bna_csr *
     }
     hipError_t e = hipMemcpyAsync(r.R, R.data(), sizeof(int32_t) * R.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_group_fill(keep, ld, g->acc, N, r.R, r.C, r.V, ctx->stream);
+    if (e == hipSuccess && exact && r.V && nnz > 0) {
+        // every kept edge's value is the exact average
+        int2 *pairs = nullptr;
+        unsigned long long *cnt = nullptr;
+        const int64_t cap = nnz / 2;
+        if ((rc = ws_get_t(ctx, "group_pairs", (size_t)cap, &pairs)) ||
+            (rc = ws_get_t(ctx, "group_pair_count", 1, &cnt))) {
+            bna_csr_release(ctx, &r);
+            return rc;
+        }
+        e = launch_zero_u64(cnt, 1, ctx->stream);
+        if (e == hipSuccess) e = launch_csr_upper_pairs(r.R, r.C, N, pairs, cnt, cap, ctx->stream);
+        if (e == hipSuccess && (rc = group_exact_pairs(ctx, g, pairs, cnt, cap))) {
+            bna_csr_release(ctx, &r);
+            return rc;
+        }
+        if (e == hipSuccess) e = launch_group_refill(r.R, r.C, g->acc, N, r.V, ctx->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         bna_csr_release(ctx, &r);

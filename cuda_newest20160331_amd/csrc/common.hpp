@@ -62,6 +62,9 @@ struct SubjDesc {
     int32_t fisher;      // accumulation pass: add Fisher z instead of r
     uint8_t *mirror;     // per row-view entry slot: position of the pair's other entry within
                          // its column-view fragment (null: not recorded)
+    uint32_t *flags;     // accumulation pass, Fisher sums: bit p set when some subject's |r| of
+                         // packed pair p exceeds flag_r (null: not recorded)
+    float flag_r;
 };
 
 constexpr int HIST_BINS = 4096;  // over [-1, 1], clamped at both ends
@@ -117,6 +120,31 @@ int64_t corr_spill_records(int prec, int grid_max);
 // Divide every sum by the subject count S in float and, for Fisher sums, map
 // back with the inverse transform.
 hipError_t launch_group_finish(float *acc, int64_t M1, int S, int fisher, hipStream_t st);
+// One subject of an exact group: its series voxel-major with row stride ld (the
+// normalisation's xv, or the caller's voxel-major x), mean and norm per voxel.
+struct GroupSubj {
+    const float *xs;
+    int64_t ld;
+    const double *mu, *sd;
+    int32_t L, pad;
+};
+// Pairs (i < j) of the packed triangle whose average lies within w of t, or whose
+// bit is set in flags (may be null), appended to pairs[] (count[0] counts every
+// match, also past cap).
+hipError_t launch_group_cands(const float *acc, int N, float t, float w, const uint32_t *flags, int2 *pairs,
+                              unsigned long long *count, int64_t cap, hipStream_t st);
+// For each listed pair: the average the reference's arithmetic gives from the
+// subjects' r64 (each taken as float, or its Fisher z, added in subject order in
+// float when it lies in [-1, 1]; divided by S in float; mapped back for Fisher
+// sums), written into acc at the pair's packed index.
+hipError_t launch_group_exact(const int2 *pairs, const unsigned long long *count, int64_t cap,
+                              const GroupSubj *subjs, int S, int fisher, float *acc, int N, hipStream_t st);
+// The entries (i, j) with j > i of a device CSR, appended to pairs[].
+hipError_t launch_csr_upper_pairs(const int32_t *R, const int32_t *C, int N, int2 *pairs,
+                                  unsigned long long *count, int64_t cap, hipStream_t st);
+// V[e] = acc at the packed index of entry e's pair.
+hipError_t launch_group_refill(const int32_t *R, const int32_t *C, const float *acc, int N, float *V,
+                               hipStream_t st);
 // 256-bin histogram of one radix digit of the order-preserving keys of the
 // values whose higher digits equal `prefix` under `mask` (added to hist).
 hipError_t launch_group_digits(const float *acc, int64_t M1, uint32_t prefix, uint32_t mask, int shift,

@@ -220,45 +220,65 @@ __device__ __forceinline__ float fisher_z(float r) {
     return (float)(0.5 * (double)logf((1.0f + r) / (1.0f - r)));
 }
 
-// Accumulation pass: every pair of the block whose r lies in [-1, 1] (a NaN
-// from an invalid voxel does not) adds r, or its Fisher z, to its slot of the
-// packed upper triangle, row-major with i < j (CorMat.cpp:851-866).  Lanes 0-31
-// and 32-63 touch two runs of 32 neighbouring slots.  Each pair belongs to one
-// tile, so the read-modify-write needs no atomic.
+// Accumulation pass: every pair of the block whose r is a number (a NaN from an
+// invalid voxel is not) adds r, or its Fisher z, to its slot of the packed upper
+// triangle, row-major with i < j (CorMat.cpp:851-866).  The reference adds r only
+// when it lies in [-1, 1]; the r of two valid voxels always does, so an MFMA
+// value just outside it (near-identical series) is clamped to the range instead
+// of being dropped, which keeps every term within the precision's error band of
+// the FP64 value.  For Fisher sums the pairs with |r| > flag_r in some subject
+// are marked in a bitmap (their z is too steep for the group's error bound).
+// Lanes 0-31 and 32-63 touch two runs of 32 neighbouring slots.  Each pair
+// belongs to one tile, so the read-modify-write needs no atomic.
 // A tile inside the matrix and off the diagonal updates every slot of the
-// block without a branch (a value outside [-1, 1] adds +0, which leaves the sum
-// as it is), so its sixteen loads can be in flight together; edge and diagonal
-// tiles test each element first.
+// block without a branch (a NaN adds +0, which leaves the sum as it is), so
+// its sixteen loads can be in flight together; edge and diagonal tiles test
+// each element first.
+__device__ __forceinline__ float accum_term(float v, int fisher) {
+    if (!(v == v)) return 0.0f;
+    const float r = fminf(fmaxf(v, -1.0f), 1.0f);
+    return fisher ? fisher_z(r) : r;
+}
+__device__ __forceinline__ void accum_flag(const EpiCtx &c, uint32_t *flags, float flag_r, float v, int64_t p) {
+    if (flags && fabsf(v) > flag_r)
+        __hip_atomic_fetch_or((__attribute__((address_space(1))) uint32_t *)flags + (p >> 5), 1u << (p & 31),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 template <int RB, int CB>
-__device__ __forceinline__ void epi_accum(const f32x16_t &a, const EpiCtx &c, float *dense, int fisher) {
+__device__ __forceinline__ void epi_accum(const f32x16_t &a, const EpiCtx &c, float *dense, int fisher,
+                                          uint32_t *flags, float flag_r) {
     const int lc = c.wc0 + CB * 32 + c.lrow;
     const int64_t j = c.gj0 + lc;
     const bool inner = !c.diag && c.gi0 + TILE <= c.N && c.gj0 + TILE <= c.N;
     if (inner) {
         float add[16];
-        g_f32 *slot[16];
+        int64_t pos[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int64_t i = c.gi0 + c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
-            const float v = a[e];
-            add[e] = (v >= -1.0f && v <= 1.0f) ? (fisher ? fisher_z(v) : v) : 0.0f;
-            slot[e] = (g_f32 *)dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
+            add[e] = accum_term(a[e], fisher);
+            pos[e] = i * c.N - (i + 1) * i / 2 + j - i - 1;
         }
         float old[16];
 #pragma unroll
-        for (int e = 0; e < 16; e++) old[e] = *slot[e];
+        for (int e = 0; e < 16; e++) old[e] = ((g_f32 *)dense)[pos[e]];
 #pragma unroll
-        for (int e = 0; e < 16; e++) *slot[e] = old[e] + add[e];
+        for (int e = 0; e < 16; e++) ((g_f32 *)dense)[pos[e]] = old[e] + add[e];
+        if (flags) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) accum_flag(c, flags, flag_r, a[e], pos[e]);
+        }
         return;
     }
 #pragma unroll
     for (int e = 0; e < 16; e++) {
         const int lr = c.wr0 + RB * 32 + (e >> 2) * 8 + c.half * 4 + (e & 3);
         const float v = a[e];
-        if (epi_in(c, lr, lc) && v >= -1.0f && v <= 1.0f) {
+        if (epi_in(c, lr, lc) && v == v) {
             const int64_t i = c.gi0 + lr;
-            g_f32 *slot = (g_f32 *)dense + (i * c.N - (i + 1) * i / 2 + j - i - 1);
-            *slot += fisher ? fisher_z(v) : v;
+            const int64_t p = i * c.N - (i + 1) * i / 2 + j - i - 1;
+            ((g_f32 *)dense)[p] += accum_term(v, fisher);
+            accum_flag(c, flags, flag_r, v, p);
         }
     }
 }
@@ -709,7 +729,7 @@ __global__ __launch_bounds__(CT_THREADS, 1) void corr_tile_kernel(const SubjDesc
         if constexpr (HIST == 1) {
             BNA_FOR_BLOCKS(epi_hist, ec, (uint32_t *)epi);
         } else if constexpr (HIST == 2) {
-            BNA_FOR_BLOCKS_T(epi_accum, ec, sd.dense, sd.fisher);
+            BNA_FOR_BLOCKS_T(epi_accum, ec, sd.dense, sd.fisher, sd.flags, sd.flag_r);
         } else {
             uint32_t *rowbits = (uint32_t *)epi;         // [256][8] tile-row bitmaps
             uint32_t *colbits = rowbits + TILE * 8;      // [256][8] tile-column bitmaps

@@ -214,6 +214,133 @@ __global__ __launch_bounds__(256) void group_fill_kernel(const uint8_t *__restri
     }
 }
 
+// ------------------------------------------------------------ exact averages
+// Fisher z of one float r in the reference's arithmetic (CorMat.cpp:36-42):
+// r == 1 moves to the float below 1 - 1e-6, the quotient and its log are float,
+// the half is taken in double and the result stored as float.
+__device__ __forceinline__ float group_fisher(float r) {
+    if (r == 1.0f) r = (float)((double)r - 1e-6);
+    return (float)(0.5 * (double)logf((1.0f + r) / (1.0f - r)));
+}
+
+// Wave-aggregated append of this lane's pair when `take`.
+__device__ __forceinline__ void append_pair(bool take, int i, int j, int2 *pairs, unsigned long long *count,
+                                            int64_t cap, int lane) {
+    const uint64_t m = __ballot(take);
+    if (!m) return;
+    const int leader = __builtin_ctzll(m);
+    unsigned long long b = 0;
+    if (lane == leader) b = atomicAdd(count, (unsigned long long)__popcll(m));
+    b = __shfl(b, leader, 64);
+    if (take) {
+        const unsigned long long k = b + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+        if ((int64_t)k < cap) pairs[k] = make_int2(i, j);
+    }
+}
+
+// One wave per packed row i: pairs (i, j > i) whose average is within w of t or
+// whose flag bit is set.
+__global__ __launch_bounds__(256) void group_cands_kernel(const float *__restrict__ acc, int N, float t, float w,
+                                                          const uint32_t *__restrict__ flags,
+                                                          int2 *__restrict__ pairs,
+                                                          unsigned long long *__restrict__ count, int64_t cap)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N - 1) return;
+    const int64_t p0 = packed_at(N, i, i + 1);
+    const float *row = acc + p0;
+    for (int64_t j0 = 0; j0 < N - 1 - i; j0 += 64) {
+        const int64_t q = j0 + lane;
+        bool take = false;
+        if (q < N - 1 - i) {
+            const float v = row[q];
+            take = fabsf(v - t) <= w;
+            if (flags) {
+                const int64_t p = p0 + q;
+                take = take || ((flags[p >> 5] >> (p & 31)) & 1u);
+            }
+        }
+        append_pair(take, (int)i, (int)(i + 1 + q), pairs, count, cap, lane);
+    }
+}
+
+// One wave per pair.  For each subject, r64 = sum_l z_i[l] z_j[l] with z
+// recomputed in double from the series as the normalisation writes it, in the
+// FP64 oracle's order (lane q accumulates l = q, q + 64, ... with fused
+// multiply-add, then a halving tree); then the reference's float arithmetic.
+__global__ __launch_bounds__(256) void group_exact_kernel(const int2 *__restrict__ pairs,
+                                                          const unsigned long long *__restrict__ count,
+                                                          int64_t cap, const GroupSubj *__restrict__ subjs, int S,
+                                                          int fisher, float *__restrict__ acc, int N)
+{
+    const int64_t n = (int64_t)min((unsigned long long)cap, *count);
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t c = wave; c < n; c += nwaves) {
+        const int2 pr = pairs[c];
+        float sum = 0.0f;
+        for (int s = 0; s < S; s++) {
+            const GroupSubj g = subjs[s];
+            const float *xi = g.xs + (int64_t)pr.x * g.ld;
+            const float *xj = g.xs + (int64_t)pr.y * g.ld;
+            const double mi = g.mu[pr.x], si = g.sd[pr.x], mj = g.mu[pr.y], sj = g.sd[pr.y];
+            double p = 0.0;
+            for (int l = lane; l < g.L; l += 64) {
+                const double zi = ((double)xi[l] - mi) / si;
+                const double zj = ((double)xj[l] - mj) / sj;
+                p = __builtin_fma(zi, zj, p);
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                const double o = __shfl_down(p, d, 64);
+                if (lane < d) p = p + o;
+            }
+            const float r = (float)p;
+            if (r >= -1.0f && r <= 1.0f) sum += fisher ? group_fisher(r) : r;
+        }
+        if (lane == 0) {
+            float v = sum / (float)S;
+            if (fisher) {
+                const float e = expf(2.0f * v);
+                v = (e - 1.0f) / (e + 1.0f);
+            }
+            acc[packed_at(N, pr.x, pr.y)] = v;
+        }
+    }
+}
+
+// One wave per row: the row's entries with a larger column.
+__global__ __launch_bounds__(256) void csr_upper_pairs_kernel(const int32_t *__restrict__ R,
+                                                              const int32_t *__restrict__ C, int N,
+                                                              int2 *__restrict__ pairs,
+                                                              unsigned long long *__restrict__ count, int64_t cap)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N) return;
+    for (int64_t e0 = R[i]; e0 < R[i + 1]; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const int j = e < R[i + 1] ? C[e] : -1;
+        append_pair(j > (int)i, (int)i, j, pairs, count, cap, lane);
+    }
+}
+
+__global__ __launch_bounds__(256) void group_refill_kernel(const int32_t *__restrict__ R,
+                                                           const int32_t *__restrict__ C,
+                                                           const float *__restrict__ acc, int N,
+                                                           float *__restrict__ V)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= N) return;
+    for (int64_t e = R[i] + lane; e < R[i + 1]; e += 64) {
+        const int64_t j = C[e];
+        V[e] = acc[i < j ? packed_at(N, i, j) : packed_at(N, j, i)];
+    }
+}
+
 static unsigned stream_blocks(int64_t n)
 {
     const int64_t b = (n + 255) / 256;
@@ -270,6 +397,39 @@ hipError_t launch_group_fill(const uint8_t *keep, int64_t ld, const float *acc, 
     if (ld % 64 || ld < N) return hipErrorInvalidValue;
     hipLaunchKernelGGL(group_fill_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, keep, ld, acc, N, R, C,
                        V);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_cands(const float *acc, int N, float t, float w, const uint32_t *flags, int2 *pairs,
+                              unsigned long long *count, int64_t cap, hipStream_t st)
+{
+    if (N < 2) return hipSuccess;
+    hipLaunchKernelGGL(group_cands_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, acc, N, t, w, flags,
+                       pairs, count, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_exact(const int2 *pairs, const unsigned long long *count, int64_t cap,
+                              const GroupSubj *subjs, int S, int fisher, float *acc, int N, hipStream_t st)
+{
+    if (S < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(group_exact_kernel, dim3(4096), dim3(256), 0, st, pairs, count, cap, subjs, S, fisher, acc, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_csr_upper_pairs(const int32_t *R, const int32_t *C, int N, int2 *pairs,
+                                  unsigned long long *count, int64_t cap, hipStream_t st)
+{
+    if (N < 1) return hipSuccess;
+    hipLaunchKernelGGL(csr_upper_pairs_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, R, C, N, pairs,
+                       count, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_refill(const int32_t *R, const int32_t *C, const float *acc, int N, float *V, hipStream_t st)
+{
+    if (N < 1) return hipSuccess;
+    hipLaunchKernelGGL(group_refill_kernel, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, R, C, acc, N, V);
     return hipGetLastError();
 }
 

@@ -194,12 +194,24 @@ int bna_series_download(bna_ctx *ctx, const bna_series *s, double *z64, double *
 typedef struct bna_group bna_group;
 int bna_group_create(bna_ctx *ctx, int N, int fisher, bna_group **out);
 int bna_group_destroy(bna_ctx *ctx, bna_group *g);
+/* Exact averages (the default, on): the group keeps every subject's normalised
+ * series (4 N L_pad bytes each), and bna_group_build_csr recomputes from FP64 r,
+ * in the reference's float arithmetic, every pair whose average the MFMA error
+ * bound leaves on either side of the threshold, and every kept edge's value;
+ * the edge set and values are then the FP64 oracle's (plain means bit for bit,
+ * Fisher means up to the ulp of logf / expf).  Off: averages carry the MFMA
+ * error of their terms.  No counterpart in the reference, whose averages are
+ * FP32 SGEMM sums; callers that cannot afford the copies turn it off.  Only
+ * before the first bna_group_add. */
+int bna_group_set_exact(bna_ctx *ctx, bna_group *g, int exact);
 /* Adds one subject (the correlation pass of CorMat_gpu, CorMat.cpp:489, and
  * post_block, :501-504).  The series is normalised for `precision` if needed;
  * every subject must have the group's N, L may differ. */
 int bna_group_add(bna_ctx *ctx, bna_group *g, bna_series *s, int precision);
 /* Subjects added so far. */
 int bna_group_count(const bna_group *g);
+/* Pairs the last bna_group_build_csr recomputed at the threshold (exact groups). */
+int64_t bna_group_band_pairs(const bna_group *g);
 /* Averages the sums (CorMat.cpp:548-558); no subject can be added afterwards. */
 int bna_group_finish(bna_ctx *ctx, bna_group *g);
 /* The packed averages (or sums, before bna_group_finish): M1 floats, the
@@ -212,9 +224,11 @@ int bna_group_download(bna_ctx *ctx, const bna_group *g, float *packed);
 int bna_group_build_csr(bna_ctx *ctx, bna_group *g, const bna_opts *opts, bna_csr *out);
 
 /* The packed r matrix of one subject, M1 floats in row-major upper-triangle
- * order with 0 for pairs outside [-1, 1] or with an invalid voxel: what
- * CorMat_gpu followed by post_block leaves (CorMat.cu:17, CorMat.cpp:259-262),
- * the payload of a subject's .cormat.  r carries the precision's MFMA error. */
+ * order with 0 for pairs with an invalid voxel: what CorMat_gpu followed by
+ * post_block leaves (CorMat.cu:17, CorMat.cpp:259-262), the payload of a
+ * subject's .cormat.  r carries the precision's MFMA error; a value just outside
+ * [-1, 1] (near-identical series) is clamped to it, where post_block would drop
+ * the reference's own out-of-range SGEMM values. */
 int bna_corr_packed(bna_ctx *ctx, bna_series *s, int precision, float *packed);
 /* The r threshold of a density: the k-th largest r of the subject, with k from
  * bna_density_to_k, as CorMat_spa2rth returns it

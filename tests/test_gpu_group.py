@@ -6,13 +6,16 @@ float and maps Fisher sums back (W/CorMat/CUCorMat/CUCorMat/CorMat.cpp:36-50,
 548-558, 824-887); the average is thresholded by FormFullGraph / FormFullGraph_s
 (:683-822) and listed by FormCSRGraph (:889-918).
 
-The device adds the tile kernel's r, so each sum differs from the oracle's by
-the MFMA error of its terms.  Bars: averages within AVG_TOL (plain) or
-AVG_TOL_Z (Fisher, whose slope 1 / (1 - r^2) magnifies the error); edge sets
-identical except pairs whose oracle average lies within that tolerance of the
-threshold, which are counted and must be few; values within the tolerance.
-Where the averages are exact (pairs with an invalid voxel in every subject
-average to exactly 0) the edge set is compared bit for bit, tie rule included.
+The device sums add the tile kernel's r, so before a graph is built each
+average differs from the oracle's by the MFMA error of its terms (bars AVG_TOL
+for plain means, AVG_TOL_Z for Fisher means, whose slope 1 / (1 - r^2)
+magnifies it).  An exact group (the default) then recomputes, from FP64 r in the
+reference's float arithmetic, every pair the error bound leaves on either side
+of the threshold and every kept edge: edge sets, tie rule included, equal the
+oracle's, and the values are the oracle's bit for bit for plain means.  Fisher
+means take logf / expf on the device and in the C library, which may differ by
+an ulp, so their values are compared at FISHER_ULP_TOL and an edge may differ
+only for a pair that close to the threshold.
 """
 import sys
 
@@ -26,6 +29,7 @@ pytestmark = pytest.mark.gpu
 
 AVG_TOL = 4e-6
 AVG_TOL_Z = 2e-5
+FISHER_ULP_TOL = 4 * 2.0 ** -24  # a few ulp of a value below 1
 
 
 def say(msg):
@@ -80,8 +84,34 @@ def check_graph(G, og, avg_ref, N, thr, tol):
     return len(diff)
 
 
+def kept_pairs(G, N):
+    rows = np.repeat(np.arange(G.N), np.diff(G.R))
+    lo, hi = np.minimum(rows, G.C), np.maximum(rows, G.C)
+    return lo * N - (lo + 1) * lo // 2 + hi - lo - 1
+
+
+def exact_graph_check(G, og, ref, N, thr, fisher):
+    """Edges equal the oracle's (Fisher: except pairs within FISHER_ULP_TOL of
+    thr); values equal the oracle's averages (Fisher: within FISHER_ULP_TOL)."""
+    e_gpu, e_orc = edges(G), edges(og)
+    diff = e_gpu ^ e_orc
+    if fisher:
+        for (i, j) in diff:
+            p = i * N - (i + 1) * i // 2 + j - i - 1
+            assert abs(float(ref[p]) - thr) <= FISHER_ULP_TOL, f"pair ({i},{j}) avg {ref[p]} thr {thr}"
+    else:
+        assert not diff, f"{len(diff)} edges differ from the oracle's"
+    if G.V is not None and G.nnz:
+        p = kept_pairs(G, N)
+        if fisher:
+            assert np.max(np.abs(G.V.astype(np.float64) - ref[p])) <= FISHER_ULP_TOL
+        else:
+            assert np.array_equal(G.V, ref[p])
+    return len(diff)
+
+
 @pytest.mark.parametrize("fisher", [False, True])
-@pytest.mark.parametrize("prec", ["bf16x3", "fp32"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp32", "bf16"])
 def test_group_average_and_threshold(ctx, fisher, prec):
     N, Ls = 700, [120, 200, 333]
     say(f"group N={N} L={Ls} fisher={fisher} prec={prec}")
@@ -92,34 +122,55 @@ def test_group_average_and_threshold(ctx, fisher, prec):
         assert g.count == 3
         g.finish()
         avg = g.packed()
-        tol = AVG_TOL_Z if fisher else AVG_TOL
+        tol = (0.1 if fisher else 0.02) if prec == "bf16" else (AVG_TOL_Z if fisher else AVG_TOL)
         err = np.abs(avg.astype(np.float64) - ref)
-        say(f"  max |avg - oracle| = {err.max():.3g}")
+        say(f"  device sums: max |avg - oracle| = {err.max():.3g}")
         assert err.max() <= tol
         # voxel 5 is invalid in subject 1 only: its pairs average the other two
         assert np.all(np.isfinite(avg))
         thr = 0.25
         G = g.build(thr)
         og = orc.threshold_packed(ref, N, thr)
-        nd = check_graph(G, og, ref, N, thr, tol)
-        say(f"  r > {thr}: nnz {G.nnz} oracle {og.nnz}, boundary differences {nd}")
-        assert nd <= 4
-        # values: the averages of the kept pairs
-        rows = np.repeat(np.arange(N), np.diff(G.R))
-        lo, hi = np.minimum(rows, G.C), np.maximum(rows, G.C)
-        p = lo * N - (lo + 1) * lo // 2 + hi - lo - 1
-        assert np.array_equal(G.V, avg[p])
+        nd = exact_graph_check(G, og, ref, N, thr, fisher)
+        say(f"  r > {thr}: nnz {G.nnz} oracle {og.nnz}, recomputed at the threshold {g.band_pairs}, "
+            f"differences {nd}")
         assert G.thr_used == pytest.approx(thr)
-        # density: exactly k edges
+        # density: exactly k edges, the oracle's r* and tie rule
         spa = 3.0
         k = orc.density_k(N, spa)
         Gd = g.build(density=spa, weighted=False)
         assert Gd.nnz == 2 * k and Gd.V is None
         okd = orc.select_packed(ref.astype(np.float64), N, k, literal=False)
-        assert abs(Gd.thr_used - okd.r_star) <= tol
-        nd = check_graph(Gd, okd, ref, N, okd.r_star, 2 * tol)
-        say(f"  density {spa}%: k={k} r*={Gd.thr_used:.7g} oracle {okd.r_star:.7g}, differences {nd}")
-        assert nd <= 8
+        if fisher:
+            assert abs(Gd.thr_used - okd.r_star) <= FISHER_ULP_TOL
+        else:
+            assert Gd.thr_used == okd.r_star
+            assert np.array_equal(Gd.R, okd.R) and np.array_equal(Gd.C, okd.C)
+        nd = exact_graph_check(Gd, okd, ref, N, okd.r_star, fisher)
+        say(f"  density {spa}%: k={k} r*={Gd.thr_used:.9g} oracle {okd.r_star:.9g}, "
+            f"recomputed {g.band_pairs}, differences {nd}")
+        # the weighted density graph: every kept value is the oracle's average
+        Gw = g.build(density=spa)
+        exact_graph_check(Gw, okd, ref, N, okd.r_star, fisher)
+    finally:
+        g.close()
+
+
+def test_group_inexact_option(ctx):
+    # exact = off: the MFMA sums thresholded as they are, within their bound
+    N, Ls = 500, [100, 160]
+    xs = subjects(N, Ls, 0xB17A7050)
+    ref = orc.group_average([oracle_packed(x) for x in xs], N, False)
+    g = ctx.group(N, False, exact=False)
+    try:
+        for x in xs:
+            with ctx.series(x) as s:
+                g.add(s, "bf16x3")
+        g.finish()
+        G = g.build(0.25)
+        assert g.band_pairs == 0
+        og = orc.threshold_packed(ref, N, 0.25)
+        assert check_graph(G, og, ref, N, 0.25, AVG_TOL) <= 4
     finally:
         g.close()
 
