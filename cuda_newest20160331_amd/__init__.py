@@ -38,7 +38,7 @@ EXPORTS = [
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
-    "bna_group_set_exact", "bna_group_band_pairs",
+    "bna_group_set_exact", "bna_group_band_pairs", "bna_lp", "bna_bc",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
     "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
     "bna_modularity", "bna_check_csr",
@@ -70,6 +70,7 @@ class Timings(C.Structure):
         ("corr_launches", C.c_int64),
         ("degree_ms", C.c_double), ("ec_ms", C.c_double), ("ec_iters", C.c_int64), ("cp_ms", C.c_double),
         ("modularity_ms", C.c_double), ("modularity_iters", C.c_int64),
+        ("lp_ms", C.c_double), ("bc_ms", C.c_double),
     ]
 
 
@@ -136,6 +137,8 @@ def lib() -> C.CDLL:
         "bna_group_add": (i32, [vp, vp, vp, i32]),
         "bna_group_count": (i32, [vp]),
         "bna_group_set_exact": (i32, [vp, vp, i32]),
+        "bna_lp": (i32, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), i32]),
+        "bna_bc": (i32, [vp, vp, fp]),
         "bna_group_band_pairs": (i64, [vp]),
         "bna_group_finish": (i32, [vp, vp]),
         "bna_group_download": (i32, [vp, vp, fp]),
@@ -278,6 +281,19 @@ class Graph:
         self.band = g.band
         self.max_err = g.max_err
 
+    @classmethod
+    def from_arrays(cls, R, Cc, V=None) -> "Graph":
+        """A host graph from CSR arrays (R of N + 1 offsets, C, optional V)."""
+        g = cls.__new__(cls)
+        g.R = np.ascontiguousarray(R, np.int64)
+        g.N = int(g.R.size - 1)
+        g.C = np.ascontiguousarray(Cc, np.int32)
+        g.nnz = int(g.C.size)
+        g.V = None if V is None else np.ascontiguousarray(V, np.float32)
+        g.thr_used = g.density_pct = g.band = g.max_err = 0.0
+        g.n_candidates = g.n_flips = 0
+        return g
+
 
 class Context:
     """One bna_ctx on a HIP device."""
@@ -357,6 +373,25 @@ class Context:
         mean = C.c_double()
         self._check(self._L.bna_cp(self._h, C.byref(h[0]), _f32p(out), C.byref(mean)), "bna_cp")
         return out, mean.value
+
+    def lp(self, g, hist_len: int = 0):
+        """Characteristic path length of the binary graph (bna_lp); with hist_len,
+        also the ordered-pair count per distance (index 0: pairs with no path).
+        g: a Graph (host) or a CSR (device)."""
+        h = (g, None) if isinstance(g, CSR) else _host_csr(g)
+        lp = C.c_double()
+        hist = np.zeros(max(hist_len, 0), np.int64)
+        self._check(self._L.bna_lp(self._h, C.byref(h[0]), C.byref(lp),
+                                   hist.ctypes.data_as(C.POINTER(C.c_int64)) if hist_len else None, hist_len),
+                    "bna_lp")
+        return (lp.value, hist) if hist_len else lp.value
+
+    def bc(self, g) -> np.ndarray:
+        """Betweenness centrality per vertex (bna_bc); g: Graph or device CSR."""
+        h = (g, None) if isinstance(g, CSR) else _host_csr(g)
+        out = np.empty(h[0].N, np.float32)
+        self._check(self._L.bna_bc(self._h, C.byref(h[0]), _f32p(out)), "bna_bc")
+        return out
 
     def cp_device(self, g: CSR):
         """Clustering coefficient of a device-resident CSR."""

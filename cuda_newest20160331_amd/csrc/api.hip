@@ -1656,6 +1656,102 @@ int bna_cp(bna_ctx *ctx, const bna_csr *g, float *cp, double *mean)
     return BNA_OK;
 }
 
+int bna_lp(bna_ctx *ctx, const bna_csr *g, double *lp, int64_t *hist, int hist_len)
+{
+    if (!ctx || !g || !lp) return fail(ctx, BNA_ERR_ARG, "bna_lp: NULL argument");
+    if (g->N < 2 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_lp: fewer than 2 vertices");
+    if (hist_len < 0 || (hist_len > 0 && !hist)) return fail(ctx, BNA_ERR_ARG, "bna_lp: bad histogram buffer");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    const int N = g->N;
+    void *vis, *fa, *fb;
+    unsigned long long *cnt;
+    if ((rc = ws_get(ctx, "lp_visited", (size_t)N * 32, &vis)) || (rc = ws_get(ctx, "lp_front", (size_t)N * 32, &fa)) ||
+        (rc = ws_get(ctx, "lp_next", (size_t)N * 32, &fb)) || (rc = ws_get_t(ctx, "lp_count", 1, &cnt)))
+        return rc;
+    // ordered pairs at each distance, over every batch of 256 sources
+    std::vector<int64_t> at(2, 0);
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    for (int s0 = 0; s0 < N; s0 += 256) {
+        const int ns = std::min(256, N - s0);
+        HIPCHK(ctx, launch_msbfs_init(N, s0, ns, vis, fa, ctx->stream));
+        void *f = fa, *nx = fb;
+        for (int d = 1; d < N; d++) {
+            HIPCHK(ctx, launch_zero_u64(cnt, 1, ctx->stream));
+            HIPCHK(ctx, launch_msbfs_level(R, C, N, f, nx, vis, ns, cnt, ctx->stream));
+            unsigned long long c = 0;
+            if ((rc = copy_d2h(ctx, &c, cnt, sizeof(c)))) return rc;
+            if (c == 0) break;
+            if ((int)at.size() <= d) at.resize((size_t)d + 1, 0);
+            at[(size_t)d] += (int64_t)c;
+            std::swap(f, nx);
+        }
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ctx->ev[9]));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.lp_ms = ms;
+    // CUBFS_LP.cpp:33-46: the sum of 1.0f / d over the pairs with d > 0, in
+    // double, divided by N and by N - 1, and inverted
+    double sum = 0.0;
+    int64_t reached = 0;
+    for (size_t d = 1; d < at.size(); d++) {
+        sum += (double)at[d] * (double)(1.0f / (float)d);
+        reached += at[d];
+    }
+    double r = sum;
+    r /= N;
+    r /= N - 1;
+    *lp = 1.0 / r;
+    if (hist_len > 0) {
+        for (int d = 0; d < hist_len; d++) hist[d] = d < (int)at.size() ? at[(size_t)d] : 0;
+        hist[0] = (int64_t)N * (N - 1) - reached;  // ordered pairs with no path
+    }
+    return BNA_OK;
+}
+
+int bna_bc(bna_ctx *ctx, const bna_csr *g, float *bc)
+{
+    if (!ctx || !g || !bc) return fail(ctx, BNA_ERR_ARG, "bna_bc: NULL argument");
+    if (g->N < 1 || !g->R || (g->nnz > 0 && !g->C)) return fail(ctx, BNA_ERR_ARG, "bna_bc: empty graph");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    const int N = g->N;
+    // sources per batch: two workgroups per CU, at most ~2 GB of per-source state
+    const int64_t per_src = (int64_t)N * 36 + 8;
+    const int B = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)2 * ctx->ncu, (int64_t)N,
+                                                              ((int64_t)2 << 30) / per_src}));
+    int32_t *dist, *queue, *lvl;
+    double *sigma, *delta, *out, *acc;
+    if ((rc = ws_get_t(ctx, "bc_dist", (size_t)B * N, &dist)) || (rc = ws_get_t(ctx, "bc_queue", (size_t)B * N, &queue)) ||
+        (rc = ws_get_t(ctx, "bc_lvl", (size_t)B * (N + 2), &lvl)) ||
+        (rc = ws_get_t(ctx, "bc_sigma", (size_t)B * N, &sigma)) ||
+        (rc = ws_get_t(ctx, "bc_delta", (size_t)B * N, &delta)) || (rc = ws_get_t(ctx, "bc_out", (size_t)B * N, &out)) ||
+        (rc = ws_get_t(ctx, "bc_acc", (size_t)N, &acc)))
+        return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    HIPCHK(ctx, launch_zero_u64(acc, N, ctx->stream));
+    for (int s0 = 0; s0 < N; s0 += B) {
+        const int ns = std::min(B, N - s0);
+        HIPCHK(ctx, launch_bc_batch(R, C, N, s0, ns, dist, queue, lvl, sigma, delta, out, acc, ctx->stream));
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
+    std::vector<double> h((size_t)N);
+    if ((rc = copy_d2h(ctx, h.data(), acc, sizeof(double) * (size_t)N))) return rc;  // synchronises
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.bc_ms = ms;
+    for (int v = 0; v < N; v++) bc[v] = (float)h[(size_t)v];
+    return BNA_OK;
+}
+
 int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int output, double *ec, double *lambda,
                 int *iters)
 {

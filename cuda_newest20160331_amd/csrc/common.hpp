@@ -188,6 +188,18 @@ int scan_blocks(int N);
 // Recomputes every value of a weighted CSR in double from the voxel-major series.
 // Rows [row0, row1) of the CSR; nnz is the whole CSR's entry count (sizes the grid).
 hipError_t launch_zero_u64(void *p, int64_t n, hipStream_t st);
+
+// Shortest paths of a binary graph (k_paths.hip).  Multi-source BFS over 256
+// sources [s0, s0 + ns) at a time: bit planes of 4 x 64-bit words per vertex.
+hipError_t launch_msbfs_init(int N, int s0, int ns, void *visited, void *front, hipStream_t st);
+// One level of the batch's searches; count += the pairs first reached at it.
+hipError_t launch_msbfs_level(const int32_t *R, const int32_t *C, int N, const void *front, void *next,
+                              void *visited, int ns, unsigned long long *count, hipStream_t st);
+// Brandes betweenness of sources [s0, s0 + ns): per-source state arrays of ns x N
+// (lvl: ns x (N + 2)), out ns x N; the deltas are added into bc in source order.
+hipError_t launch_bc_batch(const int32_t *R, const int32_t *C, int N, int s0, int ns, int32_t *dist,
+                           int32_t *queue, int32_t *lvl, double *sigma, double *delta, double *out, double *bc,
+                           hipStream_t st);
 hipError_t launch_narrow_rows(const int64_t *R64, int64_t n, int32_t *R32, hipStream_t st);
 bool refine_lists_supported(const float *xs, int64_t ld, int L);
 hipError_t launch_refine_lists(Entry *entries, const uint8_t *mirror, const int32_t *orig_rm, const int64_t *off_rm,

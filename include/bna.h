@@ -116,6 +116,8 @@ typedef struct bna_timings {
     double cp_ms;      /* last bna_cp: weight pass and row kernel */
     double modularity_ms;      /* last bna_modularity: the whole partition */
     int64_t modularity_iters;  /* last bna_modularity: power iterations over all rounds */
+    double lp_ms;              /* last bna_lp: every search on the device */
+    double bc_ms;              /* last bna_bc: every source's search and dependencies */
 } bna_timings;
 
 /* ----------------------------------------------------------------- context */
@@ -266,6 +268,29 @@ int bna_eigcent(bna_ctx *ctx, const bna_csr *g, double tol, int max_iter, int ou
  * W/.../modularity_GPU.cu:243); the binary tool starts from rand() seeded by
  * the clock, which no run can repeat.  g may be host or device. */
 int bna_modularity(bna_ctx *ctx, const bna_csr *g, int32_t *labels, double *Q, int *n_modules);
+
+/* Characteristic path length of the binary graph, the CUBFS_Lp tool
+ * (B/src/Lp/CUBFS_Lp/CUBFS_Lp/CUBFS_LP.cpp:14-53 CUBFS_Lp, BFS_APSP_GPU.cu:7-50
+ * APSP_BFS_node_kernel): d(i, j) = hops of a shortest path from i to j along
+ * the CSR rows, then Lp = 1 / (sum over ordered pairs with d > 0 of
+ * (double)(1.0f / d), divided by N and by N - 1); pairs with no path add
+ * nothing (an edgeless graph gives +inf).  hist (may be NULL, hist_len entries)
+ * gets the number of ordered pairs at each distance d = 1 .. hist_len - 1, and
+ * in hist[0] the ordered pairs with no path.  V is not used.  The reference
+ * adds the N^2 reciprocals one by one; here they are added per distance, so Lp
+ * agrees with it to the rounding of that sum (about 1e-15 relative).  g may be
+ * host or device. */
+int bna_lp(bna_ctx *ctx, const bna_csr *g, double *lp, int64_t *hist, int hist_len);
+/* Betweenness centrality of the binary graph by Brandes' dependencies, the CUBC
+ * tool (B/src/BC/CUBC/CUBC.cpp:13-125, BC_GPU_node_pred.cu): bc[v] = sum over
+ * sources s != v of delta_s(v), delta_s(v) = sum over the successors w of v of
+ * sigma_s(v) / sigma_s(w) * (1 + delta_s(w)), sigma the shortest-path counts.
+ * Ordered source-target pairs are counted, as the reference does (an
+ * undirected graph's values are twice the unordered ones); the _bc.nm values.
+ * Path counts and dependencies are double (the reference's are float), summed
+ * in source order; bc is rounded to float at the end.  V is not used.  g may be
+ * host or device. */
+int bna_bc(bna_ctx *ctx, const bna_csr *g, float *bc);
 
 /* No counterpart in the reference: bna_eigcent for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc).

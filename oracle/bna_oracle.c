@@ -56,6 +56,15 @@
  *                       W/Newman_Modularity_GPU/CUmodularity/modularity_GPU.cu:243
  *                       sets it (the binary tool draws rand() seeded by time(NULL),
  *                       which no run can reproduce).
+ *   orc_lp              CUBFS_Lp (B/src/Lp/CUBFS_Lp/CUBFS_Lp/CUBFS_LP.cpp:14-53):
+ *                       BFS hop distances from every vertex, then the N^2
+ *                       reciprocals 1.0f / d (d > 0) added one by one in double in
+ *                       row-major order, divided by N and N - 1, inverted; also
+ *                       the count of ordered pairs per distance.
+ *   orc_bc              CUBC's Brandes betweenness (B/src/BC/CUBC/BC_GPU_node_pred.cu):
+ *                       per source, path counts sigma and dependencies delta
+ *                       (pulled from each vertex's successors in CSR order), in
+ *                       double, added over sources in order.
  *   orc_ref_pipeline    The reference's CPU construction path end to end
  *                       (normalise, CorMat_cpu correlation, FormFullGraph(_s),
  *                       FormCSRGraph, degree, power iteration), timed by stage:
@@ -1135,4 +1144,92 @@ int64_t orc_ref_pipeline(const float *x, int N, int L, int layout, int row0, int
     free(rowbase);
     free(packed);
     return edges;
+}
+
+
+/* BFS hop distances from s along the CSR rows (dist -1: not reached); the
+ * vertices in visiting order go to queue, level by level. Returns how many. */
+static int bfs_order(const int64_t *R, const int *C, int N, int s, int *dist, int *queue)
+{
+    for (int v = 0; v < N; v++) dist[v] = -1;
+    dist[s] = 0;
+    queue[0] = s;
+    int head = 0, tail = 1;
+    while (head < tail) {
+        const int u = queue[head++];
+        for (int64_t e = R[u]; e < R[u + 1]; e++) {
+            const int w = C[e];
+            if (dist[w] < 0) {
+                dist[w] = dist[u] + 1;
+                queue[tail++] = w;
+            }
+        }
+    }
+    return tail;
+}
+
+/* CUBFS_LP.cpp:33-46.  hist[d] (d < hist_len) counts ordered pairs at distance
+ * d, hist[0] the pairs with no path.  Returns Lp. */
+double orc_lp(const int64_t *R, const int *C, int N, int64_t *hist, int hist_len)
+{
+    int *dist = (int *)malloc(sizeof(int) * (size_t)N);
+    int *queue = (int *)malloc(sizeof(int) * (size_t)N);
+    for (int d = 0; d < hist_len; d++) hist[d] = 0;
+    double Lp_result = 0.0;
+    int64_t reached = 0;
+    for (int i = 0; i < N; i++) {
+        bfs_order(R, C, N, i, dist, queue);
+        for (int j = 0; j < N; j++) {
+            const int d = dist[j];
+            if (d > 0) {
+                Lp_result += 1.0f / d;
+                reached++;
+                if (d < hist_len) hist[d]++;
+            }
+        }
+    }
+    if (hist_len > 0) hist[0] = (int64_t)N * (N - 1) - reached;
+    Lp_result /= N;
+    Lp_result /= N - 1;
+    free(dist);
+    free(queue);
+    return 1 / Lp_result;
+}
+
+/* Brandes betweenness, ordered pairs, in double. */
+void orc_bc(const int64_t *R, const int *C, int N, double *bc)
+{
+    int *dist = (int *)malloc(sizeof(int) * (size_t)N);
+    int *queue = (int *)malloc(sizeof(int) * (size_t)N);
+    double *sigma = (double *)malloc(sizeof(double) * (size_t)N);
+    double *delta = (double *)malloc(sizeof(double) * (size_t)N);
+    for (int v = 0; v < N; v++) bc[v] = 0.0;
+    for (int s = 0; s < N; s++) {
+        const int n = bfs_order(R, C, N, s, dist, queue);
+        for (int v = 0; v < N; v++) {
+            sigma[v] = 0.0;
+            delta[v] = 0.0;
+        }
+        sigma[s] = 1.0;
+        for (int k = 0; k < n; k++) {  /* predecessors are earlier in the queue */
+            const int u = queue[k];
+            for (int64_t e = R[u]; e < R[u + 1]; e++)
+                if (dist[C[e]] == dist[u] + 1) sigma[C[e]] += sigma[u];
+        }
+        for (int k = n - 1; k >= 0; k--) {
+            const int v = queue[k];
+            double part = 0.0;
+            for (int64_t e = R[v]; e < R[v + 1]; e++) {
+                const int w = C[e];
+                if (dist[w] == dist[v] + 1) part += sigma[v] / sigma[w] * (1.0 + delta[w]);
+            }
+            delta[v] = part;
+        }
+        for (int v = 0; v < N; v++)
+            if (v != s) bc[v] += delta[v];
+    }
+    free(dist);
+    free(queue);
+    free(sigma);
+    free(delta);
 }

@@ -27,6 +27,8 @@ def olib() -> C.CDLL:
     vp, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
     sig = {
         "orc_cpu_has_fma": (i32, []),
+        "orc_lp": (dbl, [vp, vp, i32, vp, i32]),
+        "orc_bc": (None, [vp, vp, i32, vp]),
         "orc_znorm": (None, [vp, i32, i32, i32, vp, vp, vp, vp, vp]),
         "orc_dot64": (dbl, [vp, vp, i32]),
         "orc_corr_pairs": (None, [vp, i32, vp, vp, i64, vp]),
@@ -156,6 +158,24 @@ def cp(R, Cc, V, N):
     out = np.empty(N, np.float32)
     mean = olib().orc_cp(_p(R), _p(Cc), _p(V), N, _p(out))
     return out, mean
+
+
+def lp(R, Cc, N, hist_len=64):
+    """(Lp as CUBFS_Lp sums it, ordered-pair count per distance) (orc_lp)."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    hist = np.zeros(hist_len, np.int64)
+    v = olib().orc_lp(_p(R), _p(Cc), N, _p(hist), hist_len)
+    return v, hist
+
+
+def bc(R, Cc, N):
+    """Brandes betweenness per vertex in double (orc_bc)."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    out = np.empty(N, np.float64)
+    olib().orc_bc(_p(R), _p(Cc), N, _p(out))
+    return out
 
 
 def eigcent(R, Cc, V, N, tol=1e-6, max_iter=10000, out_mode=0):
