@@ -264,6 +264,11 @@ def density_and_maps(ctx, steps: int) -> dict:
             _, cp_mean = ctx.cp_device(g)
             t = ctx.timings()["cp_ms"]
             cp_ms = t if cp_ms is None else min(cp_ms, t)
+        lp_ms = None
+        for _ in range(2):
+            lp_val = ctx.lp(g)
+            t = ctx.timings()["lp_ms"]
+            lp_ms = t if lp_ms is None else min(lp_ms, t)
         rl = np.diff(s.to_host(g).R)
         wedges = int(np.sum(rl.astype(np.int64) ** 2))  # (i, j, k) walks the row kernel tests
         s.free_device(g)
@@ -284,7 +289,34 @@ def density_and_maps(ctx, steps: int) -> dict:
     out["cp"] = {"ms": cp_ms, "mean_cp": cp_mean, "wedges": wedges, "wedges_per_s": wedges / (cp_ms / 1e3),
                  "note": "weighted clustering coefficient (weight pass + one workgroup per row); "
                          "wedges = sum of squared row lengths, the (i, j, k) walks tested; best of 2"}
+    out["lp"] = {"ms": lp_ms, "Lp": lp_val, "source_edges_per_s": N * nnz / (lp_ms / 1e3),
+                 "note": "characteristic path length of the binary graph: BFS from all N sources, 256 per pass "
+                         "over the CSR (bit planes); source_edges = N x nnz, the edge visits of N separate "
+                         "searches; best of 2"}
     return out
+
+
+def bc_form(ctx) -> dict:
+    """Betweenness on a C1-shaped subject's 1 % density graph (N = 5000, L = 200,
+    K = 8, phi = 0.7): Brandes from every source on the device CSR."""
+    N, L = 5000, 200
+    x = bna.synth_factor_model(N, L, 8, 0.7, 0xB17A0000 + 1000)
+    with ctx.series(x) as s:
+        s.normalize(bna.BF16X3)
+        g = s.build_device(density=1.0, precision="bf16x3", weighted=False)
+        try:
+            ms = None
+            for _ in range(2):
+                b = ctx.bc(g)
+                t = ctx.timings()["bc_ms"]
+                ms = t if ms is None else min(ms, t)
+            nnz = g.nnz
+        finally:
+            s.free_device(g)
+    return {"workload": f"N={N} x L={L}, 1 % density binary graph (nnz {nnz})", "ms": ms,
+            "teps": N * nnz / (ms / 1e3), "max_bc": float(b.max()),
+            "note": "Brandes from all N sources (one workgroup each, double path counts); "
+                    "teps = N x nnz / time; best of 2"}
 
 
 def batch_form(n_subj: int, seed: int) -> dict:
@@ -459,6 +491,7 @@ def main() -> int:
         if not a.no_maps:
             log("bench: 1 % density form and graph maps")
             maps = density_and_maps(ctx, 5)
+            maps["bc"] = bc_form(ctx)
             log(f"bench: maps {maps}")
 
     b = res["bf16x3"]
@@ -518,6 +551,8 @@ def main() -> int:
         line["degree"] = maps["degree"]
         line["eigcent"] = maps["eigcent"]
         line["cp"] = maps["cp"]
+        line["lp"] = maps["lp"]
+        line["bc"] = maps["bc"]
     if not a.no_batch:
         log("bench: batched form")
         line["batch"] = batch_form(a.batch_subjects, 0xB17A0000 + 4000)

@@ -222,3 +222,21 @@ def test_modularity_tool(subjects):
         txt = Path(str(f)[:-4] + "_modu.txt").read_text()
         assert txt.startswith(f"Number of Modules: {omods},")
         assert abs(float(txt.split("Q=")[1]) - oQ) <= 1e-5 * abs(oQ)
+
+
+def test_lp_and_bc_tools(subjects):
+    d, mask, Ls = subjects
+    say("cli: path length and betweenness over the unweighted outputs")
+    sub = d / "unweighted"
+    if not sub.exists() or not list(sub.glob("*.csr")):
+        pytest.skip("construction outputs missing")
+    run("bna_lp", sub, 0)
+    run("bna_bc", sub, 0)
+    for f in sorted(sub.glob("*.csr")):
+        g = bna.read_csr(f)
+        olp, _ = orc.lp(g.R.astype(np.int64), g.C, g.N)
+        lp = float(Path(str(f)[:-4] + "_Lp.txt").read_text())
+        assert abs(lp - olp) <= 1e-5 * abs(olp)  # six significant digits
+        ob = orc.bc(g.R.astype(np.int64), g.C, g.N)
+        b = bna.read_nm(str(f)[:-4] + "_bc.nm")
+        assert np.all(np.abs(b - ob) <= 2e-6 * np.abs(ob) + 1e-6)
