@@ -265,42 +265,63 @@ __global__ __launch_bounds__(256) void group_cands_kernel(const float *__restric
     }
 }
 
-// One wave per pair.  For each subject, r64 = sum_l z_i[l] z_j[l] with z
-// recomputed in double from the series as the normalisation writes it, in the
-// FP64 oracle's order (lane q accumulates l = q, q + 64, ... with fused
-// multiply-add, then a halving tree); then the reference's float arithmetic.
+// Four pairs per wave, one per 16-lane group.  For each subject, r64 = sum_l
+// z_i[l] z_j[l] with z recomputed in double from the series as the
+// normalisation writes it, in the FP64 oracle's order: 64 partial sums, sum q
+// taking l = q, q + 64, ... with fused multiply-add, then a halving tree (sum q
+// adds sum q + s for s = 32, 16, ..., 1).  Lane g of a group holds sums g,
+// g + 16, g + 32 and g + 48, so the tree's first two steps stay in the lane and
+// the last four are shuffles inside the group.  Then the reference's float
+// arithmetic: r as float, added (or its Fisher z) in subject order when it lies
+// in [-1, 1], divided by S, mapped back for Fisher sums.
 __global__ __launch_bounds__(256) void group_exact_kernel(const int2 *__restrict__ pairs,
                                                           const unsigned long long *__restrict__ count,
                                                           int64_t cap, const GroupSubj *__restrict__ subjs, int S,
                                                           int fisher, float *__restrict__ acc, int N)
 {
     const int64_t n = (int64_t)min((unsigned long long)cap, *count);
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t c = wave; c < n; c += nwaves) {
-        const int2 pr = pairs[c];
+    const int lane = threadIdx.x & 63, gl = lane & 15;
+    const int64_t grp = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int64_t ngrp = ((int64_t)gridDim.x * blockDim.x) >> 4;
+    // every group of a wave runs the same number of trips, so the shuffles
+    // below always see all 64 lanes
+    const int64_t trips = (n + ngrp - 1) / ngrp;
+    for (int64_t t = 0; t < trips; t++) {
+        const int64_t c = grp + t * ngrp;
+        const bool live = c < n;
+        const int2 pr = live ? pairs[c] : make_int2(0, 0);
         float sum = 0.0f;
         for (int s = 0; s < S; s++) {
             const GroupSubj g = subjs[s];
             const float *xi = g.xs + (int64_t)pr.x * g.ld;
             const float *xj = g.xs + (int64_t)pr.y * g.ld;
             const double mi = g.mu[pr.x], si = g.sd[pr.x], mj = g.mu[pr.y], sj = g.sd[pr.y];
-            double p = 0.0;
-            for (int l = lane; l < g.L; l += 64) {
-                const double zi = ((double)xi[l] - mi) / si;
-                const double zj = ((double)xj[l] - mj) / sj;
-                p = __builtin_fma(zi, zj, p);
-            }
+            double p[4] = {0.0, 0.0, 0.0, 0.0};
+            if (live) {
+                for (int b = 0; b < g.L; b += 64) {
 #pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                const double o = __shfl_down(p, d, 64);
-                if (lane < d) p = p + o;
+                    for (int k = 0; k < 4; k++) {
+                        const int l = b + gl + 16 * k;
+                        if (l < g.L) {
+                            const double zi = ((double)xi[l] - mi) / si;
+                            const double zj = ((double)xj[l] - mj) / sj;
+                            p[k] = __builtin_fma(zi, zj, p[k]);
+                        }
+                    }
+                }
             }
-            const float r = (float)p;
+            p[0] = p[0] + p[2];  // s = 32
+            p[1] = p[1] + p[3];
+            double q = p[0] + p[1];  // s = 16
+#pragma unroll
+            for (int d = 8; d >= 1; d >>= 1) {
+                const double o = __shfl_down(q, d, 16);
+                if (gl < d) q = q + o;
+            }
+            const float r = (float)q;
             if (r >= -1.0f && r <= 1.0f) sum += fisher ? group_fisher(r) : r;
         }
-        if (lane == 0) {
+        if (live && gl == 0) {
             float v = sum / (float)S;
             if (fisher) {
                 const float e = expf(2.0f * v);
