@@ -239,7 +239,20 @@ __device__ __forceinline__ void append_pair(bool take, int i, int j, int2 *pairs
 }
 
 // One wave per packed row i: pairs (i, j > i) whose average is within w of t or
-// whose flag bit is set.
+// whose flag bit is set.  The row is read twice: the first pass counts its
+// matches, so one atomic reserves the row's slots, and the second writes them
+// in column order (a returning atomic per 64 columns had made the scan
+// latency-bound).
+__device__ __forceinline__ bool cand_take(const float *row, int64_t p0, int64_t q, int64_t n, float t, float w,
+                                          const uint32_t *flags) {
+    if (q >= n) return false;
+    bool take = fabsf(row[q] - t) <= w;
+    if (flags) {
+        const int64_t p = p0 + q;
+        take = take || ((flags[p >> 5] >> (p & 31)) & 1u);
+    }
+    return take;
+}
 __global__ __launch_bounds__(256) void group_cands_kernel(const float *__restrict__ acc, int N, float t, float w,
                                                           const uint32_t *__restrict__ flags,
                                                           int2 *__restrict__ pairs,
@@ -250,18 +263,22 @@ __global__ __launch_bounds__(256) void group_cands_kernel(const float *__restric
     if (i >= N - 1) return;
     const int64_t p0 = packed_at(N, i, i + 1);
     const float *row = acc + p0;
-    for (int64_t j0 = 0; j0 < N - 1 - i; j0 += 64) {
+    const int64_t n = N - 1 - i;
+    unsigned long long c = 0;
+    for (int64_t j0 = 0; j0 < n; j0 += 64) c += __popcll(__ballot(cand_take(row, p0, j0 + lane, n, t, w, flags)));
+    if (c == 0) return;
+    unsigned long long b = 0;
+    if (lane == 0) b = atomicAdd(count, c);
+    b = __shfl(b, 0, 64);
+    for (int64_t j0 = 0; j0 < n; j0 += 64) {
         const int64_t q = j0 + lane;
-        bool take = false;
-        if (q < N - 1 - i) {
-            const float v = row[q];
-            take = fabsf(v - t) <= w;
-            if (flags) {
-                const int64_t p = p0 + q;
-                take = take || ((flags[p >> 5] >> (p & 31)) & 1u);
-            }
+        const bool take = cand_take(row, p0, q, n, t, w, flags);
+        const uint64_t m = __ballot(take);
+        if (take) {
+            const unsigned long long k = b + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+            if ((int64_t)k < cap) pairs[k] = make_int2((int)i, (int)(i + 1 + q));
         }
-        append_pair(take, (int)i, (int)(i + 1 + q), pairs, count, cap, lane);
+        b += (unsigned long long)__popcll(m);
     }
 }
 
