@@ -411,7 +411,8 @@ def measured_traffic(N: int, L: int):
     if (N, L) != (45000, 1200) or not f.exists():
         return None
     for line in f.read_text().splitlines():
-        if line.startswith("corr_tile_kernel<2, 0>"):
+        name = line.split(" ")[0].removeprefix("bna::")
+        if name in ("corr_tile8_kernel<2,", "corr_tile_kernel<2,") and line.split()[1] == "0>":
             return float(line.split()[-1])
     return None
 
