@@ -18,7 +18,9 @@ import argparse
 import hashlib
 import json
 import os
+import queue
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -141,37 +143,55 @@ def cpu_baseline(x: np.ndarray, N: int, L: int, thr: float, target_s: float, c2:
 
 
 def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms: float | None = None,
-                 reserve_eighths: int | None = None) -> dict:
+                 reserve_eighths: int | None = None, distinct: int = 0) -> dict:
     """The serving form: n_subj subjects with Poisson arrivals (mean `rate` per
     second), N log-uniform in [5000, 150000], L uniform in [150, 1200], 1 %
     density, a mix of weighted and binary requests, split-BF16.  Latency is
-    submit -> host result; each subject is then run alone through the same
-    server for its solo time."""
+    submit -> device CSR ready; each subject is then run alone through the same
+    server for its solo time.  Results are collected (and their device CSR
+    freed) by a second thread in submission order while the stream is still
+    being submitted.  distinct > 0: the stream's requests cycle through
+    that many generated subjects (a 600 s stream of 1200 requests would otherwise
+    need ~140 GB of host series)."""
     rng = np.random.default_rng(seed)
     subs = []
-    for q in range(n_subj):
+    n_gen = n_subj if distinct <= 0 else min(distinct, n_subj)
+    for q in range(n_gen):
         N = int(np.exp(rng.uniform(np.log(5000), np.log(150000))))
         L = int(rng.integers(150, 1201))
         subs.append((N, L, bool(rng.random() < 0.5)))
     gaps = rng.exponential(1.0 / rate, size=n_subj)
-    log(f"serve: generating {n_subj} subjects (sum N*L = {sum(N * L for N, L, _ in subs):.3g})")
+    log(f"serve: generating {n_gen} subjects (sum N*L = {sum(N * L for N, L, _ in subs):.3g})")
     xs = [bna.synth_factor_model(N, L, 20, 0.6, 0xB17A0000 + 5000 + q) for q, (N, L, _) in enumerate(subs)]
+    gen = (xs, subs)
+    if n_gen < n_subj:
+        xs = [gen[0][q % n_gen] for q in range(n_subj)]
+        subs = [gen[1][q % n_gen] for q in range(n_subj)]
     lat, batch = [], []
     with bna.Server(workers=workers, max_batch=8, reserve_per_worker=8 << 30, chunk_ms=chunk_ms,
                     reserve_eighths=reserve_eighths) as srv:
         # warm the kernels once
         srv.wait(srv.submit(xs[0][:, :2000], density=1.0, device_result=True), graph=False)
         t0 = time.perf_counter()
-        ids, t_next = [], t0
+        pending, res = queue.Queue(), []
+
+        def collector():
+            while (rid := pending.get()) is not None:
+                res.append(srv.wait(rid, graph=False))
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t_next = t0
         for q, (x, (N, L, weighted)) in enumerate(zip(xs, subs)):
             t_next += gaps[q]
             d = t_next - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            ids.append(srv.submit(x, density=1.0, weighted=weighted, device_result=True))
+            pending.put(srv.submit(x, density=1.0, weighted=weighted, device_result=True))
             if q % 16 == 15:
-                log(f"serve: submitted {q + 1}")
-        res = [srv.wait(i, graph=False) for i in ids]
+                log(f"serve: submitted {q + 1}, collected {len(res)}")
+        pending.put(None)
+        th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
         failed = [r.error for r in res if r.status != 0]
@@ -183,11 +203,13 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms
         devt = [r.build for r in res]
         batch = [r.batch_size for r in res]
         log(f"serve: stream done in {wall:.1f} s, {st['rounds']} rounds, failed {len(failed)}")
-        solo = []
-        for q, (x, (N, L, weighted)) in enumerate(zip(xs, subs)):
+        solo_of = []
+        for q, (x, (N, L, weighted)) in enumerate(zip(gen[0], gen[1])):
             r = srv.wait(srv.submit(x, density=1.0, weighted=weighted, device_result=True), graph=False)
-            solo.append(r.latency_ms)
-        solo = np.array(solo)
+            solo_of.append(r.latency_ms)
+            if q % 32 == 31:
+                log(f"serve: solo {q + 1} of {n_gen}")
+        solo = np.array([solo_of[q % n_gen] for q in range(n_subj)])
     ratio = lat / solo
     worst = np.argsort(ratio)[::-1][:6]
     log("serve: worst latency/solo: " + "; ".join(
@@ -202,6 +224,8 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms
         f"/{solo[q]:.0f} ms" for q in big))
     return {
         "subjects": n_subj,
+        "distinct_subjects": n_gen,
+        "stream_s": wall,
         "arrival_rate_per_s": rate,
         "N_range": [min(s[0] for s in subs), max(s[0] for s in subs)],
         "L_range": [min(s[1] for s in subs), max(s[1] for s in subs)],
@@ -222,6 +246,12 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms
                   "L uniform [150, 1200], half weighted / half binary, "
                   f"{workers} worker contexts",
     }
+
+
+# min(d, a + b) updates per second on the vector ALU: 256 CUs x 64 FP32 lanes
+# per clock x 2.4 GHz, at 1.5 instructions per update (v_add_f32 and half a
+# v_min3_f32)
+MINPLUS_PEAK = 256 * 64 * 2.4e9 / 1.5
 
 
 def density_and_maps(ctx, steps: int) -> dict:
@@ -269,6 +299,9 @@ def density_and_maps(ctx, steps: int) -> dict:
             lp_val = ctx.lp(g)
             t = ctx.timings()["lp_ms"]
             lp_ms = t if lp_ms is None else min(lp_ms, t)
+        log("maps: weighted Lp (blocked Floyd-Warshall)")
+        lpw_val, _, lpw_iso = ctx.lp_weighted(g)
+        lpw_ms = ctx.timings()["lpw_ms"]
         rl = np.diff(s.to_host(g).R)
         wedges = int(np.sum(rl.astype(np.int64) ** 2))  # (i, j, k) walks the row kernel tests
         s.free_device(g)
@@ -293,6 +326,14 @@ def density_and_maps(ctx, steps: int) -> dict:
                  "note": "characteristic path length of the binary graph: BFS from all N sources, 256 per pass "
                          "over the CSR (bit planes); source_edges = N x nnz, the edge visits of N separate "
                          "searches; best of 2"}
+    Np = (N + 127) // 128 * 128
+    upd = Np ** 3 / (lpw_ms / 1e3)
+    out["lp_weighted"] = {"ms": lpw_ms, "Lp": lpw_val, "isolated": lpw_iso, "minplus_updates_per_s": upd,
+                          "valu_frac": upd / MINPLUS_PEAK,
+                          "note": "weighted characteristic path length and nodal efficiency: blocked "
+                                  "Floyd-Warshall (128-vertex blocks) on the dense cost matrix 1/w of the "
+                                  f"weighted graph ({Np}^2 floats in HBM), Np^3 min-plus updates; peak = "
+                                  "vector ALU at one add and half a three-way min per update; one run"}
     return out
 
 
@@ -447,6 +488,7 @@ def main() -> int:
     ap.add_argument("--serve-subjects", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-distinct", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-chunk-ms", type=float, default=None)
     ap.add_argument("--serve-reserve-eighths", type=int, default=None)
@@ -554,6 +596,7 @@ def main() -> int:
         line["cp"] = maps["cp"]
         line["lp"] = maps["lp"]
         line["bc"] = maps["bc"]
+        line["lp_weighted"] = maps["lp_weighted"]
     if not a.no_batch:
         log("bench: batched form")
         line["batch"] = batch_form(a.batch_subjects, 0xB17A0000 + 4000)
@@ -565,7 +608,7 @@ def main() -> int:
     if not a.no_serve:
         log("bench: serving stream")
         sv = serve_stream(a.serve_subjects, a.serve_rate, 0xB17A0000 + 5000, a.serve_workers,
-                          a.serve_chunk_ms, a.serve_reserve_eighths)
+                          a.serve_chunk_ms, a.serve_reserve_eighths, a.serve_distinct)
         line["serving"] = sv
         line["p99_subject_latency_ms"] = sv["p99_ms"]
     if not a.no_cpu:

@@ -38,7 +38,7 @@ EXPORTS = [
     "bna_csr_release", "bna_set_chunking", "bna_set_grid_limit", "bna_grid_size", "bna_autotune",
     "bna_autotune_load", "bna_autotune_get", "bna_autotune_set", "bna_group_create",
     "bna_group_destroy", "bna_group_add", "bna_group_count", "bna_group_finish", "bna_group_download",
-    "bna_group_set_exact", "bna_group_band_pairs", "bna_lp", "bna_bc",
+    "bna_group_set_exact", "bna_group_band_pairs", "bna_lp", "bna_bc", "bna_lp_weighted",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
     "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
     "bna_modularity", "bna_check_csr",
@@ -70,7 +70,7 @@ class Timings(C.Structure):
         ("corr_launches", C.c_int64),
         ("degree_ms", C.c_double), ("ec_ms", C.c_double), ("ec_iters", C.c_int64), ("cp_ms", C.c_double),
         ("modularity_ms", C.c_double), ("modularity_iters", C.c_int64),
-        ("lp_ms", C.c_double), ("bc_ms", C.c_double),
+        ("lp_ms", C.c_double), ("bc_ms", C.c_double), ("lpw_ms", C.c_double),
     ]
 
 
@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
         "bna_group_set_exact": (i32, [vp, vp, i32]),
         "bna_lp": (i32, [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), i32]),
         "bna_bc": (i32, [vp, vp, fp]),
+        "bna_lp_weighted": (i32, [vp, vp, C.POINTER(C.c_double), fp, C.POINTER(C.c_int)]),
         "bna_group_band_pairs": (i64, [vp]),
         "bna_group_finish": (i32, [vp, vp]),
         "bna_group_download": (i32, [vp, vp, fp]),
@@ -385,6 +386,19 @@ class Context:
                                    hist.ctypes.data_as(C.POINTER(C.c_int64)) if hist_len else None, hist_len),
                     "bna_lp")
         return (lp.value, hist) if hist_len else lp.value
+
+    def lp_weighted(self, g):
+        """Weighted characteristic path length (bna_lp_weighted): returns (lp, eff,
+        isolated) -- lp over costs 1 / w by blocked Floyd-Warshall, eff the nodal
+        efficiency per vertex, isolated the number of vertices whose efficiency
+        is below 1 / N (set to 0).  g: Graph (host) or CSR (device)."""
+        h = (g, None) if isinstance(g, CSR) else _host_csr(g)
+        lp = C.c_double()
+        eff = np.empty(h[0].N, np.float32)
+        iso = C.c_int()
+        self._check(self._L.bna_lp_weighted(self._h, C.byref(h[0]), C.byref(lp), _f32p(eff), C.byref(iso)),
+                    "bna_lp_weighted")
+        return lp.value, eff, iso.value
 
     def bc(self, g) -> np.ndarray:
         """Betweenness centrality per vertex (bna_bc); g: Graph or device CSR."""

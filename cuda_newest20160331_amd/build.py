@@ -28,10 +28,11 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["k_znorm.hip", "k_corr.hip", "k_csr.hip", "k_select.hip", "k_graph.hip", "k_group.hip", "k_cp.hip", "k_modularity.hip", "k_paths.hip", "api.hip", "host_util.cpp", "serve.cpp"]
+LIB_SOURCES = ["k_znorm.hip", "k_corr.hip", "k_csr.hip", "k_select.hip", "k_graph.hip", "k_group.hip", "k_cp.hip", "k_modularity.hip", "k_paths.hip", "k_fw.hip", "api.hip", "host_util.cpp", "serve.cpp"]
 TOOLS = {"bna_cormat": "cli_cormat.cpp", "bna_degree": "cli_degree.cpp", "bna_ec": "cli_ec.cpp",
          "bna_convert_nii": "cli_convert.cpp", "bna_cp": "cli_cp.cpp", "bna_modularity": "cli_modularity.cpp",
-         "bna_lp": "cli_lp.cpp", "bna_bc": "cli_bc.cpp"}
+         "bna_lp": "cli_lp.cpp", "bna_bc": "cli_bc.cpp",
+         "bna_lpw": "cli_lpw.cpp"}
 
 
 def _run(cmd: list[str]) -> None:

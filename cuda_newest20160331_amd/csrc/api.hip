@@ -1714,6 +1714,69 @@ int bna_lp(bna_ctx *ctx, const bna_csr *g, double *lp, int64_t *hist, int hist_l
     return BNA_OK;
 }
 
+int bna_lp_weighted(bna_ctx *ctx, const bna_csr *g, double *lp, float *eff, int *isolated)
+{
+    if (!ctx || !g || !lp) return fail(ctx, BNA_ERR_ARG, "bna_lp_weighted: NULL argument");
+    if (g->N < 2 || !g->R || (g->nnz > 0 && !g->C))
+        return fail(ctx, BNA_ERR_ARG, "bna_lp_weighted: fewer than 2 vertices");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int32_t *R, *C;
+    const float *V;
+    int rc = device_view(ctx, g, &R, &C, &V);
+    if (rc) return rc;
+    const int N = g->N;
+    const int Np = round_up(N, FW_B);
+    unsigned *bad;
+    float *effd;
+    double *rows;
+    if ((rc = ws_get_t(ctx, "lpw_bad", 1, &bad)) || (rc = ws_get_t(ctx, "lpw_eff", (size_t)N, &effd)) ||
+        (rc = ws_get_t(ctx, "lpw_rows", (size_t)N, &rows)))
+        return rc;
+    // the Np x Np cost matrix lives only for this call
+    const size_t dbytes = (size_t)Np * Np * sizeof(float);
+    float *D = nullptr;
+    if ((rc = dev_alloc(ctx, (void **)&D, dbytes))) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+    rc = BNA_OK;
+    hipError_t e = launch_zero_u64(bad, 1, ctx->stream);
+    if (e == hipSuccess) e = launch_fw_init(R, C, V, N, Np, D, bad, ctx->stream);
+    unsigned hbad = 0;
+    if (e != hipSuccess) rc = fail(ctx, BNA_ERR_HIP, std::string("fw init: ") + hipGetErrorString(e));
+    if (!rc) rc = copy_d2h(ctx, &hbad, bad, sizeof(hbad));  // synchronises
+    if (!rc && (hbad & 1u))
+        rc = fail(ctx, BNA_ERR_ARG, "bna_lp_weighted: a weight is negative or NaN (costs are 1 / w)");
+    if (!rc && (hbad & 2u)) rc = fail(ctx, BNA_ERR_FORMAT, "CSR: column out of range");
+    for (int K = 0; !rc && K < Np / FW_B; K++)
+        if ((e = launch_fw_block(D, Np, K, ctx->stream)) != hipSuccess)
+            rc = fail(ctx, BNA_ERR_HIP, std::string("fw block: ") + hipGetErrorString(e));
+    if (!rc && (e = launch_fw_rows(D, Np, N, effd, rows, ctx->stream)) != hipSuccess)
+        rc = fail(ctx, BNA_ERR_HIP, std::string("fw rows: ") + hipGetErrorString(e));
+    dev_free(ctx, D, dbytes);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
+    std::vector<double> hr((size_t)N);
+    std::vector<float> he((size_t)N);
+    if ((rc = copy_d2h(ctx, hr.data(), rows, sizeof(double) * (size_t)N))) return rc;
+    if ((rc = copy_d2h(ctx, he.data(), effd, sizeof(float) * (size_t)N))) return rc;
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]));
+    ctx->tm.lpw_ms = ms;
+    // CUBFW_Lp.cpp:111-113: the double sum over the ordered pairs, divided by N
+    // and by N - 1, inverted; the rows' sums are added in row order
+    double sum = 0.0;
+    int iso = 0;
+    for (int u = 0; u < N; u++) {
+        sum += hr[(size_t)u];
+        iso += he[(size_t)u] == 0.f;
+    }
+    sum /= N;
+    sum /= N - 1;
+    *lp = 1.0 / sum;
+    if (eff) memcpy(eff, he.data(), sizeof(float) * (size_t)N);
+    if (isolated) *isolated = iso;
+    return BNA_OK;
+}
+
 int bna_bc(bna_ctx *ctx, const bna_csr *g, float *bc)
 {
     if (!ctx || !g || !bc) return fail(ctx, BNA_ERR_ARG, "bna_bc: NULL argument");

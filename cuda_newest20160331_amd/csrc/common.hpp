@@ -200,6 +200,20 @@ hipError_t launch_msbfs_level(const int32_t *R, const int32_t *C, int N, const v
 hipError_t launch_bc_batch(const int32_t *R, const int32_t *C, int N, int s0, int ns, int32_t *dist,
                            int32_t *queue, int32_t *lvl, double *sigma, double *delta, double *out, double *bc,
                            hipStream_t st);
+// All-pairs shortest paths of a weighted graph (k_fw.hip): blocked
+// Floyd-Warshall on an Np x Np float cost matrix, Np a multiple of FW_B.  The
+// block size fixes which intermediate values each phase uses, hence the float
+// distances.
+constexpr int FW_B = 128;
+// D = +inf, 0 on the diagonal, 1.0f / w (1 without weights) per CSR entry;
+// bad |= 1 for a negative or NaN weight, |= 2 for a column out of range.
+hipError_t launch_fw_init(const int32_t *R, const int32_t *C, const float *V, int N, int Np, float *D, unsigned *bad,
+                          hipStream_t st);
+// The three phases of block K.
+hipError_t launch_fw_block(float *D, int Np, int K, hipStream_t st);
+// eff[u] (nodal efficiency, float sum of 1.0f / d in column order) and
+// rowsum[u] (the same terms summed in double) of rows u < N.
+hipError_t launch_fw_rows(const float *D, int Np, int N, float *eff, double *rowsum, hipStream_t st);
 hipError_t launch_narrow_rows(const int64_t *R64, int64_t n, int32_t *R32, hipStream_t st);
 bool refine_lists_supported(const float *xs, int64_t ld, int L);
 hipError_t launch_refine_lists(Entry *entries, const uint8_t *mirror, const int32_t *orig_rm, const int64_t *off_rm,

@@ -118,6 +118,7 @@ typedef struct bna_timings {
     int64_t modularity_iters;  /* last bna_modularity: power iterations over all rounds */
     double lp_ms;              /* last bna_lp: every search on the device */
     double bc_ms;              /* last bna_bc: every source's search and dependencies */
+    double lpw_ms;             /* last bna_lp_weighted: cost matrix, Floyd-Warshall and row sums */
 } bna_timings;
 
 /* ----------------------------------------------------------------- context */
@@ -291,6 +292,22 @@ int bna_lp(bna_ctx *ctx, const bna_csr *g, double *lp, int64_t *hist, int hist_l
  * in source order; bc is rounded to float at the end.  V is not used.  g may be
  * host or device. */
 int bna_bc(bna_ctx *ctx, const bna_csr *g, float *bc);
+/* Characteristic path length and nodal efficiency of the weighted graph, the
+ * CUBFW_Lp tool (W/src/Lp/CUBFW_Lp/CUBFW_Lp/CUBFW_Lp.cpp:13-162 with
+ * init_block, FormBlock.cpp:45-75): the cost of edge (i, j) is 1.0f / w (1 when
+ * g has no V), d(i, j) the float shortest-path cost by blocked Floyd-Warshall
+ * over the dense N x N matrix (128-vertex blocks, the whole matrix in device
+ * memory: about 4 N^2 bytes, BNA_ERR_DEVICE_NOMEM past the budget), and over
+ * the ordered pairs i != j with d not below 0.1, 1.0f / d is summed (pairs with
+ * no path add 0).  lp = 1 / (sum / N / (N - 1)) (the _Lp.txt value; row sums in
+ * double, added in row order).  eff[i] (optional, N floats, the _eff.nm map) =
+ * the float sum of row i in column order divided by N - 1, set to 0 when below
+ * 1.0f / N; isolated (optional) = the number of such vertices.  Weights must be
+ * non-negative (BNA_ERR_ARG for a negative or NaN weight; a zero weight is an
+ * infinite cost).  Distances are the blocked recurrence's, so they can differ
+ * from the reference's 3072-vertex blocking in the last bit of a float sum.
+ * g may be host or device. */
+int bna_lp_weighted(bna_ctx *ctx, const bna_csr *g, double *lp, float *eff, int *isolated);
 
 /* No counterpart in the reference: bna_eigcent for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc).

@@ -65,6 +65,20 @@
  *                       per source, path counts sigma and dependencies delta
  *                       (pulled from each vertex's successors in CSR order), in
  *                       double, added over sources in order.
+ *   orc_lpw             CUBFW_Lp (W/src/Lp/CUBFW_Lp/CUBFW_Lp/CUBFW_Lp.cpp:13-162,
+ *                       init_block FormBlock.cpp:45-75): the dense cost matrix
+ *                       (+inf, 1.0f / w per CSR entry in CSR order, 0 on the
+ *                       diagonal), blocked Floyd-Warshall with block size B (the
+ *                       diagonal block, then the row and column panels against
+ *                       the finished diagonal block, then every other block as
+ *                       a min-plus product of the two panels), then per row, in
+ *                       column order, the float sum of 1.0f / d over d not below
+ *                       0.1 (nodal efficiency, divided by N - 1 and zeroed below
+ *                       1.0f / N) and the same terms in double; the rows' double
+ *                       sums added in row order give Lp.  Also the distances.
+ *   orc_sssp64          Dijkstra from every vertex over the same float costs
+ *                       with path sums in double: the check of orc_lpw's
+ *                       distances (equal on costs whose sums are exact).
  *   orc_ref_pipeline    The reference's CPU construction path end to end
  *                       (normalise, CorMat_cpu correlation, FormFullGraph(_s),
  *                       FormCSRGraph, degree, power iteration), timed by stage:
@@ -1232,4 +1246,126 @@ void orc_bc(const int64_t *R, const int *C, int N, double *bc)
     free(queue);
     free(sigma);
     free(delta);
+}
+
+
+/* ------------------------------------------------------------ weighted Lp */
+
+/* The cost matrix of FormBlock.cpp:45-75 on an Np x Np row-major array. */
+static void fw_costs(const int64_t *R, const int *C, const float *V, int N, int Np, float *D)
+{
+    const float inf = INFINITY;
+    for (int64_t e = 0; e < (int64_t)Np * Np; e++) D[e] = inf;
+    for (int i = 0; i < N; i++)
+        for (int64_t e = R[i]; e < R[i + 1]; e++) D[(int64_t)i * Np + C[e]] = V ? 1.0f / V[e] : 1.0f;
+    for (int i = 0; i < Np; i++) D[(int64_t)i * Np + i] = 0.f;
+}
+
+static inline void fw_relax(float *d, float a, float b)
+{
+    const float t = a + b;
+    if (t < *d) *d = t;
+}
+
+/* Blocked Floyd-Warshall with block size B (Np a multiple of B). */
+static void fw_blocked(float *D, int Np, int B)
+{
+    const int nb = Np / B;
+#define DD(i, j) D[(int64_t)(i) * Np + (j)]
+    for (int K = 0; K < nb; K++) {
+        const int k0 = K * B, k1 = k0 + B;
+        for (int k = k0; k < k1; k++)
+            for (int i = k0; i < k1; i++)
+                for (int j = k0; j < k1; j++) fw_relax(&DD(i, j), DD(i, k), DD(k, j));
+        for (int X = 0; X < nb; X++) {
+            if (X == K) continue;
+            const int x0 = X * B, x1 = x0 + B;
+            for (int k = k0; k < k1; k++)
+                for (int i = k0; i < k1; i++)
+                    for (int j = x0; j < x1; j++) fw_relax(&DD(i, j), DD(i, k), DD(k, j));
+            for (int k = k0; k < k1; k++)
+                for (int i = x0; i < x1; i++)
+                    for (int j = k0; j < k1; j++) fw_relax(&DD(i, j), DD(i, k), DD(k, j));
+        }
+        for (int I = 0; I < nb; I++) {
+            if (I == K) continue;
+            for (int J = 0; J < nb; J++) {
+                if (J == K) continue;
+                /* a min is exact: the k order of the terms does not matter */
+                for (int i = I * B; i < (I + 1) * B; i++)
+                    for (int k = k0; k < k1; k++) {
+                        const float a = DD(i, k);
+                        for (int j = J * B; j < (J + 1) * B; j++) fw_relax(&DD(i, j), a, DD(k, j));
+                    }
+            }
+        }
+    }
+#undef DD
+}
+
+/* Returns Lp; eff (N floats), isolated and dist (N x N, row-major) optional.
+ * B: the block size (a multiple-of-B padding of N is added). */
+double orc_lpw(const int64_t *R, const int *C, const float *V, int N, int B, float *eff, int *isolated,
+               float *dist)
+{
+    const int Np = (N + B - 1) / B * B;
+    float *D = (float *)malloc(sizeof(float) * (size_t)Np * (size_t)Np);
+    fw_costs(R, C, V, N, Np, D);
+    fw_blocked(D, Np, B);
+    double Lp_result = 0.0;
+    int iso = 0;
+    for (int u = 0; u < N; u++) {
+        float li = 0.f;
+        double rs = 0.0;
+        for (int v = 0; v < N; v++) {
+            const float d = D[(int64_t)u * Np + v];
+            if (v != u && !((double)d < 0.1)) {
+                li += 1.0f / d;
+                rs += (double)(1.0f / d);
+            }
+        }
+        Lp_result += rs;
+        li /= (float)(N - 1);
+        if (li < 1.0f / (float)N) {
+            li = 0.f;
+            iso++;
+        }
+        if (eff) eff[u] = li;
+    }
+    if (dist)
+        for (int u = 0; u < N; u++) memcpy(dist + (int64_t)u * N, D + (int64_t)u * Np, sizeof(float) * (size_t)N);
+    free(D);
+    if (isolated) *isolated = iso;
+    Lp_result /= N;
+    Lp_result /= N - 1;
+    return 1 / Lp_result;
+}
+
+/* Dijkstra (dense O(N^2) selection) from every vertex over the costs
+ * (double)(1.0f / w) (1 without weights), path sums in double; dist N x N,
+ * +inf where there is no path. */
+void orc_sssp64(const int64_t *R, const int *C, const float *V, int N, double *dist)
+{
+    char *done = (char *)malloc((size_t)N);
+    for (int s = 0; s < N; s++) {
+        double *d = dist + (int64_t)s * N;
+        for (int v = 0; v < N; v++) {
+            d[v] = INFINITY;
+            done[v] = 0;
+        }
+        d[s] = 0.0;
+        for (;;) {
+            int u = -1;
+            for (int v = 0; v < N; v++)
+                if (!done[v] && d[v] < INFINITY && (u < 0 || d[v] < d[u])) u = v;
+            if (u < 0) break;
+            done[u] = 1;
+            for (int64_t e = R[u]; e < R[u + 1]; e++) {
+                const int w = C[e];
+                const double c = V ? (double)(1.0f / V[e]) : 1.0;
+                if (w != u && d[u] + c < d[w]) d[w] = d[u] + c;
+            }
+        }
+    }
+    free(done);
 }

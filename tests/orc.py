@@ -29,6 +29,8 @@ def olib() -> C.CDLL:
         "orc_cpu_has_fma": (i32, []),
         "orc_lp": (dbl, [vp, vp, i32, vp, i32]),
         "orc_bc": (None, [vp, vp, i32, vp]),
+        "orc_lpw": (dbl, [vp, vp, vp, i32, i32, vp, vp, vp]),
+        "orc_sssp64": (None, [vp, vp, vp, i32, vp]),
         "orc_znorm": (None, [vp, i32, i32, i32, vp, vp, vp, vp, vp]),
         "orc_dot64": (dbl, [vp, vp, i32]),
         "orc_corr_pairs": (None, [vp, i32, vp, vp, i64, vp]),
@@ -175,6 +177,29 @@ def bc(R, Cc, N):
     Cc = np.ascontiguousarray(Cc, np.int32)
     out = np.empty(N, np.float64)
     olib().orc_bc(_p(R), _p(Cc), N, _p(out))
+    return out
+
+
+def lpw(R, Cc, V, N, B=128, with_dist=False):
+    """Weighted Lp as CUBFW_Lp sums it over blocked Floyd-Warshall distances
+    (block size B): (Lp, nodal efficiency, isolated count[, distances])."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    V = None if V is None else np.ascontiguousarray(V, np.float32)
+    eff = np.empty(N, np.float32)
+    iso = C.c_int()
+    dist = np.empty((N, N), np.float32) if with_dist else None
+    v = olib().orc_lpw(_p(R), _p(Cc), _p(V), N, B, _p(eff), C.byref(iso), _p(dist))
+    return (v, eff, iso.value, dist) if with_dist else (v, eff, iso.value)
+
+
+def sssp64(R, Cc, V, N):
+    """All-pairs Dijkstra over the float costs 1.0f / w with double path sums."""
+    R = np.ascontiguousarray(R, np.int64)
+    Cc = np.ascontiguousarray(Cc, np.int32)
+    V = None if V is None else np.ascontiguousarray(V, np.float32)
+    out = np.empty((N, N), np.float64)
+    olib().orc_sssp64(_p(R), _p(Cc), _p(V), N, _p(out))
     return out
 
 

@@ -240,3 +240,20 @@ def test_lp_and_bc_tools(subjects):
         ob = orc.bc(g.R.astype(np.int64), g.C, g.N)
         b = bna.read_nm(str(f)[:-4] + "_bc.nm")
         assert np.all(np.abs(b - ob) <= 2e-6 * np.abs(ob) + 1e-6)
+
+
+def test_lpw_tool(subjects, tmp_path):
+    d, mask, Ls = subjects
+    say("cli: weighted path length and nodal efficiency")
+    files = sorted((d / "weighted").glob("*.csr")) if (d / "weighted").exists() else []
+    if not files:
+        pytest.skip("construction outputs missing")
+    # one weighted graph in a directory of its own (the tool takes every .csr)
+    f = tmp_path / files[0].name
+    f.write_bytes(files[0].read_bytes())
+    run("bna_lpw", tmp_path, 0, "gn")
+    g = bna.read_csr(f)
+    olp, oeff, _ = orc.lpw(g.R.astype(np.int64), g.C, g.V, g.N)
+    lp = float(Path(str(f)[:-4] + "_Lp.txt").read_text())
+    assert abs(lp - olp) <= 1e-5 * abs(olp)  # six significant digits
+    assert np.array_equal(bna.read_nm(str(f)[:-4] + "_eff.nm"), oeff)
