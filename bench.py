@@ -248,10 +248,10 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms
     }
 
 
-# min(d, a + b) updates per second on the vector ALU: 256 CUs x 64 FP32 lanes
-# per clock x 2.4 GHz, at 1.5 instructions per update (v_add_f32 and half a
-# v_min3_f32)
-MINPLUS_PEAK = 256 * 64 * 2.4e9 / 1.5
+# min(d, a + b) updates per second on the vector ALU: MI355X's published FP32
+# vector rate, 157.3 TFLOP/s = 7.86e13 lane operations per second (an FMA counts
+# two flops), at 1.5 instructions per update (v_add_f32 and half a v_min3_f32)
+MINPLUS_PEAK = 157.3e12 / 2 / 1.5
 
 
 def density_and_maps(ctx, steps: int) -> dict:

@@ -209,13 +209,68 @@ __global__ __launch_bounds__(256) void fw_panel_kernel(float *__restrict__ D, in
 }
 
 // Phase 3: block (I, J), I != K != J: D(I, J) = min(D(I, J), min over k in K of
-// D(I, k) + D(k, J)).  P = D(I, K) and Q = D(K, J) are staged whole in LDS
-// (one workgroup per CU); each thread keeps its 8 x 8 outputs in registers and
-// reads, per 4 k, one float4 of P per row and two of Q per k.
-__global__ __launch_bounds__(256) void fw_minplus_kernel(float *__restrict__ D, int Np, int K, int nb)
+// D(I, k) + D(k, J)).  P = D(I, K) and Q = D(K, J) pass through LDS in chunks
+// of 32 k (35 KB), so three workgroups share a CU and the others compute while
+// one loads.  Each thread keeps its 8 x 8 outputs in registers and reads, per
+// 4 k, one float4 of P per row and two of Q per k.  P's float4 columns are
+// XOR-swizzled by row block ((row >> 3) & 3): the four row blocks a wave reads
+// at once then sit in different LDS banks.
+constexpr int KC = 32;      // k per LDS chunk
+constexpr int PP = KC + 4;  // P row pitch, floats
+
+__device__ __forceinline__ int p_col4(int row, int c4) { return c4 ^ ((row >> 3) & 3); }
+
+__device__ __forceinline__ void minplus_chunk(Own &o, const float (*P)[PP], const float (*Q)[FW_B], int i0, int j0)
 {
-    __shared__ __attribute__((aligned(16))) float P[FW_B][TP];
-    __shared__ __attribute__((aligned(16))) float Q[FW_B][FW_B];
+#pragma unroll 2
+    for (int k = 0; k < KC; k += 4) {
+        float4 a[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) a[r] = *reinterpret_cast<const float4 *>(&P[i0 + r][p_col4(i0 + r, k >> 2) * 4]);
+        float bq[4][8];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 x = *reinterpret_cast<const float4 *>(&Q[k + q][j0]);
+            const float4 y = *reinterpret_cast<const float4 *>(&Q[k + q][j0 + 4]);
+            bq[q][0] = x.x, bq[q][1] = x.y, bq[q][2] = x.z, bq[q][3] = x.w;
+            bq[q][4] = y.x, bq[q][5] = y.y, bq[q][6] = y.z, bq[q][7] = y.w;
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                float v = fminf(fminf(o.v[r][c], a[r].x + bq[0][c]), a[r].y + bq[1][c]);
+                o.v[r][c] = fminf(fminf(v, a[r].z + bq[2][c]), a[r].w + bq[3][c]);
+            }
+    }
+}
+
+// A thread's share of one chunk: 128 x 32 of P and 32 x 128 of Q are 1024
+// float4 each, 4 per thread.
+constexpr int SG4 = FW_B * KC / 4 / 256;
+
+__device__ __forceinline__ void chunk_load(float (*P)[PP], float (*Q)[FW_B], const float *__restrict__ D, int Np,
+                                           int64_t ri, int64_t kc, int64_t cj, int t)
+{
+    float4 sp[SG4], sq[SG4];
+#pragma unroll
+    for (int u = 0; u < SG4; u++) {
+        const int q = t + 256 * u;
+        sp[u] = *reinterpret_cast<const float4 *>(D + (ri + q / (KC / 4)) * Np + kc + (q % (KC / 4)) * 4);
+        sq[u] = *reinterpret_cast<const float4 *>(D + (kc + (q >> 5)) * Np + cj + (q & 31) * 4);
+    }
+#pragma unroll
+    for (int u = 0; u < SG4; u++) {
+        const int q = t + 256 * u, r = q / (KC / 4);
+        *reinterpret_cast<float4 *>(&P[r][p_col4(r, q % (KC / 4)) * 4]) = sp[u];
+        *reinterpret_cast<float4 *>(&Q[q >> 5][(q & 31) * 4]) = sq[u];
+    }
+}
+
+__global__ __launch_bounds__(256, 3) void fw_minplus_kernel(float *__restrict__ D, int Np, int K, int nb)
+{
+    __shared__ __attribute__((aligned(16))) float P[FW_B][PP];
+    __shared__ __attribute__((aligned(16))) float Q[KC][FW_B];
     const int t = threadIdx.x, bi = t >> 4, bj = t & 15;
     const int i0 = bi * 8, j0 = bj * 8;
     // XCD-aware order: consecutive workgroups go to different XCDs, so XCD x
@@ -232,32 +287,11 @@ __global__ __launch_bounds__(256) void fw_minplus_kernel(float *__restrict__ D, 
     const int64_t k0 = (int64_t)K * FW_B, ri = (int64_t)I * FW_B, cj = (int64_t)J * FW_B;
     Own o;
     load_own(o, D, Np, ri, cj, i0, j0);
-    for (int q = t; q < FW_B * FW_B / 4; q += 256) {
-        const int r = q / (FW_B / 4), c = (q % (FW_B / 4)) * 4;
-        *reinterpret_cast<float4 *>(&P[r][c]) = *reinterpret_cast<const float4 *>(D + (ri + r) * Np + k0 + c);
-        *reinterpret_cast<float4 *>(&Q[r][c]) = *reinterpret_cast<const float4 *>(D + (k0 + r) * Np + cj + c);
-    }
-    __syncthreads();
-#pragma unroll 2
-    for (int k = 0; k < FW_B; k += 4) {
-        float4 a[8];
-#pragma unroll
-        for (int r = 0; r < 8; r++) a[r] = *reinterpret_cast<const float4 *>(&P[i0 + r][k]);
-        float bq[4][8];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float4 x = *reinterpret_cast<const float4 *>(&Q[k + q][j0]);
-            const float4 y = *reinterpret_cast<const float4 *>(&Q[k + q][j0 + 4]);
-            bq[q][0] = x.x, bq[q][1] = x.y, bq[q][2] = x.z, bq[q][3] = x.w;
-            bq[q][4] = y.x, bq[q][5] = y.y, bq[q][6] = y.z, bq[q][7] = y.w;
-        }
-#pragma unroll
-        for (int r = 0; r < 8; r++)
-#pragma unroll
-            for (int c = 0; c < 8; c++) {
-                float v = fminf(fminf(o.v[r][c], a[r].x + bq[0][c]), a[r].y + bq[1][c]);
-                o.v[r][c] = fminf(fminf(v, a[r].z + bq[2][c]), a[r].w + bq[3][c]);
-            }
+    for (int ch = 0; ch < FW_B / KC; ch++) {
+        if (ch) __syncthreads();
+        chunk_load(P, Q, D, Np, ri, k0 + ch * KC, cj, t);
+        __syncthreads();
+        minplus_chunk(o, P, Q, i0, j0);
     }
     store_own(o, D, Np, ri, cj, i0, j0);
 }
