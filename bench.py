@@ -485,6 +485,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-c2", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--serve-subjects", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=2.0)
     ap.add_argument("--serve-workers", type=int, default=3)
@@ -518,14 +519,14 @@ def main() -> int:
     tuned = {}
     with bna.Context(0) as ctx:
         # tile traversal (band height) per precision and length bucket
-        for prec, Lt in (("bf16x3", L), ("fp32", L), ("bf16x3", 240)):
-            if prec == "fp32" and a.no_fp32:
+        for prec, Lt in (("bf16x3", L), ("fp32", L), ("bf16", L), ("bf16x3", 240)):
+            if (prec == "fp32" and a.no_fp32) or (prec == "bf16" and a.no_bf16):
                 continue
             gb, gms = ctx.autotune(prec, Lt, N=N)
             tuned[f"{prec}_L{Lt}"] = {"band_rows": gb, f"tile_ms_at_N{N}": round(gms, 4)}
         log(f"bench: autotuned {tuned}")
         with ctx.series(x) as s:
-            for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"])):
+            for prec in (["bf16x3"] + ([] if a.no_fp32 else ["fp32"]) + ([] if a.no_bf16 else ["bf16"])):
                 log(f"bench: {prec} warmup={a.warmup} steps={a.steps}")
                 ms, corr_ms, st, st_ms, sha, V = time_path(ctx, s, N, L, prec, thr, a.steps, a.warmup)
                 res[prec] = {"ms": ms, "corr_ms": corr_ms, "stats": st, "stages_ms": st_ms, "csr_sha": sha, "V": V}
@@ -589,6 +590,20 @@ def main() -> int:
         line["same_csr_as_fp32"] = f["csr_sha"] == b["csr_sha"]
         line["max_abs_dV_vs_fp32"] = float(np.max(np.abs(f["V"].astype(np.float64) - b["V"]), initial=0.0))
         line["fp32_stages_ms"] = f["stages_ms"]
+    if "bf16" in res:
+        # the other precision form of the same workload: plain BF16 on MFMA (one
+        # pass), a wider error band rechecked in FP64, the same CSR
+        p16 = res["bf16"]
+        v16 = M1 / (p16["ms"] / 1e3)
+        line["bf16"] = {"value": v16, "ms_per_step": p16["ms"], "corr_ms": p16["corr_ms"],
+                        "roofline_frac": flops / (p16["corr_ms"] / 1e3) / 1e12 / PEAK["bf16"],
+                        "band": p16["stats"][3], "band_pairs_rechecked": p16["stats"][1],
+                        "same_csr_as_bf16x3": p16["csr_sha"] == b["csr_sha"],
+                        "max_abs_dV_vs_bf16x3": float(np.max(np.abs(p16["V"].astype(np.float64) - b["V"]),
+                                                             initial=0.0)),
+                        "stages_ms": p16["stages_ms"]}
+        if "fp32" in res:
+            line["bf16"]["speedup_vs_fp32"] = v16 / line["fp32_value"]
     if maps is not None:
         line["density_form"] = maps["build"]
         line["degree"] = maps["degree"]
