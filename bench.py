@@ -173,11 +173,14 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms
         # warm the kernels once
         srv.wait(srv.submit(xs[0][:, :2000], density=1.0, device_result=True), graph=False)
         t0 = time.perf_counter()
-        pending, res = queue.Queue(), []
+        pending, res, errs = queue.Queue(), [], []
 
         def collector():
             while (rid := pending.get()) is not None:
-                res.append(srv.wait(rid, graph=False))
+                try:
+                    res.append(srv.wait(rid, graph=False))
+                except Exception as e:  # re-raised once the stream is submitted
+                    errs.append(e)
 
         th = threading.Thread(target=collector)
         th.start()
@@ -192,6 +195,8 @@ def serve_stream(n_subj: int, rate: float, seed: int, workers: int = 4, chunk_ms
                 log(f"serve: submitted {q + 1}, collected {len(res)}")
         pending.put(None)
         th.join()
+        if errs:
+            raise errs[0]
         wall = time.perf_counter() - t0
         st = srv.stats()
         failed = [r.error for r in res if r.status != 0]
