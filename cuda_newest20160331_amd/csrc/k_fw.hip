@@ -212,7 +212,8 @@ __global__ __launch_bounds__(256) void fw_panel_kernel(float *__restrict__ D, in
 // D(I, k) + D(k, J)).  P = D(I, K) and Q = D(K, J) pass through LDS in chunks
 // of 32 k (35 KB), so three workgroups share a CU and the others compute while
 // one loads.  Each thread keeps its 8 x 8 outputs in registers and reads, per
-// 4 k, one float4 of P per row and two of Q per k.  P's float4 columns are
+// 2 k, one float2 of P per row and two float4 of Q per k (few enough operand
+// registers for three workgroups without spills).  P's float4 columns are
 // XOR-swizzled by row block ((row >> 3) & 3): the four row blocks a wave reads
 // at once then sit in different LDS banks.
 constexpr int KC = 32;      // k per LDS chunk
@@ -223,13 +224,14 @@ __device__ __forceinline__ int p_col4(int row, int c4) { return c4 ^ ((row >> 3)
 __device__ __forceinline__ void minplus_chunk(Own &o, const float (*P)[PP], const float (*Q)[FW_B], int i0, int j0)
 {
 #pragma unroll 2
-    for (int k = 0; k < KC; k += 4) {
-        float4 a[8];
+    for (int k = 0; k < KC; k += 2) {
+        float2 a[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) a[r] = *reinterpret_cast<const float4 *>(&P[i0 + r][p_col4(i0 + r, k >> 2) * 4]);
-        float bq[4][8];
+        for (int r = 0; r < 8; r++)
+            a[r] = *reinterpret_cast<const float2 *>(&P[i0 + r][p_col4(i0 + r, k >> 2) * 4 + (k & 3)]);
+        float bq[2][8];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < 2; q++) {
             const float4 x = *reinterpret_cast<const float4 *>(&Q[k + q][j0]);
             const float4 y = *reinterpret_cast<const float4 *>(&Q[k + q][j0 + 4]);
             bq[q][0] = x.x, bq[q][1] = x.y, bq[q][2] = x.z, bq[q][3] = x.w;
@@ -238,10 +240,7 @@ __device__ __forceinline__ void minplus_chunk(Own &o, const float (*P)[PP], cons
 #pragma unroll
         for (int r = 0; r < 8; r++)
 #pragma unroll
-            for (int c = 0; c < 8; c++) {
-                float v = fminf(fminf(o.v[r][c], a[r].x + bq[0][c]), a[r].y + bq[1][c]);
-                o.v[r][c] = fminf(fminf(v, a[r].z + bq[2][c]), a[r].w + bq[3][c]);
-            }
+            for (int c = 0; c < 8; c++) o.v[r][c] = fminf(fminf(o.v[r][c], a[r].x + bq[0][c]), a[r].y + bq[1][c]);
     }
 }
 
