@@ -209,7 +209,9 @@ __global__ __launch_bounds__(ZR_T) void znorm_reg_kernel(const float *__restrict
                                                          unsigned char *__restrict__ panel)
 {
     __shared__ double s_part[ZV];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index, read as a wave-uniform value so the turn tests below are
+    // scalar branches
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int i = blockIdx.x * ZV + lane;
     const bool live = i < N;
     const int l0 = w * B * 16;  // this wave's first time point
