@@ -92,7 +92,9 @@ __global__ __launch_bounds__(256) void recheck_kernel(const float *__restrict__ 
 // where each tile's writes are contiguous.  This pass turns them row-major
 // ([row][k]) through LDS, 64 x 64 at a time, so the placement reads a row's
 // fragments contiguously, and adds each row's live counts into rowlen (which
-// must start at zero; integer atomics, so the sum is exact).
+// must start at zero; integer atomics, so the sum is exact).  A thread issues
+// its loads of all three tables before the one barrier, so they are in flight
+// together.
 constexpr int TR = 64;
 __global__ __launch_bounds__(256) void frag_rows_kernel(const int32_t *__restrict__ orig_km,
                                                         const int32_t *__restrict__ live_km,
@@ -102,45 +104,41 @@ __global__ __launch_bounds__(256) void frag_rows_kernel(const int32_t *__restric
                                                         unsigned long long *__restrict__ rowlen)
 {
     __shared__ int64_t t64[TR][TR + 1];
-    __shared__ int32_t t32[TR][TR + 1];
+    __shared__ int32_t to[TR][TR + 1], tl[TR][TR + 1];
     const int k0 = blockIdx.x * TR, r0 = blockIdx.y * TR;
     const int tx = threadIdx.x & (TR - 1), ty = threadIdx.x / TR;  // 64 x 4
-    // offsets
-    for (int k = ty; k < TR; k += 4) {
-        const int gk = k0 + k, gr = r0 + tx;
-        t64[k][tx] = (gk < nb1 && gr < N) ? off_km[(int64_t)gk * N + gr] : 0;
-    }
-    __syncthreads();
-    for (int r = ty; r < TR; r += 4) {
-        const int gr = r0 + r, gk = k0 + tx;
-        if (gr < N && gk < nb1) off_rm[(int64_t)gr * nb1 + gk] = t64[tx][r];
-    }
-    __syncthreads();
-    // original counts
-    for (int k = ty; k < TR; k += 4) {
-        const int gk = k0 + k, gr = r0 + tx;
-        t32[k][tx] = (gk < nb1 && gr < N) ? orig_km[(int64_t)gk * N + gr] : 0;
-    }
-    __syncthreads();
-    for (int r = ty; r < TR; r += 4) {
-        const int gr = r0 + r, gk = k0 + tx;
-        if (gr < N && gk < nb1) orig_rm[(int64_t)gr * nb1 + gk] = t32[tx][r];
-    }
-    __syncthreads();
-    // live counts, and their per-row partial sums
-    for (int k = ty; k < TR; k += 4) {
-        const int gk = k0 + k, gr = r0 + tx;
-        t32[k][tx] = (gk < nb1 && gr < N) ? live_km[(int64_t)gk * N + gr] : 0;
-    }
-    __syncthreads();
-    for (int r = ty; r < TR; r += 4) {
-        const int gr = r0 + r, gk = k0 + tx;
-        const int v = t32[tx][r];
-        if (gr < N && gk < nb1) live_rm[(int64_t)gr * nb1 + gk] = v;
-        int s = v;
+    int64_t a[TR / 4];
+    int32_t b[TR / 4], c[TR / 4];
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-        if (tx == 0 && gr < N && s) atomicAdd(&rowlen[gr], (unsigned long long)s);
+    for (int u = 0; u < TR / 4; u++) {
+        const int gk = k0 + ty + 4 * u, gr = r0 + tx;
+        const bool in = gk < nb1 && gr < N;
+        const int64_t e = (int64_t)gk * N + gr;
+        a[u] = in ? off_km[e] : 0;
+        b[u] = in ? orig_km[e] : 0;
+        c[u] = in ? live_km[e] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < TR / 4; u++) {
+        const int k = ty + 4 * u;
+        t64[k][tx] = a[u];
+        to[k][tx] = b[u];
+        tl[k][tx] = c[u];
+    }
+    __syncthreads();
+    for (int r = ty; r < TR; r += 4) {
+        const int gr = r0 + r, gk = k0 + tx;
+        const int v = tl[tx][r];
+        if (gr < N && gk < nb1) {
+            const int64_t o = (int64_t)gr * nb1 + gk;
+            off_rm[o] = t64[tx][r];
+            orig_rm[o] = to[tx][r];
+            live_rm[o] = v;
+        }
+        int sum = v;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d, 64);
+        if (tx == 0 && gr < N && sum) atomicAdd(&rowlen[gr], (unsigned long long)sum);
     }
 }
 

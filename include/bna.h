@@ -105,7 +105,7 @@ typedef struct bna_timings {
     double znorm_ms;
     double corr_ms;    /* MFMA tile kernel(s), all passes */
     double recheck_ms;
-    double csr_ms;     /* scan + placement */
+    double csr_ms;     /* fragment tables to rows, row lengths, scan and placement */
     double refine_ms;  /* value refinement */
     double select_ms;  /* density selection */
     double total_ms;   /* first to last event of the build */
