@@ -400,9 +400,14 @@ def batch_form(n_subj: int, seed: int) -> dict:
             ctx._L.bna_csr_release(ctx._h, bna.C.byref(g))
         for s_ in ser:
             s_.close()
+    # algorithmic flops 2L per pair, counted once although a density build makes
+    # two tile passes (histogram, then emission)
+    flops = float(sum(int(N) * (int(N) - 1) * int(L) for N, L in zip(Ns, Ls)))
+    tile_tf = flops / (tm["corr_ms"] / 1e3) / 1e12
     return {"subjects": n_subj, "failed": int(sum(1 for v in st if v)), "ms": dt * 1e3,
             "voxel_pairs_per_s": pairs / dt, "edges": edges, "edges_per_s": edges / dt,
             "corr_ms": tm["corr_ms"], "corr_launches": tm["corr_launches"],
+            "tile_algorithmic_TFLOP_per_s": tile_tf, "tile_roofline_frac": tile_tf / PEAK["bf16x3"],
             "N_range": [int(Ns.min()), int(Ns.max())], "L_set": sorted(set(int(v) for v in Ls)),
             "config": "density 1 % each, split-BF16, weighted CSR left in device memory, series resident"}
 
