@@ -283,6 +283,7 @@ def density_and_maps(ctx, steps: int) -> dict:
         ms = (time.perf_counter() - t0) * 1e3 / steps
         st = ctx.timings()
         nnz = g.nnz
+        band_pairs = g.n_candidates
         deg_ms = []
         for _ in range(3):  # the first call also pays the kernel's first launch
             ctx.degree_device(g)
@@ -317,7 +318,8 @@ def density_and_maps(ctx, steps: int) -> dict:
     out["build"] = {"workload": f"one subject, N={N} x L={L}, 1 % density (k = {nnz // 2} edges), split-BF16, "
                                 "weighted CSR", "ms_per_step": ms, "voxel_pairs_per_s": M1 / (ms / 1e3),
                     "edges_per_s": (nnz // 2) / (ms / 1e3), "corr_ms": st["corr_ms"], "select_ms": st["select_ms"],
-                    "recheck_ms": st["recheck_ms"], "refine_ms": st["refine_ms"]}
+                    "recheck_ms": st["recheck_ms"], "refine_ms": st["refine_ms"], "csr_ms": st["csr_ms"],
+                    "tile_launches": st["corr_launches"], "bracket_pairs_rechecked": band_pairs}
     out["degree"] = {"ms": deg_ms, "GB_per_s": deg_gbs, "roofline_frac": deg_gbs / 8000.0, "bytes": deg_bytes,
                      "note": "best of 3 calls"}
     out["eigcent"] = {"ms": ec_ms, "iterations": iters, "lambda": lam, "spmv_GB_per_s": spmv_gbs,

@@ -75,6 +75,14 @@ struct bna_ctx {
     int ncu = 256;   // compute units of the device
     int grid_limit = 0;  // > 0: launches use at most this many (a multiple of 8)
     bool graphs = true;  // replay captured HIP graphs where a path has them (BNA_GRAPHS=0: off)
+    // density bracket from a sample of tiles for subjects of at least this many
+    // pairs (BNA_BRACKET_SAMPLE=0: always the full histogram pass;
+    // BNA_BRACKET_SAMPLE_MIN=<pairs> changes the size; BNA_BRACKET_SAMPLE_SHIFT=<bins>
+    // moves the sampled bracket up, which tests use to take the fallback)
+    bool bracket_sample = true;
+    int64_t bracket_sample_min = (int64_t)1 << 26;
+    int bracket_sample_shift = 0;
+    int64_t bracket_fallbacks = 0;  // sampled brackets that missed r* (the build redid the full pass)
     // tile traversal band height per (precision, L bucket), from the autotuner
     int band[3][4] = {{8, 8, 8, 8}, {8, 8, 8, 8}, {8, 8, 8, 8}};
     // tile table cache (single-subject builds): key of the cached table
@@ -439,6 +447,10 @@ int bna_create(bna_ctx **out, int device, size_t hbm_budget)
     {
         const char *g = std::getenv("BNA_GRAPHS");
         ctx->graphs = !(g && g[0] == '0');
+        const char *bs = std::getenv("BNA_BRACKET_SAMPLE");
+        ctx->bracket_sample = !(bs && bs[0] == '0');
+        if (const char *bm = std::getenv("BNA_BRACKET_SAMPLE_MIN")) ctx->bracket_sample_min = std::atoll(bm);
+        if (const char *bh = std::getenv("BNA_BRACKET_SAMPLE_SHIFT")) ctx->bracket_sample_shift = std::atoi(bh);
     }
     if (hbm_budget == 0) {
         size_t fr = 0, tot = 0;
@@ -867,6 +879,10 @@ struct Job {
     SelState hs{};
     int64_t nnz = 0;
     bool refined = false;  // values refined on the lists before placement
+    // density: the bracket [s_lo, s_hi) of the emission pass, and whether it
+    // came from a sample of the tiles (then checked against r* afterwards)
+    double s_lo = -INFINITY, s_hi = INFINITY;
+    bool sampled = false;
 };
 
 template <typename T>
@@ -953,6 +969,39 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
     return BNA_OK;
 }
 
+static bool tile_sampled(int I, int J);
+static int64_t tile_pairs(int N, int I, int J);
+
+// upload_batch for the histogram pass when some jobs take a sample of their
+// tiles: sampled jobs contribute only their sampled tiles; spairs[q] is the
+// number of voxel pairs in job q's tiles of the table.
+static int upload_batch_sampled(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **dsd, TileItem **tiles,
+                                int64_t *ntiles, std::vector<int64_t> &spairs)
+{
+    std::vector<SubjDesc> hd;
+    for (Job *j : js) hd.push_back(j->h);
+    int rc;
+    if ((rc = ws_get_t(ctx, "batch_subjdesc", hd.size(), dsd))) return rc;
+    if ((rc = h2d_pinned(ctx, 0, *dsd, hd.data(), hd.size() * sizeof(SubjDesc)))) return rc;
+    std::vector<TileItem> t, one;
+    spairs.assign(js.size(), 0);
+    for (size_t q = 0; q < js.size(); q++) {
+        const Job *j = js[q];
+        one.clear();
+        make_tile_table(j->s->N_pad / TILE, (int)q, j->s->panel, (int64_t)j->s->N_pad * panel_unit(j->o.precision),
+                        j->s->L_pad / 16, ctx->band[j->o.precision][l_bucket(j->s->L_pad)], one);
+        for (const TileItem &it : one)
+            if (!j->sampled || tile_sampled(it.I, it.J)) {
+                t.push_back(it);
+                spairs[q] += tile_pairs(j->s->N, it.I, it.J);
+            }
+    }
+    if ((rc = ws_get_t(ctx, "batch_tiles_hist", t.size(), tiles))) return rc;
+    if ((rc = h2d_pinned(ctx, 1, *tiles, t.data(), t.size() * sizeof(TileItem)))) return rc;
+    *ntiles = (int64_t)t.size();
+    return BNA_OK;
+}
+
 static int corr_chunks(bna_ctx *ctx, int prec, int hist, SubjDesc *dsd, TileItem *tiles, int64_t nt,
                        const std::vector<Job *> &js)
 {
@@ -972,6 +1021,57 @@ static int corr_chunks(bna_ctx *ctx, int prec, int hist, SubjDesc *dsd, TileItem
 
 static void put_key(std::string &key, const void *p, size_t n) { key.append((const char *)p, n); }
 
+// Density bracket from a sample of the tiles.  Tile (I, J) is in the sample when
+// a hash of (I, J) is 0 mod BRACKET_SAMPLE_EVERY, so the sample spreads over the
+// whole triangle.  pairs_all / pairs_sampled: the voxel pairs (i < j < N) of all
+// tiles and of the sampled ones.
+constexpr int BRACKET_SAMPLE_EVERY = 8;
+static bool tile_sampled(int I, int J)
+{
+    uint32_t h = (uint32_t)I * 0x9E3779B1u ^ (uint32_t)J * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 13;
+    return h % BRACKET_SAMPLE_EVERY == 0;
+}
+static int64_t tile_pairs(int N, int I, int J)
+{
+    const int64_t ri = std::max(0, std::min(TILE, N - I * TILE)), rj = std::max(0, std::min(TILE, N - J * TILE));
+    return I == J ? ri * (ri - 1) / 2 : ri * rj;
+}
+
+// Sampled form of density_bracket: k_s = k scaled to the sample, and the
+// bracket spans the bins whose cumulative sampled count lies within a margin of
+// k_s (0.5 % plus 12 standard deviations of a binomial count), widened by the
+// error band as in the full form.  shift moves it up by that many bins.
+static DensityBracket density_bracket_sampled(const unsigned long long *hist, int64_t k, double band,
+                                              double frac, int shift)
+{
+    std::vector<int64_t> H(HIST_BINS + 2, 0);
+    for (int b = HIST_BINS - 1; b >= 0; b--) H[b] = H[b + 1] + (int64_t)hist[b];
+    const double ks = (double)k * frac;
+    const double margin = 0.005 * ks + 12.0 * std::sqrt(ks + 1.0);
+    auto top_bin = [&](double count) {  // largest bin with H >= count
+        for (int t = HIST_BINS - 1; t >= 0; t--)
+            if ((double)H[t] >= count) return t;
+        return 0;
+    };
+    int b_top = top_bin(std::max(1.0, ks - margin)), b_bot = top_bin(ks + margin);
+    b_top = std::max(0, std::min(HIST_BINS - 1, b_top + shift));
+    b_bot = std::max(0, std::min(b_top, b_bot + shift));
+    DensityBracket d{};
+    d.s_lo = b_bot - 1 <= 0 ? -INFINITY : hist_edge(b_bot - 1) - 2.0 * band;
+    d.s_hi = b_top + 2 >= HIST_BINS ? INFINITY : hist_edge(b_top + 2) + 2.0 * band;
+    int b_lo = 0, b_hi = HIST_BINS;
+    if (std::isfinite(d.s_lo)) b_lo = std::max(0, std::min(HIST_BINS - 1, (int)std::floor((d.s_lo + 1.0) * (HIST_BINS / 2)) - 1));
+    if (std::isfinite(d.s_hi)) b_hi = std::max(0, std::min(HIST_BINS, (int)std::ceil((d.s_hi + 1.0) * (HIST_BINS / 2)) + 1));
+    // capacities scaled up from the sample, with room for its error
+    d.emit_max = (int64_t)((double)H[b_lo] / frac * 1.25) + 4096;
+    d.cand_max = (int64_t)((double)(H[b_lo] - H[b_hi]) / frac * 1.25) + 4096;
+    d.valid_pairs = -1;  // unknown from a sample
+    return d;
+}
+
 static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_norm)
 {
     const int prec = jobs[0].o.precision;
@@ -990,232 +1090,273 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
         if ((j.rc = job_prepare(ctx, j))) return j.rc;
     }
 
-    // pass 1 (density subjects): histogram of the MFMA r over every valid pair
-    std::vector<Job *> dj;
-    for (Job &j : jobs)
-        if (j.k > 0 && j.rc == BNA_OK) dj.push_back(&j);
-    if (!dj.empty()) {
-        for (Job *j : dj) {
-            j->h.hist = j->hist;
-            j->h.keep_thr = INFINITY;
-            j->h.sure_thr = INFINITY;
-            HIPCHK(ctx, launch_zero_u64(j->hist, HIST_BINS, ctx->stream));
-        }
-        SubjDesc *dsd;
-        TileItem *tiles;
-        int64_t nt;
-        if ((rc = upload_batch(ctx, dj, &dsd, &tiles, &nt))) return rc;
-        HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        if ((rc = corr_chunks(ctx, prec, 1, dsd, tiles, nt, dj))) return rc;
-        launches++;
-        HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-        std::vector<unsigned long long> hh(HIST_BINS * dj.size());
-        if ((rc = pin_grow(ctx, 2, hh.size() * sizeof(unsigned long long)))) return rc;
-        unsigned long long *hp = (unsigned long long *)ctx->pin[2];
-        for (size_t q = 0; q < dj.size(); q++)
-            HIPCHK(ctx, hipMemcpyAsync(hp + q * HIST_BINS, dj[q]->hist, HIST_BINS * sizeof(unsigned long long),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        memcpy(hh.data(), hp, hh.size() * sizeof(unsigned long long));
-        float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
-        ms_corr += ms;
-        for (size_t q = 0; q < dj.size(); q++) {
-            Job &j = *dj[q];
-            j.h.hist = nullptr;
-            const DensityBracket br = density_bracket(hh.data() + q * HIST_BINS, j.k, j.band_used);
-            if (j.k > br.valid_pairs) {
-                j.rc = fail(ctx, BNA_ERR_DENSITY, "density: " + std::to_string(j.k) + " edges requested, " +
-                                                      std::to_string(br.valid_pairs) + " valid pairs");
-                continue;
-            }
-            j.h.keep_thr = std::isfinite(br.s_lo) ? f_below(br.s_lo) : -INFINITY;  // emit v >= s_lo
-            j.h.sure_thr = std::isfinite(br.s_hi) ? f_below(br.s_hi) : INFINITY;   // candidate v < s_hi
-            j.ecap = std::max<int64_t>(j.ecap, 2 * br.emit_max + 4096);
-            j.ccap = std::max<int64_t>(j.ccap, br.cand_max + 4096);
-        }
-    }
-
-    // pass 2: emission, recheck (and selection), row lengths and scan; then one
-    // host read of every job's counters.  A job whose lists overflowed grows them
-    // and runs again (the recheck and the selection skip slots past capacity).
-    for (int attempt = 0;; attempt++) {
-        std::vector<Job *> act;
+    // Density subjects bracket r* from a histogram of the MFMA r (pass 1), then
+    // emit, recheck and select (pass 2).  A large subject histograms a sample of
+    // its tiles instead of all of them; its bracket is then checked against the
+    // selected r* (the selection is exact iff s_lo + band <= r* < s_hi - band:
+    // every dropped pair lies below r* and every sure pair above it), and a
+    // subject whose sampled bracket missed runs both passes again in full.
+    for (int round = 0; round < 2; round++) {
+        // pass 1 (density subjects): histogram of the MFMA r over every valid pair
+        // (or over a sample of the tiles)
+        std::vector<Job *> dj;
         for (Job &j : jobs)
-            if (j.rc == BNA_OK && !j.done) act.push_back(&j);
-        if (act.empty()) break;
-        if (attempt == 3) return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
-        for (Job *j : act) {
-            Entry *entries;
-            Candidate *cands;
-            uint8_t *mirror = nullptr;
-            const bool refine = j->o.weighted && j->o.refine;
-            if ((refine && (rc = job_ws(ctx, *j, "mirror", (size_t)j->ecap, &mirror))) ||
-                (rc = job_ws(ctx, *j, "entries", (size_t)j->ecap, &entries)) ||
-                (rc = job_ws(ctx, *j, "cands", (size_t)j->ccap, &cands)) ||
-                (j->k > 0 && (rc = job_ws(ctx, *j, "cand_r64", (size_t)j->ccap, &j->r64c))))
+            if (j.k > 0 && j.rc == BNA_OK && !j.done) dj.push_back(&j);
+        if (!dj.empty()) {
+            bool any_sampled = false;
+            for (Job *j : dj) {
+                const int64_t N = j->s->N;
+                j->sampled = round == 0 && ctx->bracket_sample && N * (N - 1) / 2 >= ctx->bracket_sample_min;
+                any_sampled = any_sampled || j->sampled;
+                j->h.hist = j->hist;
+                j->h.keep_thr = INFINITY;
+                j->h.sure_thr = INFINITY;
+                HIPCHK(ctx, launch_zero_u64(j->hist, HIST_BINS, ctx->stream));
+            }
+            SubjDesc *dsd;
+            TileItem *tiles;
+            int64_t nt;
+            std::vector<int64_t> spairs;
+            if ((rc = any_sampled ? upload_batch_sampled(ctx, dj, &dsd, &tiles, &nt, spairs)
+                                  : upload_batch(ctx, dj, &dsd, &tiles, &nt)))
                 return rc;
-            j->h.entries = entries;
-            j->h.entry_cap = j->ecap;
-            j->h.mirror = mirror;
-            j->h.cands = cands;
-            j->h.cand_cap = j->ccap;
-        }
-        SubjDesc *dsd;
-        TileItem *tiles;
-        int64_t nt;
-        if ((rc = upload_batch(ctx, act, &dsd, &tiles, &nt))) return rc;
-        // per job, one pinned readback record: selection state, counters, nnz
-        constexpr size_t RB = (sizeof(SelState) + sizeof(act[0]->hc) + sizeof(int64_t) + 63) / 64 * 64;
-        if ((rc = pin_grow(ctx, 2, RB * act.size()))) return rc;
-        unsigned char *rb = (unsigned char *)ctx->pin[2];
-        // The segment from the emission launch to the counter readbacks.  In a
-        // batch it is captured as one HIP graph (stage events are not recorded
-        // inside; the tile kernel's time then covers the tails too) and
-        // replayed while its launch parameters, the key, stay the same.
-        const bool graph = ctx->graphs && jobs.size() > 1 && ctx->chunk_ms <= 0.0;
-        auto segment = [&](bool events) -> int {
-            for (Job *j : act) HIPCHK(ctx, launch_zero_u64(j->ctr, 8, ctx->stream));
-            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-            int rc2;
-            if ((rc2 = corr_chunks(ctx, prec, 0, dsd, tiles, nt, act))) return rc2;
-            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-            for (Job *j : act) {
-                bna_series *s = j->s;
-                const int N = s->N, nb = s->N_pad / TILE;
-                const float *xs = s->xv ? s->xv : s->x;
-                const int64_t ld = s->xv ? s->L_pad : s->L;
-                const int mode = j->k > 0 ? (j->o.recheck ? 1 : 2) : 0;
-                HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, j->h.cands, j->ctr + 1, j->h.cand_cap,
-                                           j->thr_eff, j->h.frag_live, j->h.entries, j->h.entry_cap, j->ctr + 2, mode,
-                                           j->r64c, ctx->stream));
-            }
-            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-            for (size_t q = 0; q < act.size(); q++) {
-                Job *j = act[q];
-                if (j->k > 0) {
-                    HIPCHK(ctx, launch_select(j->sel, j->ctr, j->h.cands, j->r64c, j->h.cand_cap, j->k, j->s->N,
-                                              j->h.frag_live, j->h.entries, j->h.entry_cap, ctx->stream));
-                    HIPCHK(ctx, hipMemcpyAsync(rb + q * RB, j->sel, sizeof(SelState), hipMemcpyDeviceToHost,
-                                               ctx->stream));
-                }
-            }
-            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-            for (size_t q = 0; q < act.size(); q++) {
-                Job *j = act[q];
-                const int N = j->s->N, nb = j->s->N_pad / TILE;
-                HIPCHK(ctx, launch_frag_rows(j->h.frag_orig, j->h.frag_live, j->h.frag_off, N, nb, j->orig_rm,
-                                             j->live_rm, j->off_rm, j->rowlen, ctx->stream));
-                HIPCHK(ctx, launch_scan(j->rowlen, N, j->R64, nullptr, j->bsum, ctx->stream));
-                HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState), j->ctr, sizeof(j->hc),
-                                           hipMemcpyDeviceToHost, ctx->stream));
-                HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState) + sizeof(j->hc), j->R64 + N,
-                                           sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-            }
-            if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[12], ctx->stream));  // the CSR's row pass is done
-            return BNA_OK;
-        };
-        bool replayed = false;
-        if (graph) {
-            // the tile kernel's workspaces exist before the capture, so nothing
-            // is allocated inside it
-            unsigned int *claim_ws = nullptr;
-            uint64_t *spill_ws = nullptr;
-            if ((rc = ws_get_t(ctx, "tile_claim", 8, &claim_ws)) ||
-                (rc = ws_get_t(ctx, "tile_spill", (size_t)corr_spill_records(prec, ctx->grid), &spill_ws)))
-                return rc;
-            // the key: every value the segment's launches take
-            std::string key;
-            put_key(key, &claim_ws, sizeof claim_ws);
-            put_key(key, &spill_ws, sizeof spill_ws);
-            auto put = [&](const void *p, size_t n) { put_key(key, p, n); };
-            const int grid = ctx->grid_limit > 0 ? std::min(ctx->grid, ctx->grid_limit) : ctx->grid;
-            put(&prec, sizeof prec);
-            put(&grid, sizeof grid);
-            put(&dsd, sizeof dsd);
-            put(&tiles, sizeof tiles);
-            put(&nt, sizeof nt);
-            put(&rb, sizeof rb);
-            for (Job *j : act) {
-                put(&j->h, sizeof(SubjDesc));
-                const void *ptrs[] = {j->ctr, j->sel, j->r64c, j->rowlen, j->R64, j->bsum, j->orig_rm,
-                                      j->live_rm, j->off_rm, j->s->x, j->s->xv, j->s->mu, j->s->sd};
-                put(ptrs, sizeof ptrs);
-                const int ints[] = {j->s->N, j->s->L, j->s->L_pad, j->s->N_pad, j->o.recheck};
-                put(ints, sizeof ints);
-                put(&j->thr_eff, sizeof j->thr_eff);
-                put(&j->k, sizeof j->k);
-            }
-            if (ctx->bg_exec && key == ctx->bg_key) {
-                HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-                HIPCHK(ctx, hipGraphLaunch(ctx->bg_exec, ctx->stream));
-                ctx->bg_replays++;
-                replayed = true;
-            } else {
-                if (ctx->bg_exec) (void)hipGraphExecDestroy(ctx->bg_exec);
-                if (ctx->bg_graph) (void)hipGraphDestroy(ctx->bg_graph);
-                ctx->bg_exec = nullptr;
-                ctx->bg_graph = nullptr;
-                ctx->bg_key.clear();
-                HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                const int rc2 = segment(false);
-                hipGraph_t g = nullptr;
-                const hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
-                if (rc2) {
-                    if (g) (void)hipGraphDestroy(g);
-                    return rc2;
-                }
-                HIPCHK(ctx, ee);
-                hipGraphExec_t ex = nullptr;
-                const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-                if (ei != hipSuccess) {
-                    (void)hipGraphDestroy(g);
-                    HIPCHK(ctx, ei);
-                }
-                ctx->bg_graph = g;
-                ctx->bg_exec = ex;
-                ctx->bg_key = key;
-                ctx->bg_captures++;
-                HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-                HIPCHK(ctx, hipGraphLaunch(ex, ctx->stream));
-                replayed = true;
-            }
-            // the whole segment is timed as the tile kernel's
+            HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+            if ((rc = corr_chunks(ctx, prec, 1, dsd, tiles, nt, dj))) return rc;
+            launches++;
             HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
-            HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
-            HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
-            HIPCHK(ctx, hipEventRecord(ctx->ev[12], ctx->stream));
-        } else if ((rc = segment(true))) {
-            return rc;
-        }
-        (void)replayed;
-        launches++;
-        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-        for (size_t q = 0; q < act.size(); q++) {
-            Job *j = act[q];
-            if (j->k > 0) memcpy(&j->hs, rb + q * RB, sizeof(SelState));
-            memcpy(j->hc, rb + q * RB + sizeof(SelState), sizeof(j->hc));
-            memcpy(&j->nnz, rb + q * RB + sizeof(SelState) + sizeof(j->hc), sizeof(int64_t));
-        }
-        float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
-        ms_corr += ms;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
-        ms_re += ms;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
-        ms_sel += ms;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[12]));
-        ms_rows += ms;
-        for (Job *j : act) {
-            const unsigned long long *hc = j->hc;
-            const bool e_over = (int64_t)hc[0] > j->ecap, c_over = (int64_t)hc[1] > j->ccap;
-            if (!e_over && !c_over) {
-                j->done = true;
-                continue;
+            std::vector<unsigned long long> hh(HIST_BINS * dj.size());
+            if ((rc = pin_grow(ctx, 2, hh.size() * sizeof(unsigned long long)))) return rc;
+            unsigned long long *hp = (unsigned long long *)ctx->pin[2];
+            for (size_t q = 0; q < dj.size(); q++)
+                HIPCHK(ctx, hipMemcpyAsync(hp + q * HIST_BINS, dj[q]->hist, HIST_BINS * sizeof(unsigned long long),
+                                           hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            memcpy(hh.data(), hp, hh.size() * sizeof(unsigned long long));
+            float ms = 0.f;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+            ms_corr += ms;
+            for (size_t q = 0; q < dj.size(); q++) {
+                Job &j = *dj[q];
+                j.h.hist = nullptr;
+                const int64_t N = j.s->N;
+                DensityBracket br;
+                if (j.sampled && spairs[q] > 0)
+                    br = density_bracket_sampled(hh.data() + q * HIST_BINS, j.k, j.band_used,
+                                                 (double)spairs[q] / (double)(N * (N - 1) / 2), ctx->bracket_sample_shift);
+                else {
+                    j.sampled = false;
+                    br = density_bracket(hh.data() + q * HIST_BINS, j.k, j.band_used);
+                    if (j.k > br.valid_pairs) {
+                        j.rc = fail(ctx, BNA_ERR_DENSITY, "density: " + std::to_string(j.k) + " edges requested, " +
+                                                              std::to_string(br.valid_pairs) + " valid pairs");
+                        continue;
+                    }
+                }
+                j.s_lo = br.s_lo;
+                j.s_hi = br.s_hi;
+                j.h.keep_thr = std::isfinite(br.s_lo) ? f_below(br.s_lo) : -INFINITY;  // emit v >= s_lo
+                j.h.sure_thr = std::isfinite(br.s_hi) ? f_below(br.s_hi) : INFINITY;   // candidate v < s_hi
+                j.ecap = std::max<int64_t>(j.ecap, 2 * br.emit_max + 4096);
+                j.ccap = std::max<int64_t>(j.ccap, br.cand_max + 4096);
             }
-            if (e_over) j->ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
-            if (c_over) j->ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
         }
+
+        // pass 2: emission, recheck (and selection), row lengths and scan; then one
+        // host read of every job's counters.  A job whose lists overflowed grows them
+        // and runs again (the recheck and the selection skip slots past capacity).
+        for (int attempt = 0;; attempt++) {
+            std::vector<Job *> act;
+            for (Job &j : jobs)
+                if (j.rc == BNA_OK && !j.done) act.push_back(&j);
+            if (act.empty()) break;
+            if (attempt == 3) return fail(ctx, BNA_ERR_DEVICE_NOMEM, "edge lists keep overflowing");
+            for (Job *j : act) {
+                Entry *entries;
+                Candidate *cands;
+                uint8_t *mirror = nullptr;
+                const bool refine = j->o.weighted && j->o.refine;
+                if ((refine && (rc = job_ws(ctx, *j, "mirror", (size_t)j->ecap, &mirror))) ||
+                    (rc = job_ws(ctx, *j, "entries", (size_t)j->ecap, &entries)) ||
+                    (rc = job_ws(ctx, *j, "cands", (size_t)j->ccap, &cands)) ||
+                    (j->k > 0 && (rc = job_ws(ctx, *j, "cand_r64", (size_t)j->ccap, &j->r64c))))
+                    return rc;
+                j->h.entries = entries;
+                j->h.entry_cap = j->ecap;
+                j->h.mirror = mirror;
+                j->h.cands = cands;
+                j->h.cand_cap = j->ccap;
+            }
+            SubjDesc *dsd;
+            TileItem *tiles;
+            int64_t nt;
+            if ((rc = upload_batch(ctx, act, &dsd, &tiles, &nt))) return rc;
+            // per job, one pinned readback record: selection state, counters, nnz
+            constexpr size_t RB = (sizeof(SelState) + sizeof(act[0]->hc) + sizeof(int64_t) + 63) / 64 * 64;
+            if ((rc = pin_grow(ctx, 2, RB * act.size()))) return rc;
+            unsigned char *rb = (unsigned char *)ctx->pin[2];
+            // The segment from the emission launch to the counter readbacks.  In a
+            // batch it is captured as one HIP graph (stage events are not recorded
+            // inside; the tile kernel's time then covers the tails too) and
+            // replayed while its launch parameters, the key, stay the same.
+            const bool graph = ctx->graphs && jobs.size() > 1 && ctx->chunk_ms <= 0.0;
+            auto segment = [&](bool events) -> int {
+                for (Job *j : act) HIPCHK(ctx, launch_zero_u64(j->ctr, 8, ctx->stream));
+                if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+                int rc2;
+                if ((rc2 = corr_chunks(ctx, prec, 0, dsd, tiles, nt, act))) return rc2;
+                if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+                for (Job *j : act) {
+                    bna_series *s = j->s;
+                    const int N = s->N, nb = s->N_pad / TILE;
+                    const float *xs = s->xv ? s->xv : s->x;
+                    const int64_t ld = s->xv ? s->L_pad : s->L;
+                    const int mode = j->k > 0 ? (j->o.recheck ? 1 : 2) : 0;
+                    HIPCHK(ctx, launch_recheck(xs, ld, s->mu, s->sd, N, s->L, nb, j->h.cands, j->ctr + 1, j->h.cand_cap,
+                                               j->thr_eff, j->h.frag_live, j->h.entries, j->h.entry_cap, j->ctr + 2, mode,
+                                               j->r64c, ctx->stream));
+                }
+                if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+                for (size_t q = 0; q < act.size(); q++) {
+                    Job *j = act[q];
+                    if (j->k > 0) {
+                        HIPCHK(ctx, launch_select(j->sel, j->ctr, j->h.cands, j->r64c, j->h.cand_cap, j->k, j->s->N,
+                                                  j->h.frag_live, j->h.entries, j->h.entry_cap, ctx->stream));
+                        HIPCHK(ctx, hipMemcpyAsync(rb + q * RB, j->sel, sizeof(SelState), hipMemcpyDeviceToHost,
+                                                   ctx->stream));
+                    }
+                }
+                if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+                for (size_t q = 0; q < act.size(); q++) {
+                    Job *j = act[q];
+                    const int N = j->s->N, nb = j->s->N_pad / TILE;
+                    HIPCHK(ctx, launch_frag_rows(j->h.frag_orig, j->h.frag_live, j->h.frag_off, N, nb, j->orig_rm,
+                                                 j->live_rm, j->off_rm, j->rowlen, ctx->stream));
+                    HIPCHK(ctx, launch_scan(j->rowlen, N, j->R64, nullptr, j->bsum, ctx->stream));
+                    HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState), j->ctr, sizeof(j->hc),
+                                               hipMemcpyDeviceToHost, ctx->stream));
+                    HIPCHK(ctx, hipMemcpyAsync(rb + q * RB + sizeof(SelState) + sizeof(j->hc), j->R64 + N,
+                                               sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+                }
+                if (events) HIPCHK(ctx, hipEventRecord(ctx->ev[12], ctx->stream));  // the CSR's row pass is done
+                return BNA_OK;
+            };
+            bool replayed = false;
+            if (graph) {
+                // the tile kernel's workspaces exist before the capture, so nothing
+                // is allocated inside it
+                unsigned int *claim_ws = nullptr;
+                uint64_t *spill_ws = nullptr;
+                if ((rc = ws_get_t(ctx, "tile_claim", 8, &claim_ws)) ||
+                    (rc = ws_get_t(ctx, "tile_spill", (size_t)corr_spill_records(prec, ctx->grid), &spill_ws)))
+                    return rc;
+                // the key: every value the segment's launches take
+                std::string key;
+                put_key(key, &claim_ws, sizeof claim_ws);
+                put_key(key, &spill_ws, sizeof spill_ws);
+                auto put = [&](const void *p, size_t n) { put_key(key, p, n); };
+                const int grid = ctx->grid_limit > 0 ? std::min(ctx->grid, ctx->grid_limit) : ctx->grid;
+                put(&prec, sizeof prec);
+                put(&grid, sizeof grid);
+                put(&dsd, sizeof dsd);
+                put(&tiles, sizeof tiles);
+                put(&nt, sizeof nt);
+                put(&rb, sizeof rb);
+                for (Job *j : act) {
+                    put(&j->h, sizeof(SubjDesc));
+                    const void *ptrs[] = {j->ctr, j->sel, j->r64c, j->rowlen, j->R64, j->bsum, j->orig_rm,
+                                          j->live_rm, j->off_rm, j->s->x, j->s->xv, j->s->mu, j->s->sd};
+                    put(ptrs, sizeof ptrs);
+                    const int ints[] = {j->s->N, j->s->L, j->s->L_pad, j->s->N_pad, j->o.recheck};
+                    put(ints, sizeof ints);
+                    put(&j->thr_eff, sizeof j->thr_eff);
+                    put(&j->k, sizeof j->k);
+                }
+                if (ctx->bg_exec && key == ctx->bg_key) {
+                    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+                    HIPCHK(ctx, hipGraphLaunch(ctx->bg_exec, ctx->stream));
+                    ctx->bg_replays++;
+                    replayed = true;
+                } else {
+                    if (ctx->bg_exec) (void)hipGraphExecDestroy(ctx->bg_exec);
+                    if (ctx->bg_graph) (void)hipGraphDestroy(ctx->bg_graph);
+                    ctx->bg_exec = nullptr;
+                    ctx->bg_graph = nullptr;
+                    ctx->bg_key.clear();
+                    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                    const int rc2 = segment(false);
+                    hipGraph_t g = nullptr;
+                    const hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+                    if (rc2) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return rc2;
+                    }
+                    HIPCHK(ctx, ee);
+                    hipGraphExec_t ex = nullptr;
+                    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                    if (ei != hipSuccess) {
+                        (void)hipGraphDestroy(g);
+                        HIPCHK(ctx, ei);
+                    }
+                    ctx->bg_graph = g;
+                    ctx->bg_exec = ex;
+                    ctx->bg_key = key;
+                    ctx->bg_captures++;
+                    HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+                    HIPCHK(ctx, hipGraphLaunch(ex, ctx->stream));
+                    replayed = true;
+                }
+                // the whole segment is timed as the tile kernel's
+                HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+                HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+                HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+                HIPCHK(ctx, hipEventRecord(ctx->ev[12], ctx->stream));
+            } else if ((rc = segment(true))) {
+                return rc;
+            }
+            (void)replayed;
+            launches++;
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            for (size_t q = 0; q < act.size(); q++) {
+                Job *j = act[q];
+                if (j->k > 0) memcpy(&j->hs, rb + q * RB, sizeof(SelState));
+                memcpy(j->hc, rb + q * RB + sizeof(SelState), sizeof(j->hc));
+                memcpy(&j->nnz, rb + q * RB + sizeof(SelState) + sizeof(j->hc), sizeof(int64_t));
+            }
+            float ms = 0.f;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+            ms_corr += ms;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
+            ms_re += ms;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+            ms_sel += ms;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[12]));
+            ms_rows += ms;
+            for (Job *j : act) {
+                const unsigned long long *hc = j->hc;
+                const bool e_over = (int64_t)hc[0] > j->ecap, c_over = (int64_t)hc[1] > j->ccap;
+                if (!e_over && !c_over) {
+                    j->done = true;
+                    continue;
+                }
+                if (e_over) j->ecap = (int64_t)(hc[0] + hc[0] / 8 + 1024);
+                if (c_over) j->ccap = (int64_t)(hc[1] + hc[1] / 8 + 1024);
+            }
+        }
+
+        // a sampled bracket must contain r* with the band to spare on both sides
+        bool redo = false;
+        for (Job &j : jobs) {
+            if (j.rc != BNA_OK || j.k <= 0 || !j.sampled) continue;
+            j.sampled = false;
+            const double rs = decode_desc_key(j.hs.kmax);
+            const bool ok = !j.hs.err && j.nnz == 2 * j.k && j.s_lo + j.band_used <= rs && rs < j.s_hi - j.band_used;
+            if (!ok) {
+                j.done = false;
+                redo = true;
+                ctx->bracket_fallbacks++;
+            }
+        }
+        if (!redo) break;
     }
 
     // value refinement on the lists, then placement, of every finished job
