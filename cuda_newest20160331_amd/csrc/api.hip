@@ -970,6 +970,7 @@ static int upload_batch(bna_ctx *ctx, const std::vector<Job *> &js, SubjDesc **d
 }
 
 static bool tile_sampled(int I, int J);
+static int sampled_tile_count(int nb);
 static int64_t tile_pairs(int N, int I, int J);
 
 // upload_batch for the histogram pass when some jobs take a sample of their
@@ -1033,6 +1034,14 @@ static bool tile_sampled(int I, int J)
     h *= 0x2C1B3C6Du;
     h ^= h >> 13;
     return h % BRACKET_SAMPLE_EVERY == 0;
+}
+// Tiles of an nb x nb upper triangle that are in the sample.
+static int sampled_tile_count(int nb)
+{
+    int c = 0;
+    for (int J = 0; J < nb; J++)
+        for (int I = 0; I <= J; I++) c += tile_sampled(I, J);
+    return c;
 }
 static int64_t tile_pairs(int N, int I, int J)
 {
@@ -1106,7 +1115,8 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
             bool any_sampled = false;
             for (Job *j : dj) {
                 const int64_t N = j->s->N;
-                j->sampled = round == 0 && ctx->bracket_sample && N * (N - 1) / 2 >= ctx->bracket_sample_min;
+                j->sampled = round == 0 && ctx->bracket_sample && N * (N - 1) / 2 >= ctx->bracket_sample_min &&
+                             sampled_tile_count(j->s->N_pad / TILE) >= 8;
                 any_sampled = any_sampled || j->sampled;
                 j->h.hist = j->hist;
                 j->h.keep_thr = INFINITY;

@@ -121,3 +121,18 @@ def test_sampled_bracket_binary_and_batch(monkeypatch, subject):
         solo, _ = build(monkeypatch, {"BNA_BRACKET_SAMPLE": "0"}, v, density=spa, precision="bf16x3")
         say(f"  subject {q}: nnz {h.nnz}")
         same(h, solo, thr=False)  # a copied-out device CSR carries no r*
+
+
+def test_small_subject_keeps_the_full_pass(monkeypatch):
+    # N = 300 has 3 tiles, too few to sample: the build takes the full
+    # histogram even with the sampling threshold at zero
+    N, L, spa = 300, 40, 5.0
+    say("sampled bracket small subject")
+    x = bna.synth_factor_model(N, L, 6, 0.5, 0xB4AF)
+    z64, _, _, _, valid = orc.znorm(x, 0)
+    k = orc.density_k(N, spa)
+    og = orc.build_density(z64, valid, k)
+    g, t = build(monkeypatch, {"BNA_BRACKET_SAMPLE_MIN": "0"}, x, density=spa, precision="bf16x3")
+    assert t["corr_launches"] == 2
+    compare(g, og)
+    assert g.thr_used == og.r_star
