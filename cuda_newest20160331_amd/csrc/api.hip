@@ -1102,7 +1102,7 @@ static int build_batch(bna_ctx *ctx, std::vector<Job> &jobs, bool timing_from_no
     // Density subjects bracket r* from a histogram of the MFMA r (pass 1), then
     // emit, recheck and select (pass 2).  A large subject histograms a sample of
     // its tiles instead of all of them; its bracket is then checked against the
-    // selected r* (the selection is exact iff s_lo + band <= r* < s_hi - band:
+    // selected r* (the selection is exact when s_lo + band <= r* < s_hi - band:
     // every dropped pair lies below r* and every sure pair above it), and a
     // subject whose sampled bracket missed runs both passes again in full.
     for (int round = 0; round < 2; round++) {
