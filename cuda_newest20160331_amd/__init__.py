@@ -41,7 +41,7 @@ EXPORTS = [
     "bna_group_set_exact", "bna_group_band_pairs", "bna_lp", "bna_bc", "bna_lp_weighted",
     "bna_group_build_csr", "bna_write_cormat", "bna_nm_to_nifti", "bna_corr_packed", "bna_select_density",
     "bna_build_csr_into", "bna_set_graphs", "bna_batch_graph_stats", "bna_eigcent_noalloc",
-    "bna_modularity", "bna_check_csr",
+    "bna_modularity", "bna_check_csr", "bna_maslov",
 ]
 
 
@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
         "bna_modularity": (i32, [vp, C.POINTER(CSR), C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int)]),
         "bna_check_csr": (i32, [C.POINTER(CSR), i32]),
+        "bna_maslov": (i32, [C.POINTER(CSR), i32, C.c_uint64, C.POINTER(CSR)]),
         "bna_density_to_k": (i64, [i32, C.c_float]),
         "bna_synth_factor_model": (i32, [fp, i32, i32, i32, dbl, C.c_uint64, i32]),
         "bna_write_csr": (i32, [C.c_char_p, C.POINTER(CSR)]),
@@ -217,6 +218,20 @@ def check_csr(g: "Graph", ascending: bool = True) -> int:
     """bna_check_csr on a host Graph: 0 (BNA_OK) or the error status."""
     h = _host_csr(g)
     return lib().bna_check_csr(C.byref(h[0]), int(ascending))
+
+
+def maslov(g: "Graph", seed: int, permute_weights: bool = False) -> "Graph":
+    """A degree-preserving random network of a host Graph (bna_maslov): the
+    Maslov rewiring the Cp and Lp tools compare against."""
+    h = _host_csr(g)
+    out = CSR()
+    rc = lib().bna_maslov(C.byref(h[0]), int(permute_weights), C.c_uint64(seed & (2**64 - 1)), C.byref(out))
+    if rc:
+        raise BnaError(rc, "bna_maslov")
+    try:
+        return Graph(out)
+    finally:
+        lib().bna_csr_free(C.byref(out))
 
 
 def read_nm(path: str) -> np.ndarray:

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,8 @@ struct Opts {
     int refine = 1;
     int compat_eps = 0;
     int binary = 0;
+    bool seeded = false;  // --seed=S: the random networks' generator seed (else the clock)
+    uint64_t seed = 0;
 };
 
 inline bool parse_opts(int &argc, char **&argv, Opts &o, std::string &err)
@@ -99,12 +102,24 @@ inline bool parse_opts(int &argc, char **&argv, Opts &o, std::string &err)
         else if (key == "no-refine") o.refine = 0;
         else if (key == "compat-eps") o.compat_eps = 1;
         else if (key == "binary") o.binary = 1;
+        else if (key == "seed") {
+            o.seeded = true;
+            o.seed = strtoull(val.c_str(), nullptr, 0);
+        }
         else { err = "unknown option " + a; return false; }
     }
     argv[k - 1] = argv[0];
     argv += k - 1;
     argc -= k - 1;
     return true;
+}
+
+// Seed of random network l of a file: --seed=S gives S + l; without it the
+// clock, as the reference's srand(time(0)) (Maslov.cpp:58), so runs differ.
+inline uint64_t net_seed(const Opts &o, int l)
+{
+    static const uint64_t clock_seed = (uint64_t)time(nullptr) * 0x9E3779B97F4A7C15ull;
+    return (o.seeded ? o.seed : clock_seed) + (uint64_t)l;
 }
 
 }  // namespace bnacli

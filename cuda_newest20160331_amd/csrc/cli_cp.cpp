@@ -1,13 +1,14 @@
 // bna_cp -- clustering coefficient for every .csr of a directory, the CUCP tool
 // (W/Cp_GPU/CUCP/Cp_main.cpp) on libbna.
 //
-//   bna_cp [--device=D] Dir [num_random_networks]
+//   bna_cp [--device=D] [--seed=S] Dir [num_random_networks]
 //
 // <stem>.cp (int32 N, float[N]) and <stem>_cp.txt (the mean |Cp|, six
 // significant digits) beside each .csr.  A binary file (no V section) is taken
-// with weight 1 on every edge.  The Maslov random-network comparison the
-// reference appends to _cp.txt (Cp_main.cpp, Maslov_weighted_1 + Cp_1) is not
-// provided: a positive num_random_networks is reported and ignored.
+// with weight 1 on every edge.  Each random network (bna_maslov with the
+// weights carried along, Maslov_weighted_1, then Cp; Cp_main.cpp:130-151) adds
+// a line with its mean Cp to _cp.txt.  Random networks are seeded from the
+// clock unless --seed is given.
 #include "cli_common.hpp"
 
 using namespace bnacli;
@@ -17,7 +18,7 @@ int main(int argc, char **argv)
     Opts o;
     std::string err;
     if (!parse_opts(argc, argv, o, err) || argc < 2) {
-        fprintf(stderr, "%s%susage: bna_cp [--device=D] dir_for_csr [num_random_networks]\n", err.c_str(),
+        fprintf(stderr, "%s%susage: bna_cp [--device=D] [--seed=S] dir_for_csr [num_random_networks]\n", err.c_str(),
                 err.empty() ? "" : "\n");
         return 1;
     }
@@ -27,8 +28,7 @@ int main(int argc, char **argv)
         fprintf(stderr, "can't open %s\n", dir.c_str());
         return 1;
     }
-    if (argc > 2 && atoi(argv[2]) > 0)
-        fprintf(stderr, "random-network (Maslov) comparison is not provided; only the network's Cp is written\n");
+    const int nrandom = argc > 2 ? std::max(0, atoi(argv[2])) : 0;
     const std::vector<std::string> files = list_files(dir, ".csr");
     printf("%zu files to be processed.\n", files.size());
     bna_ctx *ctx = nullptr;
@@ -54,10 +54,26 @@ int main(int argc, char **argv)
         const std::string out = stem(in) + ".cp";
         printf("Save Clustering Coefficient for each node as %s\n", out.c_str());
         if ((rc = bna_write_nm(out.c_str(), cp.data(), g.N))) status = die(ctx, rc, "write " + out);
+        std::vector<double> means{mean};
+        if (nrandom > 0) printf("Calculating Cp for random networks...\n");
+        for (int l = 0; l < nrandom && status == 0; l++) {
+            bna_csr r;
+            double m = 0.0;
+            if ((rc = bna_maslov(&g, 0, net_seed(o, l), &r))) {
+                status = die(ctx, rc, "random network of " + in);
+                break;
+            }
+            if ((rc = bna_cp(ctx, &r, cp.data(), &m))) status = die(ctx, rc, "cp of a random network of " + in);
+            else means.push_back(m);
+            bna_csr_free(&r);
+        }
         const std::string txt = stem(in) + "_cp.txt";
         FILE *ft = fopen(txt.c_str(), "w");
-        if (!ft || fprintf(ft, "%.6g\n", mean) < 0) status = die(ctx, BNA_ERR_IO, "write " + txt);
+        bool ok = ft != nullptr;
+        for (double m : means) ok = ok && fprintf(ft, "%.6g\n", m) >= 0;
+        if (!ok) status = die(ctx, BNA_ERR_IO, "write " + txt);
         if (ft) fclose(ft);
+        printf("Save average Cp for the brain network and random networks as %s\n", txt.c_str());
         bna_csr_free(&g);
     }
     bna_destroy(ctx);

@@ -1,12 +1,12 @@
 // bna_lp -- characteristic path length for every .csr of a directory, the
 // CUBFS_Lp tool (B/src/Lp/CUBFS_Lp/CUBFS_Lp/CUBFS_LP_main.cpp) on libbna.
 //
-//   bna_lp [--device=D] Dir [num_random_networks]
+//   bna_lp [--device=D] [--seed=S] Dir [num_random_networks]
 //
-// <stem>_Lp.txt beside each .csr: the network's Lp with six significant digits
-// (CUBFS_LP_main.cpp:117-122).  The graph is taken as binary.  The Maslov
-// random networks the reference adds one line each for are not provided: a
-// positive num_random_networks is reported and ignored.
+// <stem>_Lp.txt beside each .csr: the network's Lp with six significant digits,
+// then one line per Maslov random network (bna_maslov, CUBFS_LP_main.cpp:
+// 110-123).  The graph is taken as binary.  Random networks are seeded from the
+// clock unless --seed is given.
 #include "cli_common.hpp"
 
 using namespace bnacli;
@@ -16,7 +16,7 @@ int main(int argc, char **argv)
     Opts o;
     std::string err;
     if (!parse_opts(argc, argv, o, err) || argc < 2) {
-        fprintf(stderr, "%s%susage: bna_lp [--device=D] dir_for_csr [num_random_networks]\n", err.c_str(),
+        fprintf(stderr, "%s%susage: bna_lp [--device=D] [--seed=S] dir_for_csr [num_random_networks]\n", err.c_str(),
                 err.empty() ? "" : "\n");
         return 1;
     }
@@ -26,8 +26,7 @@ int main(int argc, char **argv)
         fprintf(stderr, "can't open %s\n", dir.c_str());
         return 1;
     }
-    if (argc > 2 && atoi(argv[2]) > 0)
-        fprintf(stderr, "random-network (Maslov) comparison is not provided; only the network's Lp is written\n");
+    const int nrandom = argc > 2 ? std::max(0, atoi(argv[2])) : 0;
     const std::vector<std::string> files = list_files(dir, ".csr");
     printf("%zu files to be processed.\n", files.size());
     bna_ctx *ctx = nullptr;
@@ -49,11 +48,26 @@ int main(int argc, char **argv)
             continue;
         }
         printf("average Lp:\t%.6g\n", lp);
+        std::vector<double> lps{lp};
+        if (nrandom > 0) printf("Calculating Lp for random networks...\n");
+        for (int l = 0; l < nrandom && status == 0; l++) {
+            bna_csr r;
+            double m = 0.0;
+            if ((rc = bna_maslov(&g, 0, net_seed(o, l), &r))) {
+                status = die(ctx, rc, "random network of " + in);
+                break;
+            }
+            if ((rc = bna_lp(ctx, &r, &m, nullptr, 0))) status = die(ctx, rc, "lp of a random network of " + in);
+            else lps.push_back(m);
+            bna_csr_free(&r);
+        }
         const std::string txt = stem(in) + "_Lp.txt";
         FILE *ft = fopen(txt.c_str(), "w");
-        if (!ft || fprintf(ft, "%.6g\n", lp) < 0) status = die(ctx, BNA_ERR_IO, "write " + txt);
+        bool ok = ft != nullptr;
+        for (double m : lps) ok = ok && fprintf(ft, "%.6g\n", m) >= 0;
+        if (!ok) status = die(ctx, BNA_ERR_IO, "write " + txt);
         if (ft) fclose(ft);
-        printf("Save Lp for the brain network as %s\n", txt.c_str());
+        printf("Save Lp for the brain network and random networks as %s\n", txt.c_str());
         bna_csr_free(&g);
     }
     bna_destroy(ctx);

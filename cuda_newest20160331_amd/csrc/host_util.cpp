@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 extern "C" {
@@ -386,3 +387,120 @@ int bna_nm_to_nifti(const char *mask_path, double mask_thr, const float *values,
 void bna_free(void *p) { free(p); }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ Maslov rewiring
+namespace {
+
+// splitmix64: the random networks' generator (any seed, every output 64 bits)
+struct SplitMix {
+    uint64_t x;
+    uint64_t next()
+    {
+        uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    // uniform in [0, n), n > 0 (the high half of a 64 x 64-bit product)
+    int64_t below(int64_t n) { return (int64_t)(((unsigned __int128)next() * (uint64_t)n) >> 64); }
+};
+
+struct MasEdge {
+    int32_t a, b;  // the endpoints (either order)
+    float v;
+};
+
+}  // namespace
+
+// Degree-preserving rewiring by double edge swaps, the reference's Rewire
+// (W/Cp_GPU/CUCP/Maslov.cpp:49-98): 2 M attempts over the M undirected edges;
+// an attempt draws two edges a-b and c-d, gives up when they share a vertex,
+// and otherwise replaces them by a-c, b-d or by a-d, c-b (one of the two at
+// random) unless one of the new edges exists already.  Edge membership is a
+// hash set of vertex pairs instead of the reference's dense N x N bool matrix.
+extern "C" int bna_maslov(const bna_csr *src, int permute_weights, uint64_t seed, bna_csr *out)
+{
+    if (!src || !out || src->on_device || src->N < 0 || !src->R || (src->nnz > 0 && !src->C)) return BNA_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    int rc = bna_check_csr(src, 1);
+    if (rc) return rc;
+    const int N = src->N;
+    const int64_t nnz = src->nnz;
+    auto key = [N](int64_t p, int64_t q) -> uint64_t {
+        return p < q ? (uint64_t)p * (uint64_t)N + (uint64_t)q : (uint64_t)q * (uint64_t)N + (uint64_t)p;
+    };
+    // the upper triangle as an edge list (weights along, Maslov.cpp:117-128)
+    std::vector<MasEdge> E;
+    E.reserve((size_t)(nnz / 2));
+    std::unordered_set<uint64_t> S;
+    S.reserve((size_t)(nnz / 2) * 2 + 16);
+    for (int i = 0; i < N; i++)
+        for (int64_t k = src->R[i]; k < src->R[i + 1]; k++) {
+            const int j = src->C[k];
+            if (j == i) return BNA_ERR_FORMAT;  // a self-loop has no swap partner semantics
+            if (j > i) {
+                E.push_back(MasEdge{i, j, src->V ? src->V[k] : 1.0f});
+                S.insert(key(i, j));
+            }
+        }
+    const int64_t M = (int64_t)E.size();
+    if (2 * M != nnz) return BNA_ERR_FORMAT;  // not symmetric
+    for (int i = 0; i < N; i++)
+        for (int64_t k = src->R[i]; k < src->R[i + 1]; k++)
+            if (src->C[k] < i && !S.count(key(i, src->C[k]))) return BNA_ERR_FORMAT;
+    SplitMix rng{seed};
+    if (M >= 2)
+        for (int64_t t = 0; t < 2 * M; t++) {
+            const int64_t e = rng.below(M), f = rng.below(M);
+            const int v1 = E[e].a, v2 = E[e].b, v3 = E[f].a, v4 = E[f].b;
+            if (v1 == v3 || v1 == v4 || v2 == v3 || v2 == v4) continue;
+            const bool cross = rng.next() >> 63;
+            const int p = cross ? v3 : v4, q = cross ? v4 : v3;  // new edges v1-p and v2-q
+            if (S.count(key(v1, p)) || S.count(key(v2, q))) continue;
+            S.erase(key(v1, v2));
+            S.erase(key(v3, v4));
+            S.insert(key(v1, p));
+            S.insert(key(v2, q));
+            E[e] = MasEdge{v1, p, E[e].v};
+            E[f] = MasEdge{v2, q, E[f].v};
+        }
+    // Maslov_weighted_2 then permutes the weights over the edges (randperm,
+    // Maslov.cpp:18-30)
+    if (permute_weights)
+        for (int64_t i = M - 1; i > 0; i--) std::swap(E[(size_t)rng.below(i + 1)].v, E[(size_t)i].v);
+    // the CSR: R unchanged (every degree kept), each edge in both rows with its
+    // weight, columns ascending (Maslov.cpp:161-174 scan the bool matrix row by
+    // row and hand each entry its edge's weight)
+    out->R = (int32_t *)malloc(4 * (size_t)(N + 1));
+    out->C = (int32_t *)malloc(4 * (size_t)std::max<int64_t>(nnz, 1));
+    out->V = src->V ? (float *)malloc(4 * (size_t)std::max<int64_t>(nnz, 1)) : nullptr;
+    if (!out->R || !out->C || (src->V && !out->V)) {
+        bna_csr_free(out);
+        memset(out, 0, sizeof(*out));
+        return BNA_ERR_NOMEM;
+    }
+    memcpy(out->R, src->R, 4 * (size_t)(N + 1));
+    std::vector<int64_t> pos(src->R, src->R + N);
+    std::vector<std::pair<int32_t, float>> ent((size_t)nnz);
+    for (const MasEdge &x : E) {
+        ent[(size_t)pos[x.a]++] = {x.b, x.v};
+        ent[(size_t)pos[x.b]++] = {x.a, x.v};
+    }
+    for (int i = 0; i < N; i++) {
+        if (pos[i] != src->R[i + 1]) {  // cannot happen: swaps keep every degree
+            bna_csr_free(out);
+            memset(out, 0, sizeof(*out));
+            return BNA_ERR_FORMAT;
+        }
+        std::sort(ent.begin() + src->R[i], ent.begin() + src->R[i + 1],
+                  [](const std::pair<int32_t, float> &p, const std::pair<int32_t, float> &q) { return p.first < q.first; });
+    }
+    for (int64_t k = 0; k < nnz; k++) {
+        out->C[k] = ent[(size_t)k].first;
+        if (out->V) out->V[k] = ent[(size_t)k].second;
+    }
+    out->N = N;
+    out->nnz = nnz;
+    out->on_device = 0;
+    return BNA_OK;
+}

@@ -360,6 +360,18 @@ int bna_read_csr(const char *path, bna_csr *out);
  * row (what FormCSRGraph writes, CorMat.cpp:889-918).  BNA_OK or
  * BNA_ERR_FORMAT; BNA_ERR_ARG for a device CSR.  Needs no device. */
 int bna_check_csr(const bna_csr *g, int ascending);
+/* Degree-preserving random network of a host CSR, the Maslov rewiring behind the
+ * Cp and Lp tools' random-network comparisons (W/Cp_GPU/CUCP/Maslov.cpp:49-289,
+ * W/Maslov/Maslov/Maslov.cpp; B/.../CUBFS_LP_main.cpp:110-116 for binary
+ * graphs): 2 x edges attempts at a double edge swap (a-b, c-d -> a-c, b-d or
+ * a-d, c-b, refused when the vertices are not distinct or a new edge exists).
+ * Weights travel with their edges (Maslov_weighted_1); permute_weights != 0
+ * then shuffles them over the edges (Maslov_weighted_2).  R is unchanged and
+ * each row's columns ascend.  The reference seeds rand() from the clock; seed
+ * drives a splitmix64 generator here, so a network can be made again.  out gets
+ * host arrays (bna_csr_free).  The input must be symmetric without self-loops,
+ * columns ascending (BNA_ERR_FORMAT otherwise).  Needs no device. */
+int bna_maslov(const bna_csr *src, int permute_weights, uint64_t seed, bna_csr *out);
 /* .nm node maps (Degree.cpp:95-99, weighted-net.cu:305-322): int32 N, float[N]. */
 int bna_write_nm(const char *path, const float *v, int N);
 /* A vertex map (N floats, e.g. a _deg.nm or _ec.nm payload) as a NIfTI volume

@@ -257,3 +257,45 @@ def test_lpw_tool(subjects, tmp_path):
     lp = float(Path(str(f)[:-4] + "_Lp.txt").read_text())
     assert abs(lp - olp) <= 1e-5 * abs(olp)  # six significant digits
     assert np.array_equal(bna.read_nm(str(f)[:-4] + "_eff.nm"), oeff)
+
+
+def test_random_network_tools(subjects, tmp_path):
+    # With --seed the tools' Maslov networks are bna_maslov(g, seed + l), so the
+    # lines they add can be recomputed here: Cp, binary Lp and weighted Lp of the
+    # same random networks, and lambda = Lp / mean of the random networks' Lp
+    # (Cp_main.cpp:130-151, CUBFS_LP_main.cpp:110-123, CUBFW_Lp_main.cpp:219-253)
+    d, mask, Ls = subjects
+    say("cli: random-network lines of the Cp and Lp tools")
+    wf = sorted((d / "weighted").glob("*.csr")) if (d / "weighted").exists() else []
+    uf = sorted((d / "unweighted").glob("*.csr")) if (d / "unweighted").exists() else []
+    if not wf or not uf:
+        pytest.skip("construction outputs missing")
+    w = tmp_path / "w"
+    u = tmp_path / "u"
+    w.mkdir()
+    u.mkdir()
+    fw = w / wf[0].name
+    fw.write_bytes(wf[0].read_bytes())
+    fu = u / uf[0].name
+    fu.write_bytes(uf[0].read_bytes())
+    gw, gu = bna.read_csr(fw), bna.read_csr(fu)
+
+    run("bna_cp", "--seed=5", w, 2)
+    lines = [float(v) for v in Path(str(fw)[:-4] + "_cp.txt").read_text().split()]
+    want = [orc.cp(g.R.astype(np.int64), g.C, g.V, g.N)[1] for g in (gw, bna.maslov(gw, 5), bna.maslov(gw, 6))]
+    assert len(lines) == 3
+    assert all(abs(a - b) <= 1e-5 * abs(b) for a, b in zip(lines, want)), (lines, want)
+
+    run("bna_lp", "--seed=3", u, 2)
+    lines = [float(v) for v in Path(str(fu)[:-4] + "_Lp.txt").read_text().split()]
+    want = [orc.lp(g.R.astype(np.int64), g.C, g.N)[0] for g in (gu, bna.maslov(gu, 3), bna.maslov(gu, 4))]
+    assert len(lines) == 3
+    assert all(abs(a - b) <= 1e-5 * abs(b) for a, b in zip(lines, want)), (lines, want)
+
+    run("bna_lpw", "--seed=9", w, 1, "gl")
+    lines = [float(v) for v in Path(str(fw)[:-4] + "_Lp.txt").read_text().split()]
+    want = [orc.lpw(g.R.astype(np.int64), g.C, g.V, g.N)[0] for g in (gw, bna.maslov(gw, 9))]
+    assert len(lines) == 2
+    assert all(abs(a - b) <= 1e-5 * abs(b) for a, b in zip(lines, want)), (lines, want)
+    lam = float(Path(str(fw)[:-4] + "_lambda.txt").read_text())
+    assert abs(lam - want[0] / want[1]) <= 1e-5 * abs(want[0] / want[1])
